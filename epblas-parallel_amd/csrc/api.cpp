@@ -1,0 +1,576 @@
+// api.cpp -- the C ABI of libmwgemm_hip.so: the reference's Fortran-ABI GEMM entry points
+// (mgemm_/mgemm_64_/wgemm_/wgemm_64_/mgemm_serial/wgemm_serial), host<->device staging,
+// the numeric-path switch, error reporting and kernel timers.
+//
+// Semantics follow the reference driver (src/epblas-parallel/multifloats/
+// mgemm_parallel.cpp:44-187, wgemm_parallel.cpp:44-194):
+//   trans letters upper-cased, real 'C' folded to 'T'       (common/blas_char.h:33-39)
+//   m <= 0 or n <= 0: return, C untouched                  (mgemm_parallel.cpp:66)
+//   beta pre-pass: beta == 0 -> exact zeros (NaN cleared), beta == 1 -> skip,
+//                  else C*beta                              (:68-77)
+//   alpha == 0 or k == 0: only the pre-pass                 (:78)
+// Deviations, all toward failing loudly instead of silently: invalid trans letters or
+// leading dimensions are rejected (the reference does not check them), and a device
+// failure is reported and aborts (or, with MWGEMM_ON_ERROR=return, leaves C unmodified and
+// is readable through mwgemm_last_error) where the reference's serial path would silently
+// leave C = beta*C (mgemm_serial.cpp:353-365).
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cctype>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mwgemm_hip.h"
+#include "mw_internal.hpp"
+#include "ozaki.hpp"
+#include "timer.hpp"
+
+namespace mw {
+
+// ----------------------------------------------------------------------------------
+// Errors.
+// ----------------------------------------------------------------------------------
+namespace {
+std::mutex g_err_mu;
+int g_err_code = 0;
+std::string g_err_msg;
+std::atomic<int> g_abort{-1};   // -1: read MWGEMM_ON_ERROR on first use
+
+bool abort_on_error()
+{
+    int a = g_abort.load();
+    if (a < 0) {
+        const char *e = getenv("MWGEMM_ON_ERROR");
+        a = (e && strcmp(e, "return") == 0) ? 0 : 1;
+        g_abort.store(a);
+    }
+    return a != 0;
+}
+}  // namespace
+
+int record_error(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    {
+        std::lock_guard<std::mutex> lk(g_err_mu);
+        g_err_code = code ? code : -1;
+        g_err_msg = buf;
+    }
+    fprintf(stderr, "libmwgemm_hip: error %d: %s\n", code, buf);
+    fflush(stderr);
+    return code ? code : -1;
+}
+
+#define MW_TRY(expr)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return record_error((int)e_, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+// ----------------------------------------------------------------------------------
+// Per-device state: one stream and a grow-only staging/workspace arena.
+// ----------------------------------------------------------------------------------
+namespace {
+struct DevState {
+    hipStream_t stream = nullptr;
+    void *stage = nullptr;   // host-pointer staging (A | B | C)
+    size_t stage_bytes = 0;
+};
+std::mutex g_state_mu;
+std::vector<DevState> g_dev;
+std::mutex g_call_mu;        // host-pointer entry points are serialised
+std::atomic<int> g_path{-1};
+
+int current_device(int *dev)
+{
+    MW_TRY(hipGetDevice(dev));
+    return 0;
+}
+
+DevState *dev_state(int dev)
+{
+    std::lock_guard<std::mutex> lk(g_state_mu);
+    if ((int)g_dev.size() <= dev) g_dev.resize(dev + 1);
+    return &g_dev[dev];
+}
+
+int ensure_stream(DevState *st)
+{
+    if (!st->stream) MW_TRY(hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking));
+    return 0;
+}
+
+int ensure_stage(DevState *st, size_t bytes)
+{
+    if (st->stage_bytes >= bytes) return 0;
+    if (st->stage) {
+        MW_TRY(hipStreamSynchronize(st->stream));
+        MW_TRY(hipFree(st->stage));
+        st->stage = nullptr;
+        st->stage_bytes = 0;
+    }
+    MW_TRY(hipMalloc(&st->stage, bytes));
+    st->stage_bytes = bytes;
+    return 0;
+}
+
+int default_path()
+{
+    int p = g_path.load();
+    if (p < 0) {
+        const char *e = getenv("MWGEMM_PATH");
+        p = MWGEMM_PATH_PLAIN;
+        if (e && strcmp(e, "ozaki") == 0) p = MWGEMM_PATH_OZAKI;
+        if (e && strcmp(e, "auto") == 0) p = MWGEMM_PATH_AUTO;
+        g_path.store(p);
+    }
+    return p;
+}
+
+// Letters: returns OP_N/OP_T/OP_C or -1.
+int op_real(char c)
+{
+    c = (char)toupper((unsigned char)c);
+    if (c == 'N') return OP_N;
+    if (c == 'T' || c == 'C') return OP_T;
+    return -1;
+}
+int op_cplx(char c)
+{
+    c = (char)toupper((unsigned char)c);
+    if (c == 'N') return OP_N;
+    if (c == 'T') return OP_T;
+    if (c == 'C') return OP_C;
+    return -1;
+}
+
+bool dd_is0(const ddv &x) { return x.hi == 0.0 && x.lo == 0.0; }
+bool dd_is1(const ddv &x) { return x.hi == 1.0 && x.lo == 0.0; }
+bool zdd_is0(const zddv &x) { return dd_is0(x.re) && dd_is0(x.im); }
+bool zdd_is1(const zddv &x) { return dd_is1(x.re) && dd_is0(x.im); }
+
+// Elements spanned by a column-major rows x cols matrix with leading dimension ld.
+int64_t extent(int64_t rows, int64_t cols, int64_t ld)
+{
+    if (rows <= 0 || cols <= 0) return 0;
+    return (cols - 1) * ld + rows;
+}
+
+bool is_device_ptr(const void *p)
+{
+    if (!p) return false;
+    hipPointerAttribute_t attr;
+    hipError_t e = hipPointerGetAttributes(&attr, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged ||
+           attr.type == hipMemoryTypeUnified;
+}
+
+int choose_path(int path, int64_t m, int64_t n, int64_t k)
+{
+    if (path == MWGEMM_PATH_AUTO)
+        return ozaki_supported(k) && (double)m * (double)n * (double)k >= 1.0e9
+                   ? MWGEMM_PATH_OZAKI : MWGEMM_PATH_PLAIN;
+    if (path == MWGEMM_PATH_OZAKI && !ozaki_supported(k)) return MWGEMM_PATH_PLAIN;
+    return path;
+}
+}  // namespace
+
+// ----------------------------------------------------------------------------------
+// Device-resident cores.
+// ----------------------------------------------------------------------------------
+int gemm_real_device(GemmArgsR g, int path, hipStream_t s)
+{
+    if (g.m <= 0 || g.n <= 0) return 0;
+    if (g.opa < 0 || g.opb < 0) return record_error(-2, "mgemm: invalid trans letter");
+    const int64_t arows = g.opa == OP_N ? g.m : g.k;
+    const int64_t brows = g.opb == OP_N ? g.k : g.n;
+    const bool need_ab = g.k > 0 && !dd_is0(g.alpha);
+    if (g.ldc < g.m || (need_ab && (g.lda < (arows > 1 ? arows : 1) || g.ldb < (brows > 1 ? brows : 1))))
+        return record_error(-3, "mgemm: leading dimension too small (lda=%lld ldb=%lld ldc=%lld)",
+                            (long long)g.lda, (long long)g.ldb, (long long)g.ldc);
+    g.beta_mode = dd_is0(g.beta) ? BETA_ZERO : dd_is1(g.beta) ? BETA_ONE : BETA_SCALE;
+    if (!need_ab) {
+        MW_TRY(plain_beta_only_real(g, s));
+        return 0;
+    }
+    if (choose_path(path, g.m, g.n, g.k) == MWGEMM_PATH_OZAKI) return ozaki_gemm_real(g, s);
+    MW_TRY(plain_gemm_real(g, s));
+    return 0;
+}
+
+int gemm_cplx_device(GemmArgsZ g, int path, hipStream_t s)
+{
+    if (g.m <= 0 || g.n <= 0) return 0;
+    if (g.opa < 0 || g.opb < 0) return record_error(-2, "wgemm: invalid trans letter");
+    const int64_t arows = g.opa == OP_N ? g.m : g.k;
+    const int64_t brows = g.opb == OP_N ? g.k : g.n;
+    const bool need_ab = g.k > 0 && !zdd_is0(g.alpha);
+    if (g.ldc < g.m || (need_ab && (g.lda < (arows > 1 ? arows : 1) || g.ldb < (brows > 1 ? brows : 1))))
+        return record_error(-3, "wgemm: leading dimension too small (lda=%lld ldb=%lld ldc=%lld)",
+                            (long long)g.lda, (long long)g.ldb, (long long)g.ldc);
+    g.beta_mode = zdd_is0(g.beta) ? BETA_ZERO : zdd_is1(g.beta) ? BETA_ONE : BETA_SCALE;
+    if (!need_ab) {
+        MW_TRY(plain_beta_only_cplx(g, s));
+        return 0;
+    }
+    if (choose_path(path, g.m, g.n, g.k) == MWGEMM_PATH_OZAKI) return ozaki_gemm_cplx(g, s);
+    MW_TRY(plain_gemm_cplx(g, s));
+    return 0;
+}
+
+// ----------------------------------------------------------------------------------
+// Host-pointer staging shared by the real and complex drop-in entry points.
+// ----------------------------------------------------------------------------------
+namespace {
+template <typename T, typename Args, typename Run>
+int staged_call(Args g, int64_t a_rows, int64_t a_cols, int64_t b_rows, int64_t b_cols,
+                bool beta_zero, Run run)
+{
+    const int64_t a_elems = extent(a_rows, a_cols, g.lda);
+    const int64_t b_elems = extent(b_rows, b_cols, g.ldb);
+    const int64_t c_elems = extent(g.m, g.n, g.ldc);
+    if (g.m <= 0 || g.n <= 0) return 0;
+    std::lock_guard<std::mutex> call_lk(g_call_mu);
+    int dev = 0;
+    if (int rc = current_device(&dev)) return rc;
+    DevState *st = dev_state(dev);
+    if (int rc = ensure_stream(st)) return rc;
+    hipStream_t s = st->stream;
+
+    const T *hA = reinterpret_cast<const T *>(g.A);
+    const T *hB = reinterpret_cast<const T *>(g.B);
+    T *hC = reinterpret_cast<T *>(g.C);
+    const bool dA = a_elems == 0 || is_device_ptr(hA);
+    const bool dB = b_elems == 0 || is_device_ptr(hB);
+    const bool dC = is_device_ptr(hC);
+    size_t need = 0;
+    auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+    size_t offA = need; if (!dA) need += align(a_elems * sizeof(T));
+    size_t offB = need; if (!dB) need += align(b_elems * sizeof(T));
+    size_t offC = need; if (!dC) need += align(c_elems * sizeof(T));
+    if (need) {
+        if (int rc = ensure_stage(st, need)) return rc;
+    }
+    char *base = static_cast<char *>(st->stage);
+    // Only the logical rows x cols of each operand travel; the device copies keep the
+    // caller's leading dimensions, and C's padding rows are never written back.
+    if (!dA) {
+        g.A = reinterpret_cast<decltype(g.A)>(base + offA);
+        MW_TRY(hipMemcpy2DAsync(base + offA, g.lda * sizeof(T), hA, g.lda * sizeof(T),
+                                a_rows * sizeof(T), a_cols, hipMemcpyHostToDevice, s));
+    }
+    if (!dB) {
+        g.B = reinterpret_cast<decltype(g.B)>(base + offB);
+        MW_TRY(hipMemcpy2DAsync(base + offB, g.ldb * sizeof(T), hB, g.ldb * sizeof(T),
+                                b_rows * sizeof(T), b_cols, hipMemcpyHostToDevice, s));
+    }
+    if (!dC) {
+        g.C = reinterpret_cast<decltype(g.C)>(base + offC);
+        if (!beta_zero)
+            MW_TRY(hipMemcpy2DAsync(base + offC, g.ldc * sizeof(T), hC, g.ldc * sizeof(T),
+                                    g.m * sizeof(T), g.n, hipMemcpyHostToDevice, s));
+    }
+    if (int rc = run(g, s)) {
+        (void)hipStreamSynchronize(s);
+        return rc;
+    }
+    if (!dC)
+        MW_TRY(hipMemcpy2DAsync(hC, g.ldc * sizeof(T), base + offC, g.ldc * sizeof(T),
+                                g.m * sizeof(T), g.n, hipMemcpyDeviceToHost, s));
+    MW_TRY(hipStreamSynchronize(s));
+    return 0;
+}
+
+int mgemm_host(char transa, char transb, int64_t m, int64_t n, int64_t k,
+               const mw_float64x2 *alpha, const mw_float64x2 *a, int64_t lda,
+               const mw_float64x2 *b, int64_t ldb, const mw_float64x2 *beta, mw_float64x2 *c,
+               int64_t ldc)
+{
+    if (m <= 0 || n <= 0) return 0;
+    GemmArgsR g{};
+    g.m = m; g.n = n; g.k = k;
+    g.opa = op_real(transa); g.opb = op_real(transb);
+    g.alpha = ddv{alpha->limbs[0], alpha->limbs[1]};
+    g.beta = ddv{beta->limbs[0], beta->limbs[1]};
+    g.A = reinterpret_cast<const ddv *>(a); g.lda = lda;
+    g.B = reinterpret_cast<const ddv *>(b); g.ldb = ldb;
+    g.C = reinterpret_cast<ddv *>(c); g.ldc = ldc;
+    if (g.opa < 0 || g.opb < 0) return record_error(-2, "mgemm: invalid trans letter");
+    const bool need_ab = k > 0 && !dd_is0(g.alpha);
+    const int64_t ar = need_ab ? (g.opa == OP_N ? m : k) : 0, ac = g.opa == OP_N ? k : m;
+    const int64_t br = need_ab ? (g.opb == OP_N ? k : n) : 0, bc = g.opb == OP_N ? n : k;
+    if (ldc < m || (need_ab && (lda < ar || ldb < br)))
+        return record_error(-3, "mgemm: leading dimension too small (lda=%lld ldb=%lld ldc=%lld)",
+                            (long long)lda, (long long)ldb, (long long)ldc);
+    const int path = default_path();
+    return staged_call<ddv>(g, ar, ac, br, bc, dd_is0(g.beta),
+                            [path](GemmArgsR gg, hipStream_t s) { return gemm_real_device(gg, path, s); });
+}
+
+int wgemm_host(char transa, char transb, int64_t m, int64_t n, int64_t k,
+               const mw_complex64x2 *alpha, const mw_complex64x2 *a, int64_t lda,
+               const mw_complex64x2 *b, int64_t ldb, const mw_complex64x2 *beta,
+               mw_complex64x2 *c, int64_t ldc)
+{
+    if (m <= 0 || n <= 0) return 0;
+    GemmArgsZ g{};
+    g.m = m; g.n = n; g.k = k;
+    g.opa = op_cplx(transa); g.opb = op_cplx(transb);
+    g.alpha = zddv{{alpha->re.limbs[0], alpha->re.limbs[1]}, {alpha->im.limbs[0], alpha->im.limbs[1]}};
+    g.beta = zddv{{beta->re.limbs[0], beta->re.limbs[1]}, {beta->im.limbs[0], beta->im.limbs[1]}};
+    g.A = reinterpret_cast<const zddv *>(a); g.lda = lda;
+    g.B = reinterpret_cast<const zddv *>(b); g.ldb = ldb;
+    g.C = reinterpret_cast<zddv *>(c); g.ldc = ldc;
+    if (g.opa < 0 || g.opb < 0) return record_error(-2, "wgemm: invalid trans letter");
+    const bool need_ab = k > 0 && !zdd_is0(g.alpha);
+    const int64_t ar = need_ab ? (g.opa == OP_N ? m : k) : 0, ac = g.opa == OP_N ? k : m;
+    const int64_t br = need_ab ? (g.opb == OP_N ? k : n) : 0, bc = g.opb == OP_N ? n : k;
+    if (ldc < m || (need_ab && (lda < ar || ldb < br)))
+        return record_error(-3, "wgemm: leading dimension too small (lda=%lld ldb=%lld ldc=%lld)",
+                            (long long)lda, (long long)ldb, (long long)ldc);
+    const int path = default_path();
+    return staged_call<zddv>(g, ar, ac, br, bc, zdd_is0(g.beta),
+                             [path](GemmArgsZ gg, hipStream_t s) { return gemm_cplx_device(gg, path, s); });
+}
+
+void finish(int rc)
+{
+    if (rc != 0 && abort_on_error()) {
+        fprintf(stderr, "libmwgemm_hip: aborting (set MWGEMM_ON_ERROR=return to get the error back instead)\n");
+        abort();
+    }
+}
+}  // namespace
+
+// ----------------------------------------------------------------------------------
+// Kernel timers.
+// ----------------------------------------------------------------------------------
+namespace {
+struct TimerSlot { int cls; hipEvent_t a, b; };
+std::mutex g_timer_mu;
+std::atomic<bool> g_timer_on{false};
+std::vector<TimerSlot> g_slots;
+double g_timer_ms[KC_COUNT];
+int64_t g_timer_cnt[KC_COUNT];
+const char *kClassNames[KC_COUNT] = {
+    "plain_real", "plain_cplx", "beta_only", "oz_exponent", "oz_split", "oz_gemm", "oz_crt",
+    "batch_plain", "batch_oz_split", "batch_oz_gemm", "batch_oz_crt"};
+
+void drain_slots()
+{
+    for (auto &sl : g_slots) {
+        float ms = 0.f;
+        if (hipEventSynchronize(sl.b) == hipSuccess && hipEventElapsedTime(&ms, sl.a, sl.b) == hipSuccess) {
+            g_timer_ms[sl.cls] += ms;
+            g_timer_cnt[sl.cls] += 1;
+        }
+        (void)hipEventDestroy(sl.a);
+        (void)hipEventDestroy(sl.b);
+    }
+    g_slots.clear();
+}
+}  // namespace
+
+int timer_begin(int cls, hipStream_t s)
+{
+    if (!g_timer_on.load(std::memory_order_relaxed)) return -1;
+    std::lock_guard<std::mutex> lk(g_timer_mu);
+    TimerSlot sl{cls, nullptr, nullptr};
+    if (hipEventCreate(&sl.a) != hipSuccess || hipEventCreate(&sl.b) != hipSuccess) return -1;
+    if (hipEventRecord(sl.a, s) != hipSuccess) return -1;
+    g_slots.push_back(sl);
+    return (int)g_slots.size() - 1;
+}
+
+void timer_end(int id, hipStream_t s)
+{
+    if (id < 0) return;
+    std::lock_guard<std::mutex> lk(g_timer_mu);
+    if (id < (int)g_slots.size()) (void)hipEventRecord(g_slots[id].b, s);
+}
+
+}  // namespace mw
+
+using namespace mw;
+
+// ----------------------------------------------------------------------------------
+// Exported C ABI.
+// ----------------------------------------------------------------------------------
+extern "C" {
+
+void mgemm_(const char *transa, const char *transb, const int *m, const int *n, const int *k,
+            const mw_float64x2 *alpha, const mw_float64x2 *a, const int *lda,
+            const mw_float64x2 *b, const int *ldb, const mw_float64x2 *beta, mw_float64x2 *c,
+            const int *ldc)
+{
+    finish(mgemm_host(*transa, *transb, *m, *n, *k, alpha, a, *lda, b, *ldb, beta, c, *ldc));
+}
+
+void mgemm_64_(const char *transa, const char *transb, const int64_t *m, const int64_t *n,
+               const int64_t *k, const mw_float64x2 *alpha, const mw_float64x2 *a,
+               const int64_t *lda, const mw_float64x2 *b, const int64_t *ldb,
+               const mw_float64x2 *beta, mw_float64x2 *c, const int64_t *ldc)
+{
+    finish(mgemm_host(*transa, *transb, *m, *n, *k, alpha, a, *lda, b, *ldb, beta, c, *ldc));
+}
+
+void wgemm_(const char *transa, const char *transb, const int *m, const int *n, const int *k,
+            const mw_complex64x2 *alpha, const mw_complex64x2 *a, const int *lda,
+            const mw_complex64x2 *b, const int *ldb, const mw_complex64x2 *beta,
+            mw_complex64x2 *c, const int *ldc)
+{
+    finish(wgemm_host(*transa, *transb, *m, *n, *k, alpha, a, *lda, b, *ldb, beta, c, *ldc));
+}
+
+void wgemm_64_(const char *transa, const char *transb, const int64_t *m, const int64_t *n,
+               const int64_t *k, const mw_complex64x2 *alpha, const mw_complex64x2 *a,
+               const int64_t *lda, const mw_complex64x2 *b, const int64_t *ldb,
+               const mw_complex64x2 *beta, mw_complex64x2 *c, const int64_t *ldc)
+{
+    finish(wgemm_host(*transa, *transb, *m, *n, *k, alpha, a, *lda, b, *ldb, beta, c, *ldc));
+}
+
+void mgemm_serial(char transa, char transb, ptrdiff_t m, ptrdiff_t n, ptrdiff_t k,
+                  const mw_float64x2 *alpha, const mw_float64x2 *a, ptrdiff_t lda,
+                  const mw_float64x2 *b, ptrdiff_t ldb, const mw_float64x2 *beta,
+                  mw_float64x2 *c, ptrdiff_t ldc)
+{
+    finish(mgemm_host(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc));
+}
+
+void wgemm_serial(char transa, char transb, ptrdiff_t m, ptrdiff_t n, ptrdiff_t k,
+                  const mw_complex64x2 *alpha, const mw_complex64x2 *a, ptrdiff_t lda,
+                  const mw_complex64x2 *b, ptrdiff_t ldb, const mw_complex64x2 *beta,
+                  mw_complex64x2 *c, ptrdiff_t ldc)
+{
+    finish(wgemm_host(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc));
+}
+
+int mwgemm_set_path(int path)
+{
+    if (path < MWGEMM_PATH_PLAIN || path > MWGEMM_PATH_AUTO) return -1;
+    int prev = default_path();
+    g_path.store(path);
+    return prev;
+}
+
+int mwgemm_get_path(void) { return default_path(); }
+
+int mwgemm_last_error(char *buf, size_t len)
+{
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    if (buf && len) {
+        strncpy(buf, g_err_msg.c_str(), len - 1);
+        buf[len - 1] = '\0';
+    }
+    return g_err_code;
+}
+
+void mwgemm_clear_error(void)
+{
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    g_err_code = 0;
+    g_err_msg.clear();
+}
+
+int mwgemm_set_abort(int on)
+{
+    int prev = abort_on_error() ? 1 : 0;
+    g_abort.store(on ? 1 : 0);
+    return prev;
+}
+
+int mwgemm_mgemm_device(char transa, char transb, int64_t m, int64_t n, int64_t k,
+                        const mw_float64x2 *alpha, const mw_float64x2 *dA, int64_t lda,
+                        const mw_float64x2 *dB, int64_t ldb, const mw_float64x2 *beta,
+                        mw_float64x2 *dC, int64_t ldc, int path, void *stream)
+{
+    GemmArgsR g{};
+    g.m = m; g.n = n; g.k = k;
+    g.opa = op_real(transa); g.opb = op_real(transb);
+    g.alpha = ddv{alpha->limbs[0], alpha->limbs[1]};
+    g.beta = ddv{beta->limbs[0], beta->limbs[1]};
+    g.A = reinterpret_cast<const ddv *>(dA); g.lda = lda;
+    g.B = reinterpret_cast<const ddv *>(dB); g.ldb = ldb;
+    g.C = reinterpret_cast<ddv *>(dC); g.ldc = ldc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (!s) {
+        int dev = 0;
+        if (int rc = current_device(&dev)) return rc;
+        DevState *st = dev_state(dev);
+        if (int rc = ensure_stream(st)) return rc;
+        s = st->stream;
+    }
+    return gemm_real_device(g, path, s);
+}
+
+int mwgemm_wgemm_device(char transa, char transb, int64_t m, int64_t n, int64_t k,
+                        const mw_complex64x2 *alpha, const mw_complex64x2 *dA, int64_t lda,
+                        const mw_complex64x2 *dB, int64_t ldb, const mw_complex64x2 *beta,
+                        mw_complex64x2 *dC, int64_t ldc, int path, void *stream)
+{
+    GemmArgsZ g{};
+    g.m = m; g.n = n; g.k = k;
+    g.opa = op_cplx(transa); g.opb = op_cplx(transb);
+    g.alpha = zddv{{alpha->re.limbs[0], alpha->re.limbs[1]}, {alpha->im.limbs[0], alpha->im.limbs[1]}};
+    g.beta = zddv{{beta->re.limbs[0], beta->re.limbs[1]}, {beta->im.limbs[0], beta->im.limbs[1]}};
+    g.A = reinterpret_cast<const zddv *>(dA); g.lda = lda;
+    g.B = reinterpret_cast<const zddv *>(dB); g.ldb = ldb;
+    g.C = reinterpret_cast<zddv *>(dC); g.ldc = ldc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (!s) {
+        int dev = 0;
+        if (int rc = current_device(&dev)) return rc;
+        DevState *st = dev_state(dev);
+        if (int rc = ensure_stream(st)) return rc;
+        s = st->stream;
+    }
+    return gemm_cplx_device(g, path, s);
+}
+
+int mwgemm_timer_enable(int on)
+{
+    bool prev = g_timer_on.exchange(on != 0);
+    return prev ? 1 : 0;
+}
+
+int mwgemm_timer_reset(void)
+{
+    std::lock_guard<std::mutex> lk(g_timer_mu);
+    drain_slots();
+    for (int i = 0; i < KC_COUNT; ++i) { g_timer_ms[i] = 0.0; g_timer_cnt[i] = 0; }
+    return 0;
+}
+
+int mwgemm_timer_read(double *ms, int64_t *count, int cap, char *names, size_t names_len)
+{
+    std::lock_guard<std::mutex> lk(g_timer_mu);
+    drain_slots();
+    size_t pos = 0;
+    for (int i = 0; i < KC_COUNT && i < cap; ++i) {
+        if (ms) ms[i] = g_timer_ms[i];
+        if (count) count[i] = g_timer_cnt[i];
+        if (names) {
+            size_t l = strlen(kClassNames[i]) + 1;
+            if (pos + l <= names_len) { memcpy(names + pos, kClassNames[i], l); pos += l; }
+        }
+    }
+    return KC_COUNT;
+}
+
+}  // extern "C"

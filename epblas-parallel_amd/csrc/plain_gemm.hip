@@ -1,0 +1,395 @@
+// plain_gemm.hip -- the "plain" double-double GEMMs on the FP64 vector ALU.
+//
+// One workgroup (256 lanes = 4 wave64) owns a BM x BN tile of C; each lane owns a TM x TN
+// register tile whose rows are strided by 16 (lane%16 + 16*i) and columns by 16
+// (lane/16 + 16*j), so every LDS read is either conflict-free or a broadcast.  op(A) and
+// op(B) K-slabs of depth BK are staged global -> registers -> LDS (transpose and
+// conjugation folded into the staging), with the next slab's global loads in flight while
+// the current slab is consumed.
+//
+// Per output element the arithmetic is exactly the reference's (bit-for-bit):
+//   C := beta-prepass(C)                      (multifloats/mgemm_parallel.cpp:68-77)
+//   for each K chunk of 128 from k = 0        (mgemm_serial.cpp:35-37, :126)
+//       acc := (0,0); acc := fast_add(acc, fast_mul(a_ip, b_pj)), p ascending
+//                                             (mgemm_serial.cpp:241-269)
+//       C := faithful_add(C, fast_mul(alpha, acc))   (mgemm_serial.cpp:173-191)
+// The running C lives in HBM between chunk flushes: one read-modify-write per 128-deep
+// chunk costs ~1 byte per CU-clock, far below what the FP64 ALU leaves idle.
+#include "mw_internal.hpp"
+#include "timer.hpp"
+
+namespace mw {
+
+constexpr int KCHUNK = 128;   // reference KC: the accumulator flush period
+
+// ----------------------------------------------------------------------------------
+// Tile staging helpers (generic over element type T = ddv or zddv).
+// ----------------------------------------------------------------------------------
+
+DDI ddv conj_elem(ddv x) { return x; }
+DDI zddv conj_elem(zddv x) { return zdd_conj(x); }
+
+// op(A) slab [i0, i0+BM) x [k0, k0+BK) -> regs.  TRANS: A is stored K x M.
+template <typename T, int BM, int BK, bool TRANS, bool CONJ>
+struct SlabA {
+    static constexpr int PER = BM * BK / 256;
+    T v[PER];
+    DDI void load(const T *__restrict__ A, int64_t lda, int64_t m, int64_t k, int64_t i0,
+                  int64_t k0, int tid)
+    {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            int e = tid + 256 * q;
+            int r, c;
+            if (!TRANS) { r = e % BM; c = e / BM; }
+            else        { c = e % BK; r = e / BK; }
+            int64_t gi = i0 + r, gk = k0 + c;
+            T x{};
+            if (gi < m && gk < k) {
+                x = TRANS ? A[gi * lda + gk] : A[gk * lda + gi];
+                if constexpr (CONJ) x = conj_elem(x);
+            }
+            v[q] = x;
+        }
+    }
+    template <int LDW>
+    DDI void store(T (*S)[LDW], int tid) const
+    {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            int e = tid + 256 * q;
+            int r, c;
+            if (!TRANS) { r = e % BM; c = e / BM; }
+            else        { c = e % BK; r = e / BK; }
+            S[c][r] = v[q];
+        }
+    }
+};
+
+// op(B) slab [k0, k0+BK) x [j0, j0+BN) -> regs.  TRANS: B is stored N x K.
+template <typename T, int BN, int BK, bool TRANS, bool CONJ>
+struct SlabB {
+    static constexpr int PER = BN * BK / 256;
+    T v[PER];
+    DDI void load(const T *__restrict__ B, int64_t ldb, int64_t n, int64_t k, int64_t j0,
+                  int64_t k0, int tid)
+    {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            int e = tid + 256 * q;
+            int r, c;   // r: column offset in [0,BN), c: k offset in [0,BK)
+            if (!TRANS) { c = e % BK; r = e / BK; }
+            else        { r = e % BN; c = e / BN; }
+            int64_t gj = j0 + r, gk = k0 + c;
+            T x{};
+            if (gj < n && gk < k) {
+                x = TRANS ? B[gk * ldb + gj] : B[gj * ldb + gk];
+                if constexpr (CONJ) x = conj_elem(x);
+            }
+            v[q] = x;
+        }
+    }
+    template <int LDW>
+    DDI void store(T (*S)[LDW], int tid) const
+    {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            int e = tid + 256 * q;
+            int r, c;
+            if (!TRANS) { c = e % BK; r = e / BK; }
+            else        { r = e % BN; c = e / BN; }
+            S[c][r] = v[q];
+        }
+    }
+};
+
+// ----------------------------------------------------------------------------------
+// Real kernel.
+// ----------------------------------------------------------------------------------
+template <int TM, int TN, int BK, bool TA, bool TB>
+__global__ __launch_bounds__(256)
+void k_mgemm_plain(int64_t m, int64_t n, int64_t k, ddv alpha, ddv beta, int beta_mode,
+                   const ddv *__restrict__ A, int64_t lda, const ddv *__restrict__ B,
+                   int64_t ldb, ddv *__restrict__ C, int64_t ldc, int tiles_m)
+{
+    constexpr int BM = 16 * TM, BN = 16 * TN;
+    static_assert(KCHUNK % BK == 0, "BK must divide the flush period");
+    __shared__ ddv As[BK][BM];
+    __shared__ ddv Bs[BK][BN];
+
+    const int tid = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
+    const int64_t i0 = (int64_t)(blockIdx.x % tiles_m) * BM;
+    const int64_t j0 = (int64_t)(blockIdx.x / tiles_m) * BN;
+
+    SlabA<ddv, BM, BK, TA, false> sa;
+    SlabB<ddv, BN, BK, TB, false> sb;
+
+    ddv acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = ddv{0.0, 0.0};
+
+    const int64_t ntiles = (k + BK - 1) / BK;
+    sa.load(A, lda, m, k, i0, 0, tid);
+    sb.load(B, ldb, n, k, j0, 0, tid);
+    sa.store(As, tid);
+    sb.store(Bs, tid);
+    __syncthreads();
+
+    for (int64_t t = 0; t < ntiles; ++t) {
+        const int64_t k0 = t * BK;
+        if (t + 1 < ntiles) {
+            sa.load(A, lda, m, k, i0, k0 + BK, tid);
+            sb.load(B, ldb, n, k, j0, k0 + BK, tid);
+        }
+        const int kmax = (k - k0 < BK) ? (int)(k - k0) : BK;
+        if (kmax == BK) {
+#pragma unroll 2
+            for (int kk = 0; kk < BK; ++kk) {
+                ddv a[TM], b[TN];
+#pragma unroll
+                for (int i = 0; i < TM; ++i) a[i] = As[kk][tx + 16 * i];
+#pragma unroll
+                for (int j = 0; j < TN; ++j) b[j] = Bs[kk][ty + 16 * j];
+#pragma unroll
+                for (int i = 0; i < TM; ++i)
+#pragma unroll
+                    for (int j = 0; j < TN; ++j)
+                        acc[i][j] = dd_add_fast(acc[i][j], dd_mul_fast(a[i], b[j]));
+            }
+        } else {
+            for (int kk = 0; kk < kmax; ++kk) {
+                ddv a[TM], b[TN];
+#pragma unroll
+                for (int i = 0; i < TM; ++i) a[i] = As[kk][tx + 16 * i];
+#pragma unroll
+                for (int j = 0; j < TN; ++j) b[j] = Bs[kk][ty + 16 * j];
+#pragma unroll
+                for (int i = 0; i < TM; ++i)
+#pragma unroll
+                    for (int j = 0; j < TN; ++j)
+                        acc[i][j] = dd_add_fast(acc[i][j], dd_mul_fast(a[i], b[j]));
+            }
+        }
+        const int64_t kend = k0 + kmax;
+        if (kend % KCHUNK == 0 || kend == k) {
+            const bool first = (kend <= KCHUNK);
+#pragma unroll
+            for (int i = 0; i < TM; ++i) {
+                const int64_t gi = i0 + tx + 16 * i;
+#pragma unroll
+                for (int j = 0; j < TN; ++j) {
+                    const int64_t gj = j0 + ty + 16 * j;
+                    if (gi < m && gj < n) {
+                        ddv *cp = &C[gj * ldc + gi];
+                        ddv c = *cp;
+                        if (first) c = dd_beta_prepass(c, beta, beta_mode);
+                        *cp = dd_add_exact(c, dd_mul_fast(alpha, acc[i][j]));
+                    }
+                    acc[i][j] = ddv{0.0, 0.0};
+                }
+            }
+        }
+        __syncthreads();
+        if (t + 1 < ntiles) {
+            sa.store(As, tid);
+            sb.store(Bs, tid);
+            __syncthreads();
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------------
+// Complex kernel (same structure; op letters N/T/C with conjugation in the staging).
+// ----------------------------------------------------------------------------------
+template <int TM, int TN, int BK, int OPA, int OPB>
+__global__ __launch_bounds__(256)
+void k_wgemm_plain(int64_t m, int64_t n, int64_t k, zddv alpha, zddv beta, int beta_mode,
+                   const zddv *__restrict__ A, int64_t lda, const zddv *__restrict__ B,
+                   int64_t ldb, zddv *__restrict__ C, int64_t ldc, int tiles_m)
+{
+    constexpr int BM = 16 * TM, BN = 16 * TN;
+    static_assert(KCHUNK % BK == 0, "BK must divide the flush period");
+    __shared__ zddv As[BK][BM];
+    __shared__ zddv Bs[BK][BN];
+
+    const int tid = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
+    const int64_t i0 = (int64_t)(blockIdx.x % tiles_m) * BM;
+    const int64_t j0 = (int64_t)(blockIdx.x / tiles_m) * BN;
+
+    SlabA<zddv, BM, BK, OPA != OP_N, OPA == OP_C> sa;
+    SlabB<zddv, BN, BK, OPB != OP_N, OPB == OP_C> sb;
+
+    zddv acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = zddv{{0.0, 0.0}, {0.0, 0.0}};
+
+    const int64_t ntiles = (k + BK - 1) / BK;
+    sa.load(A, lda, m, k, i0, 0, tid);
+    sb.load(B, ldb, n, k, j0, 0, tid);
+    sa.store(As, tid);
+    sb.store(Bs, tid);
+    __syncthreads();
+
+    for (int64_t t = 0; t < ntiles; ++t) {
+        const int64_t k0 = t * BK;
+        if (t + 1 < ntiles) {
+            sa.load(A, lda, m, k, i0, k0 + BK, tid);
+            sb.load(B, ldb, n, k, j0, k0 + BK, tid);
+        }
+        const int kmax = (k - k0 < BK) ? (int)(k - k0) : BK;
+        for (int kk = 0; kk < kmax; ++kk) {
+            zddv a[TM], b[TN];
+#pragma unroll
+            for (int i = 0; i < TM; ++i) a[i] = As[kk][tx + 16 * i];
+#pragma unroll
+            for (int j = 0; j < TN; ++j) b[j] = Bs[kk][ty + 16 * j];
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j)
+                    acc[i][j] = zdd_add_fast(acc[i][j], zdd_mul_fast(a[i], b[j]));
+        }
+        const int64_t kend = k0 + kmax;
+        if (kend % KCHUNK == 0 || kend == k) {
+            const bool first = (kend <= KCHUNK);
+#pragma unroll
+            for (int i = 0; i < TM; ++i) {
+                const int64_t gi = i0 + tx + 16 * i;
+#pragma unroll
+                for (int j = 0; j < TN; ++j) {
+                    const int64_t gj = j0 + ty + 16 * j;
+                    if (gi < m && gj < n) {
+                        zddv *cp = &C[gj * ldc + gi];
+                        zddv c = *cp;
+                        if (first) c = zdd_beta_prepass(c, beta, beta_mode);
+                        *cp = zdd_add_exact(c, zdd_mul_fast(alpha, acc[i][j]));
+                    }
+                    acc[i][j] = zddv{{0.0, 0.0}, {0.0, 0.0}};
+                }
+            }
+        }
+        __syncthreads();
+        if (t + 1 < ntiles) {
+            sa.store(As, tid);
+            sb.store(Bs, tid);
+            __syncthreads();
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------------
+// beta pre-pass alone (k == 0 or alpha == 0: mgemm_parallel.cpp:68-78).
+// ----------------------------------------------------------------------------------
+__global__ __launch_bounds__(256)
+void k_mgemm_beta_only(int64_t m, int64_t n, ddv beta, int beta_mode, ddv *__restrict__ C,
+                       int64_t ldc)
+{
+    int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= m * n) return;
+    int64_t i = e % m, j = e / m;
+    ddv *cp = &C[j * ldc + i];
+    *cp = dd_beta_prepass(*cp, beta, beta_mode);
+}
+
+__global__ __launch_bounds__(256)
+void k_wgemm_beta_only(int64_t m, int64_t n, zddv beta, int beta_mode, zddv *__restrict__ C,
+                       int64_t ldc)
+{
+    int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= m * n) return;
+    int64_t i = e % m, j = e / m;
+    zddv *cp = &C[j * ldc + i];
+    *cp = zdd_beta_prepass(*cp, beta, beta_mode);
+}
+
+// ----------------------------------------------------------------------------------
+// Launchers.
+// ----------------------------------------------------------------------------------
+namespace {
+constexpr int RTM = 4, RTN = 4, RBK = 16;   // real tile: 64 x 64, 16-deep slabs
+constexpr int CTM = 2, CTN = 2, CBK = 16;   // complex tile: 32 x 32
+
+template <bool TA, bool TB>
+hipError_t launch_real(const GemmArgsR &g, hipStream_t s)
+{
+    constexpr int BM = 16 * RTM, BN = 16 * RTN;
+    int tiles_m = (int)((g.m + BM - 1) / BM);
+    int64_t tiles = (int64_t)tiles_m * ((g.n + BN - 1) / BN);
+    if (tiles > INT32_MAX) return hipErrorInvalidValue;
+    ScopedTimer tm(KC_PLAIN_REAL, s);
+    hipLaunchKernelGGL((k_mgemm_plain<RTM, RTN, RBK, TA, TB>), dim3((unsigned)tiles), dim3(256),
+                       0, s, g.m, g.n, g.k, g.alpha, g.beta, g.beta_mode, g.A, g.lda, g.B,
+                       g.ldb, g.C, g.ldc, tiles_m);
+    return hipGetLastError();
+}
+
+template <int OA, int OB>
+hipError_t launch_cplx(const GemmArgsZ &g, hipStream_t s)
+{
+    constexpr int BM = 16 * CTM, BN = 16 * CTN;
+    int tiles_m = (int)((g.m + BM - 1) / BM);
+    int64_t tiles = (int64_t)tiles_m * ((g.n + BN - 1) / BN);
+    if (tiles > INT32_MAX) return hipErrorInvalidValue;
+    ScopedTimer tm(KC_PLAIN_CPLX, s);
+    hipLaunchKernelGGL((k_wgemm_plain<CTM, CTN, CBK, OA, OB>), dim3((unsigned)tiles), dim3(256),
+                       0, s, g.m, g.n, g.k, g.alpha, g.beta, g.beta_mode, g.A, g.lda, g.B,
+                       g.ldb, g.C, g.ldc, tiles_m);
+    return hipGetLastError();
+}
+
+template <int OA>
+hipError_t launch_cplx_b(const GemmArgsZ &g, hipStream_t s)
+{
+    switch (g.opb) {
+    case OP_N: return launch_cplx<OA, OP_N>(g, s);
+    case OP_T: return launch_cplx<OA, OP_T>(g, s);
+    default:   return launch_cplx<OA, OP_C>(g, s);
+    }
+}
+}  // namespace
+
+hipError_t plain_beta_only_real(const GemmArgsR &g, hipStream_t s)
+{
+    int64_t cnt = g.m * g.n;
+    if (cnt <= 0 || g.beta_mode == BETA_ONE) return hipSuccess;
+    ScopedTimer tm(KC_BETA_ONLY, s);
+    hipLaunchKernelGGL(k_mgemm_beta_only, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s,
+                       g.m, g.n, g.beta, g.beta_mode, g.C, g.ldc);
+    return hipGetLastError();
+}
+
+hipError_t plain_beta_only_cplx(const GemmArgsZ &g, hipStream_t s)
+{
+    int64_t cnt = g.m * g.n;
+    if (cnt <= 0 || g.beta_mode == BETA_ONE) return hipSuccess;
+    ScopedTimer tm(KC_BETA_ONLY, s);
+    hipLaunchKernelGGL(k_wgemm_beta_only, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s,
+                       g.m, g.n, g.beta, g.beta_mode, g.C, g.ldc);
+    return hipGetLastError();
+}
+
+hipError_t plain_gemm_real(const GemmArgsR &g, hipStream_t s)
+{
+    const bool ta = g.opa != OP_N, tb = g.opb != OP_N;
+    if (!ta && !tb) return launch_real<false, false>(g, s);
+    if (ta && !tb)  return launch_real<true, false>(g, s);
+    if (!ta && tb)  return launch_real<false, true>(g, s);
+    return launch_real<true, true>(g, s);
+}
+
+hipError_t plain_gemm_cplx(const GemmArgsZ &g, hipStream_t s)
+{
+    switch (g.opa) {
+    case OP_N: return launch_cplx_b<OP_N>(g, s);
+    case OP_T: return launch_cplx_b<OP_T>(g, s);
+    default:   return launch_cplx_b<OP_C>(g, s);
+    }
+}
+
+}  // namespace mw

@@ -1,0 +1,211 @@
+"""Host-side mirror of the reference mwblas GEMM interface over libmwgemm_hip.so.
+
+The functions keep the reference's names, argument order and meaning
+(``mgemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc)``,
+src/epblas-parallel/multifloats/mgemm_parallel.cpp:189-191 through
+common/epblas_facade.h:234-240) and call the library's Fortran-ABI symbols by
+reference, exactly as a Fortran or C caller of the reference would.
+
+Array convention: a column-major DD matrix with leading dimension ``ld`` and ``cols``
+columns is a float64 array of shape ``(cols, ld, 2)`` (C-contiguous) holding
+``(hi, lo)``; complex DD uses shape ``(cols, ld, 4)`` holding
+``(re.hi, re.lo, im.hi, im.lo)``.  Host numpy arrays and device ``torch`` tensors are
+both accepted by the drop-in calls (device memory is used in place).
+
+There is no CPU fallback: if the library is missing or a call fails, this module raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmwgemm_hip.so")
+
+PATH_PLAIN, PATH_OZAKI, PATH_AUTO = 0, 1, 2
+
+_LIB = None
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+
+
+class MwgemmError(RuntimeError):
+    pass
+
+
+
+
+def lib():
+    """Load libmwgemm_hip.so (built by build.py); raise if it is not there."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise MwgemmError(f"{LIB_PATH} is missing: run epblas-parallel_amd/build.py first")
+    L = ctypes.CDLL(LIB_PATH)
+    c = ctypes.c_char
+    for name, ip in (("mgemm_", _i32p), ("mgemm_64_", _i64p), ("wgemm_", _i32p),
+                     ("wgemm_64_", _i64p)):
+        f = getattr(L, name)
+        f.restype = None
+        f.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ip, ip, ip, _vp, _vp, ip, _vp, ip, _vp,
+                      _vp, ip]
+    for name in ("mgemm_serial", "wgemm_serial"):
+        f = getattr(L, name)
+        f.restype = None
+        f.argtypes = [c, c, _i64, _i64, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _i64]
+    for name in ("mwgemm_mgemm_device", "mwgemm_wgemm_device"):
+        f = getattr(L, name)
+        f.restype = ctypes.c_int
+        f.argtypes = [c, c, _i64, _i64, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _i64,
+                      ctypes.c_int, _vp]
+    L.mwgemm_set_path.restype = ctypes.c_int
+    L.mwgemm_set_path.argtypes = [ctypes.c_int]
+    L.mwgemm_get_path.restype = ctypes.c_int
+    L.mwgemm_last_error.restype = ctypes.c_int
+    L.mwgemm_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+    L.mwgemm_clear_error.restype = None
+    L.mwgemm_set_abort.restype = ctypes.c_int
+    L.mwgemm_set_abort.argtypes = [ctypes.c_int]
+    L.mwgemm_timer_enable.restype = ctypes.c_int
+    L.mwgemm_timer_enable.argtypes = [ctypes.c_int]
+    L.mwgemm_timer_reset.restype = ctypes.c_int
+    L.mwgemm_timer_read.restype = ctypes.c_int
+    L.mwgemm_timer_read.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
+    if hasattr(L, "mwgemm_mgemm_batched_device"):
+        f = L.mwgemm_mgemm_batched_device
+        f.restype = ctypes.c_int
+        f.argtypes = [ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                      _vp, _vp, ctypes.c_int, _vp]
+    if hasattr(L, "mwgemm_ozaki_config"):
+        L.mwgemm_ozaki_config.restype = ctypes.c_int
+        L.mwgemm_ozaki_config.argtypes = [ctypes.c_int64, _vp, _vp]
+    _LIB = L
+    # Errors come back as exceptions here rather than aborting the interpreter.
+    L.mwgemm_set_abort(0)
+    return L
+
+
+def last_error():
+    buf = ctypes.create_string_buffer(512)
+    code = lib().mwgemm_last_error(buf, 512)
+    return code, buf.value.decode(errors="replace")
+
+
+def clear_error():
+    lib().mwgemm_clear_error()
+
+
+def _check(rc=None):
+    code, msg = last_error()
+    if (rc is not None and rc != 0) or code != 0:
+        clear_error()
+        raise MwgemmError(f"libmwgemm_hip error {code if code else rc}: {msg}")
+
+
+def set_path(path):
+    prev = lib().mwgemm_set_path(path)
+    if prev < 0:
+        raise ValueError(f"bad path {path}")
+    return prev
+
+
+def _ptr(x):
+    """Data pointer of a numpy array or a torch tensor (host or device)."""
+    if x is None:
+        return None
+    if isinstance(x, np.ndarray):
+        return x.ctypes.data_as(ctypes.c_void_p)
+    return ctypes.c_void_p(x.data_ptr())
+
+
+def _scalar(x, width):
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(-1))
+    if a.size != width:
+        raise ValueError(f"scalar must have {width} limbs")
+    return a
+
+
+def _gemm(name, width, ip, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    L = lib()
+    ct = ctypes.c_int32 if ip is _i32p else ctypes.c_int64
+    al, be = _scalar(alpha, width), _scalar(beta, width)
+    args = [ct(v) for v in (m, n, k, lda, ldb, ldc)]
+    clear_error()
+    getattr(L, name)(transa.encode(), transb.encode(), ctypes.byref(args[0]),
+                     ctypes.byref(args[1]), ctypes.byref(args[2]), _ptr(al), _ptr(a),
+                     ctypes.byref(args[3]), _ptr(b), ctypes.byref(args[4]), _ptr(be), _ptr(c),
+                     ctypes.byref(args[5]))
+    _check()
+
+
+def mgemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    """C := alpha*op(A)*op(B) + beta*C (real DD) through the LP64 drop-in ``mgemm_``."""
+    _gemm("mgemm_", 2, _i32p, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc)
+
+
+def mgemm_64(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    """ILP64 twin ``mgemm_64_``."""
+    _gemm("mgemm_64_", 2, _i64p, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc)
+
+
+def wgemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    """Complex DD GEMM through the LP64 drop-in ``wgemm_``."""
+    _gemm("wgemm_", 4, _i32p, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc)
+
+
+def wgemm_64(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    _gemm("wgemm_64_", 4, _i64p, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc)
+
+
+def mgemm_serial(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    """The by-value core ``mgemm_serial`` (reference mgemm_kernel.h:68-75)."""
+    al, be = _scalar(alpha, 2), _scalar(beta, 2)
+    clear_error()
+    lib().mgemm_serial(transa.encode(), transb.encode(), m, n, k, _ptr(al), _ptr(a), lda,
+                       _ptr(b), ldb, _ptr(be), _ptr(c), ldc)
+    _check()
+
+
+def wgemm_serial(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    al, be = _scalar(alpha, 4), _scalar(beta, 4)
+    clear_error()
+    lib().wgemm_serial(transa.encode(), transb.encode(), m, n, k, _ptr(al), _ptr(a), lda,
+                       _ptr(b), ldb, _ptr(be), _ptr(c), ldc)
+    _check()
+
+
+def gemm_device(cplx, transa, transb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc,
+                path=PATH_PLAIN, stream=None):
+    """Device-pointer GEMM (tensors on the GPU), asynchronous on ``stream`` (an int handle
+    such as ``torch.cuda.current_stream().cuda_stream``; None = the library's stream)."""
+    L = lib()
+    w = 4 if cplx else 2
+    al, be = _scalar(alpha, w), _scalar(beta, w)
+    f = L.mwgemm_wgemm_device if cplx else L.mwgemm_mgemm_device
+    clear_error()
+    rc = f(transa.encode(), transb.encode(), m, n, k, _ptr(al), _ptr(dA), lda, _ptr(dB), ldb,
+           _ptr(be), _ptr(dC), ldc, path, None if stream is None else ctypes.c_void_p(stream))
+    if rc != 0:
+        _check(rc)
+
+
+def timer_enable(on=True):
+    return lib().mwgemm_timer_enable(1 if on else 0)
+
+
+def timer_reset():
+    lib().mwgemm_timer_reset()
+
+
+def timer_read():
+    """{kernel class: (total ms, launches)} since the last reset (synchronises)."""
+    cap = 32
+    ms = np.zeros(cap)
+    cnt = np.zeros(cap, dtype=np.int64)
+    names = ctypes.create_string_buffer(2048)
+    n = lib().mwgemm_timer_read(_ptr(ms), _ptr(cnt), cap, names, 2048)
+    labels = names.raw.split(b"\0")[:n]
+    return {labels[i].decode(): (float(ms[i]), int(cnt[i])) for i in range(n) if cnt[i]}

@@ -1,0 +1,127 @@
+/*
+ * mwgemm_hip.h -- C ABI of the MI355X (gfx950) double-double GEMM library
+ * (libmwgemm_hip.so).
+ *
+ * The first group of entry points is the drop-in boundary: the same symbols, argument
+ * meaning and calling convention as the reference's mwblas GEMMs, so an existing caller
+ * links this library in front of (or instead of) epblas-parallel::mwblas and its mgemm_ /
+ * wgemm_ calls run on the GPU.  Matrices are column-major; every argument is passed by
+ * reference (Fortran ABI); no hidden character-length arguments are declared.  Pointers may
+ * be host memory (staged through the device) or device memory (used in place).
+ *
+ * The second group has no counterpart in the reference: device-pointer entry points with
+ * an explicit HIP stream, a batched entry point, the numeric-path switch, error reporting
+ * and kernel timers.
+ */
+#ifndef MWGEMM_HIP_H
+#define MWGEMM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifndef MWGEMM_API
+#define MWGEMM_API __attribute__((visibility("default")))
+#endif
+
+/* POD layouts of multifloats::float64x2 {double limbs[2]} and complex64x2 {re, im}
+ * (third-party multifloats v0.8.1; mirrored at
+ * benchmark/drivers/target_multifloats/perf_mgemm.cpp:13-14). */
+typedef struct { double limbs[2]; } mw_float64x2;
+typedef struct { mw_float64x2 re, im; } mw_complex64x2;
+
+/* ---------------------------------------------------------------------------------
+ * Drop-in boundary.
+ * ------------------------------------------------------------------------------- */
+
+/* C := alpha*op(A)*op(B) + beta*C over double-double, LP64 integers.
+ * Replaces mgemm_ = EPBLAS_FACADE_GEMM(mgemm, TR) at
+ * src/epblas-parallel/multifloats/mgemm_parallel.cpp:189-191
+ * (macro src/epblas-parallel/common/epblas_facade.h:234-240, LP64 twin :63-65). */
+MWGEMM_API void mgemm_(const char *transa, const char *transb, const int *m, const int *n, const int *k,
+            const mw_float64x2 *alpha, const mw_float64x2 *a, const int *lda,
+            const mw_float64x2 *b, const int *ldb, const mw_float64x2 *beta,
+            mw_float64x2 *c, const int *ldc);
+
+/* ILP64 twin of mgemm_ (epblas_facade.h:63-65, suffix _64_). */
+MWGEMM_API void mgemm_64_(const char *transa, const char *transb, const int64_t *m, const int64_t *n,
+               const int64_t *k, const mw_float64x2 *alpha, const mw_float64x2 *a,
+               const int64_t *lda, const mw_float64x2 *b, const int64_t *ldb,
+               const mw_float64x2 *beta, mw_float64x2 *c, const int64_t *ldc);
+
+/* Complex double-double GEMM, op letters N/T/C.  Replaces wgemm_ at
+ * src/epblas-parallel/multifloats/wgemm_parallel.cpp:196-198 (same facade macro). */
+MWGEMM_API void wgemm_(const char *transa, const char *transb, const int *m, const int *n, const int *k,
+            const mw_complex64x2 *alpha, const mw_complex64x2 *a, const int *lda,
+            const mw_complex64x2 *b, const int *ldb, const mw_complex64x2 *beta,
+            mw_complex64x2 *c, const int *ldc);
+
+/* ILP64 twin of wgemm_. */
+MWGEMM_API void wgemm_64_(const char *transa, const char *transb, const int64_t *m, const int64_t *n,
+               const int64_t *k, const mw_complex64x2 *alpha, const mw_complex64x2 *a,
+               const int64_t *lda, const mw_complex64x2 *b, const int64_t *ldb,
+               const mw_complex64x2 *beta, mw_complex64x2 *c, const int64_t *ldc);
+
+/* By-value core called by the reference's other mw level-3 routines for their trailing
+ * updates (msyrk_serial.cpp:336-352, msymm_serial.cpp:396-455, mtrsm_serial.cpp:772-813).
+ * Replaces mgemm_serial, src/epblas-parallel/multifloats/mgemm_kernel.h:68-75. */
+MWGEMM_API void mgemm_serial(char transa, char transb, ptrdiff_t m, ptrdiff_t n, ptrdiff_t k,
+                  const mw_float64x2 *alpha, const mw_float64x2 *a, ptrdiff_t lda,
+                  const mw_float64x2 *b, ptrdiff_t ldb, const mw_float64x2 *beta,
+                  mw_float64x2 *c, ptrdiff_t ldc);
+
+/* Replaces wgemm_serial, src/epblas-parallel/multifloats/wgemm_kernel.h:62-69. */
+MWGEMM_API void wgemm_serial(char transa, char transb, ptrdiff_t m, ptrdiff_t n, ptrdiff_t k,
+                  const mw_complex64x2 *alpha, const mw_complex64x2 *a, ptrdiff_t lda,
+                  const mw_complex64x2 *b, ptrdiff_t ldb, const mw_complex64x2 *beta,
+                  mw_complex64x2 *c, ptrdiff_t ldc);
+
+/* ---------------------------------------------------------------------------------
+ * Library extensions (no counterpart in the reference).
+ * ------------------------------------------------------------------------------- */
+
+/* Numeric path.  PLAIN reproduces the reference's results bit for bit (FP64 TwoProd/TwoSum
+ * on the vector ALU); OZAKI computes the product exactly from INT8 residues on the matrix
+ * cores (normwise error below 1e-30, not bitwise equal to PLAIN); AUTO picks OZAKI for
+ * large products.  The default comes from the environment variable MWGEMM_PATH
+ * ("plain" | "ozaki" | "auto"), else PLAIN. */
+enum { MWGEMM_PATH_PLAIN = 0, MWGEMM_PATH_OZAKI = 1, MWGEMM_PATH_AUTO = 2 };
+MWGEMM_API int mwgemm_set_path(int path);          /* returns the previous path, -1 on a bad value */
+MWGEMM_API int mwgemm_get_path(void);
+
+/* Error handling.  Every failure is reported on stderr and recorded; the drop-in entry
+ * points then abort the process unless MWGEMM_ON_ERROR=return (or mwgemm_set_abort(0)),
+ * in which case C is left unmodified and the error stays readable here.  The device entry
+ * points never abort; they return the error code. */
+MWGEMM_API int mwgemm_last_error(char *buf, size_t len);   /* last error code (0 = none), message */
+MWGEMM_API void mwgemm_clear_error(void);
+MWGEMM_API int mwgemm_set_abort(int on);                  /* returns the previous setting */
+
+/* Device-pointer GEMMs on an explicit stream (NULL = the library's stream).  alpha/beta
+ * are host values.  Return 0 or a hipError_t code.  The call is asynchronous w.r.t. the
+ * host; path is one of MWGEMM_PATH_*. */
+MWGEMM_API int mwgemm_mgemm_device(char transa, char transb, int64_t m, int64_t n, int64_t k,
+                        const mw_float64x2 *alpha, const mw_float64x2 *dA, int64_t lda,
+                        const mw_float64x2 *dB, int64_t ldb, const mw_float64x2 *beta,
+                        mw_float64x2 *dC, int64_t ldc, int path, void *stream);
+MWGEMM_API int mwgemm_wgemm_device(char transa, char transb, int64_t m, int64_t n, int64_t k,
+                        const mw_complex64x2 *alpha, const mw_complex64x2 *dA, int64_t lda,
+                        const mw_complex64x2 *dB, int64_t ldb, const mw_complex64x2 *beta,
+                        mw_complex64x2 *dC, int64_t ldc, int path, void *stream);
+
+/* Kernel timers: while enabled, every kernel the library launches is bracketed by HIP
+ * events on its own stream.  mwgemm_timer_read synchronises, then writes up to cap
+ * entries of {total milliseconds, launch count} per kernel class into ms/count and their
+ * names into names (NUL-separated, names_len bytes); it returns the number of classes. */
+MWGEMM_API int mwgemm_timer_enable(int on);
+MWGEMM_API int mwgemm_timer_reset(void);
+MWGEMM_API int mwgemm_timer_read(double *ms, int64_t *count, int cap, char *names, size_t names_len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MWGEMM_HIP_H */
