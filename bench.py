@@ -1,0 +1,193 @@
+"""Benchmark: double-double GEMM at 8192^3 on one MI355X, Ozaki-INT8 path vs plain FP64 path.
+
+One step = one C := alpha*A*B + beta*C (real double-double, NN, column-major, M=N=K=8192)
+through the exact-product path (INT8 residue GEMMs on the matrix cores + FP64 split and
+reconstruction), inputs resident in HBM:
+    hi = +-U[1,2) * 2^e, e ~ U_int[-20, 20];  lo = hi * 2^-53 * U[-1/2, 1/2];  alpha=1, beta=0.
+Reported value = effective DD-GFLOP/s = 2*M*N*K / t.  The plain FP64 TwoProd/TwoSum kernel
+runs on the same inputs for the ratio.  The roofline entry is the dominant kernel (the INT8
+residue GEMM) timed with HIP events on the library's stream; the CPU baseline is the
+oracle's restatement of the reference OpenMP mgemm_ timed on this host on a bounded row
+sample of the same problem.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--size 8192]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _sub in ("epblas-parallel_amd", "oracle"):
+    sys.path.insert(0, os.path.join(ROOT, _sub))
+
+METRIC = "DD-GEMM GFLOP/s at 8192³: Ozaki-INT8 vs plain FP64-FMA, rel err ≤1e-30"
+INT8_PEAK_TOPS = 5000.0      # MI355X dense INT8 MFMA, 5.0 POP/s
+FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector
+
+
+def gen_wide(n_rows, n_cols, gen, device):
+    import torch
+    shape = (n_cols, n_rows)
+    sign = torch.where(torch.rand(shape, generator=gen, device=device, dtype=torch.float64) < 0.5,
+                       -1.0, 1.0)
+    e = torch.randint(-20, 21, shape, generator=gen, device=device, dtype=torch.int32)
+    hi = sign * torch.ldexp(1.0 + torch.rand(shape, generator=gen, device=device,
+                                             dtype=torch.float64), e)
+    lo = hi * 2.0 ** -53 * (torch.rand(shape, generator=gen, device=device,
+                                       dtype=torch.float64) - 0.5)
+    return torch.stack([hi, lo], dim=-1).contiguous()
+
+
+def cpu_baseline(A, B, n, k, target_s=12.0):
+    """The oracle's CPU restatement of the reference mgemm_ (AVX2 + OpenMP jc split) on the
+    first rows of the same problem.  Returns the bench JSON object."""
+    import numpy as np
+    import oracle
+    L = oracle.lib()
+    threads = L.port_max_threads()
+    alpha = np.array([1.0, 0.0])
+    beta = np.zeros(2)
+    Bh = B.cpu().numpy()
+
+    def run(rows):
+        Ah = A[:, :rows, :].contiguous().cpu().numpy()
+        C = np.zeros((n, rows, 2))
+        t0 = time.perf_counter()
+        oracle.port_mgemm("N", "N", rows, n, k, alpha, Ah, rows, Bh, k, beta, C, rows)
+        return time.perf_counter() - t0
+
+    probe_rows = 64
+    t_probe = run(probe_rows)
+    rows = int(min(A.shape[1], max(64, probe_rows * target_s / max(t_probe, 1e-3))))
+    rows = max(64, rows // 64 * 64)
+    t = run(rows)
+    gflops = 2.0 * rows * n * k / t / 1e9
+    busy = min(threads, (n + 255) // 256)   # the reference splits N into 256-wide panels
+    return {"value": round(gflops, 3), "unit": "DD-GFLOP/s", "cores": busy, "kind": "port",
+            "sample": f"rows 0..{rows} of the {A.shape[1]}x{n}x{k} NN problem (same inputs), "
+                      f"{t:.1f} s, OMP threads={threads} (jc split keeps {busy} busy), "
+                      f"AVX2={bool(L.port_has_avx2())}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=8192)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-steps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on exactly one GPU (the GEMM shards along N with no "
+              "exchange beyond a broadcast of A; see the sharding note in the design doc). "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import mwblas
+
+    dev = torch.device("cuda:0")
+    torch.cuda.init()
+    n = k = m = args.size
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(8192)
+    A = gen_wide(m, k, gen, dev)          # (cols=k, ld=m, 2)
+    B = gen_wide(k, n, gen, dev)          # (cols=n, ld=k, 2)
+    C = torch.zeros((n, m, 2), dtype=torch.float64, device=dev)
+    alpha, beta = [1.0, 0.0], [0.0, 0.0]
+    stream = torch.cuda.current_stream().cuda_stream
+    nmod, bits = mwblas.ozaki_config(k)
+
+    def step(path, out):
+        mwblas.gemm_device(False, "N", "N", m, n, k, alpha, A, m, B, k, beta, out, m,
+                           path=path, stream=stream)
+
+    for _ in range(args.warmup):
+        step(mwblas.PATH_OZAKI, C)
+    torch.cuda.synchronize()
+    mwblas.timer_reset()
+    mwblas.timer_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(mwblas.PATH_OZAKI, C)
+    torch.cuda.synchronize()
+    t_oz = (time.perf_counter() - t0) / args.steps
+    mwblas.timer_enable(False)
+    kt = mwblas.timer_read()
+
+    # plain FP64 path on the same inputs
+    Cp = torch.zeros_like(C)
+    step(mwblas.PATH_PLAIN, Cp)
+    torch.cuda.synchronize()
+    mwblas.timer_reset()
+    mwblas.timer_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(args.plain_steps):
+        step(mwblas.PATH_PLAIN, Cp)
+    torch.cuda.synchronize()
+    t_plain = (time.perf_counter() - t0) / args.plain_steps
+    mwblas.timer_enable(False)
+    kp = mwblas.timer_read()
+
+    # agreement of the two GPU results, normwise (|alpha| K max|A| max|B|)
+    amax = A[..., 0].abs().max().item()
+    bmax = B[..., 0].abs().max().item()
+    diff = ((C[..., 0] - Cp[..., 0]) + (C[..., 1] - Cp[..., 1])).abs().max().item()
+    agree = diff / (k * amax * bmax)
+
+    flops = 2.0 * m * n * k
+    oz_gflops = flops / t_oz / 1e9
+    plain_gflops = flops / t_plain / 1e9
+    g_ms, g_cnt = kt.get("oz_gemm", (0.0, 0))
+    g_avg = g_ms / max(g_cnt, 1)
+    ops = 2.0 * m * n * k * nmod                         # INT8 ops per residue-GEMM launch
+    achieved = ops / (g_avg * 1e-3) / 1e12 if g_avg > 0 else None
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "oz_gemm_8192_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    p_ms, p_cnt = kp.get("plain_real", (0.0, 0))
+
+    result = {
+        "metric": METRIC,
+        "value": round(oz_gflops, 2),
+        "unit": "DD-GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_oz * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8-mfma+f64",
+        "data": "synthetic",
+        "config": {"workload": f"ozaki_int8_real_dd_gemm_{m}", "m": m, "n": n, "k": k,
+                   "trans": "NN", "alpha": alpha, "beta": beta,
+                   "inputs": "hi=+-U[1,2)*2^U_int[-20,20], lo=hi*2^-53*U[-1/2,1/2]",
+                   "moduli": nmod, "bits": bits, "parallelism": "single-gpu"},
+        "plain_fp64": {"value": round(plain_gflops, 2), "unit": "DD-GFLOP/s",
+                       "ms_per_step": round(t_plain * 1e3, 2),
+                       "kernel_ms": round(p_ms / max(p_cnt, 1), 2),
+                       "fp64_frac": round(18 * m * n * k / (p_ms / max(p_cnt, 1) * 1e-3) / 1e12
+                                          / (FP64_PEAK_TFLOPS / 2), 4) if p_cnt else None},
+        "speedup_vs_plain": round(oz_gflops / plain_gflops, 2),
+        "ozaki_vs_plain_normwise": agree,
+        "kernels_ms_per_step": {name: round(v[0] / max(v[1], 1) * (v[1] / args.steps), 3)
+                                for name, v in kt.items()},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 1) if achieved else None,
+                     "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / INT8_PEAK_TOPS, 4) if achieved else None,
+                     "traffic": traffic, "kernel": "k_oz_gemm (INT8 residue GEMM, int8 ops)"},
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(A, B, n, k)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

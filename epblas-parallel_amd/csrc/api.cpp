@@ -180,12 +180,16 @@ bool is_device_ptr(const void *p)
            attr.type == hipMemoryTypeUnified;
 }
 
-int choose_path(int path, int64_t m, int64_t n, int64_t k)
+// AUTO takes the exact-product path once the product is large enough to amortise the
+// residue split; OZAKI falls back to PLAIN (still on the GPU) beyond the depth the INT32
+// accumulation keeps exact.
+int choose_path(int path, int64_t m, int64_t n, int64_t k, bool cplx)
 {
+    const bool ok = cplx ? ozaki_supported_cplx(k) : ozaki_supported(k);
     if (path == MWGEMM_PATH_AUTO)
-        return ozaki_supported(k) && (double)m * (double)n * (double)k >= 1.0e9
+        return ok && (double)m * (double)n * (double)k * (cplx ? 4.0 : 1.0) >= 1.0e9
                    ? MWGEMM_PATH_OZAKI : MWGEMM_PATH_PLAIN;
-    if (path == MWGEMM_PATH_OZAKI && !ozaki_supported(k)) return MWGEMM_PATH_PLAIN;
+    if (path == MWGEMM_PATH_OZAKI && !ok) return MWGEMM_PATH_PLAIN;
     return path;
 }
 }  // namespace
@@ -208,7 +212,7 @@ int gemm_real_device(GemmArgsR g, int path, hipStream_t s)
         MW_TRY(plain_beta_only_real(g, s));
         return 0;
     }
-    if (choose_path(path, g.m, g.n, g.k) == MWGEMM_PATH_OZAKI) return ozaki_gemm_real(g, s);
+    if (choose_path(path, g.m, g.n, g.k, false) == MWGEMM_PATH_OZAKI) return ozaki_gemm_real(g, s);
     MW_TRY(plain_gemm_real(g, s));
     return 0;
 }
@@ -228,7 +232,7 @@ int gemm_cplx_device(GemmArgsZ g, int path, hipStream_t s)
         MW_TRY(plain_beta_only_cplx(g, s));
         return 0;
     }
-    if (choose_path(path, g.m, g.n, g.k) == MWGEMM_PATH_OZAKI) return ozaki_gemm_cplx(g, s);
+    if (choose_path(path, g.m, g.n, g.k, true) == MWGEMM_PATH_OZAKI) return ozaki_gemm_cplx(g, s);
     MW_TRY(plain_gemm_cplx(g, s));
     return 0;
 }
@@ -541,6 +545,41 @@ int mwgemm_wgemm_device(char transa, char transb, int64_t m, int64_t n, int64_t 
         s = st->stream;
     }
     return gemm_cplx_device(g, path, s);
+}
+
+int mwgemm_ozaki_config(int64_t k, int cplx, int *nmod, int *bits)
+{
+    OzPlan p;
+    if (!ozaki_make_plan(cplx ? 2 * k : k, ozaki_target_bits(), ozaki_forced_nmod(), &p)) return -1;
+    if (nmod) *nmod = p.nmod;
+    if (bits) *bits = p.bits;
+    return 0;
+}
+
+int mwgemm_ozaki_set(int bits, int nmod)
+{
+    if ((bits != 0 && (bits < 8 || bits > 104)) || nmod < 0 || nmod > OZ_MAX_MOD) return -1;
+    ozaki_set_params(bits ? bits : 103, nmod);
+    return 0;
+}
+
+int mwgemm_ozaki_moduli(int nmod, int *m, int *y)
+{
+    OzPlan p;
+    if (nmod < 1 || nmod > OZ_MAX_MOD || !ozaki_make_plan(1, 8, nmod, &p)) return -1;
+    for (int t = 0; t < nmod; ++t) {
+        if (m) m[t] = (int)p.mc[t].m;
+        if (y) y[t] = (int)p.mc[t].y;
+    }
+    return 0;
+}
+
+int mwgemm_testing_residue_gemm(const void *dA, const void *dB, void *dU, int64_t mp,
+                                int64_t np, int64_t kp, int mod_index, void *stream)
+{
+    return ozaki_residue_gemm_test(static_cast<const int8_t *>(dA), static_cast<const int8_t *>(dB),
+                                   static_cast<uint8_t *>(dU), mp, np, kp, mod_index,
+                                   static_cast<hipStream_t>(stream));
 }
 
 int mwgemm_timer_enable(int on)

@@ -1,6 +1,813 @@
+// ozaki.hip -- exact-product double-double GEMM on the INT8 matrix cores.
+//
+// Method (Ozaki scheme II, Ozaki-Uchino-Imamura 2024/25, "integer modular" emulation):
+//   1. Scale every row i of op(A) by 2^sigma_i and every column j of op(B) by 2^tau_j so
+//      that the largest entry of the row/column lies in [2^(b-1), 2^b); round the scaled
+//      double-double to an integer A'_ik, B'_kj (|A'|,|B'| <= 2^b, error <= ~1/2 ulp of the
+//      integer grid).
+//   2. For each modulus m_t of a table of pairwise-coprime moduli <= 256, take symmetric
+//      residues A'_t = A' mod m_t, B'_t = B' mod m_t, which fit in int8.
+//   3. C'_t = A'_t B'_t on the INT8 MFMA units with INT32 accumulation -- exact, since
+//      k * 128^2 < 2^31 -- reduced mod m_t and pre-multiplied by the CRT weight y_t.
+//   4. Reconstruct C' = A'B' exactly (|C'| < M/2, M = prod m_t) from its residues: C'/M =
+//      frac(sum_t u_t / m_t), evaluated in 128-bit fixed point with exact FP64 integer
+//      arithmetic, then C' = M * that as a double-double, scaled back by 2^-(sigma+tau),
+//      and combined as C := beta-prepass(C0) + alpha*C' (faithful DD ops).
+// Error: normwise (relative to K*max|A|*max|B|) at most ~4*2^-b from the rounding in
+// step 1; b = 103 bounds it by 4e-31.  FP64 is used only for the split, the
+// reconstruction and the alpha/beta update; every product runs on the matrix cores.
+//
+// Complex: Re C' = [Ar | -Ai] [Br ; Bi], Im C' = [Ar | Ai] [Bi ; Br] -- two real products
+// of doubled depth per modulus (four real slice products), conjugation folded into signs.
 #include "ozaki.hpp"
+#include "timer.hpp"
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
 namespace mw {
-bool ozaki_supported(int64_t) { return false; }
-int ozaki_gemm_real(const GemmArgsR &, hipStream_t) { return record_error(-4, "ozaki path not built"); }
-int ozaki_gemm_cplx(const GemmArgsZ &, hipStream_t) { return record_error(-4, "ozaki path not built"); }
+
+// Pairwise coprime, largest first: 2^8, 3*5*17, 11*23, 251, 13*19, primes, 7*31, primes.
+static const int kModuli[OZ_MAX_MOD] = {
+    256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199,
+    197, 193, 191, 181, 179, 173, 167, 163, 157, 151, 149, 139, 137, 131,
+    127, 113, 109, 107, 103, 101, 97,  89,  83,  79,  73,  71};
+
+// ==================================================================================
+// Host: plan construction.
+// ==================================================================================
+namespace {
+
+int64_t mod_pow2(int e, int64_t m)
+{
+    int64_t r = 1 % m;
+    for (int i = 0; i < e; ++i) r = (r * 2) % m;
+    return r;
 }
+
+int64_t mod_inverse(int64_t a, int64_t m)
+{
+    int64_t g = m, x = 0, x1 = 1, a1 = a % m;
+    while (a1) {
+        int64_t q = g / a1;
+        int64_t t = g - q * a1; g = a1; a1 = t;
+        t = x - q * x1; x = x1; x1 = t;
+    }
+    return ((x % m) + m) % m;   // g == 1 for coprime inputs
+}
+
+void host_two_sum(double a, double b, double &s, double &e)
+{
+    volatile double t = a + b;
+    volatile double bv = t - a;
+    volatile double av = t - bv;
+    e = (a - av) + (b - bv);
+    s = t;
+}
+
+// Little-endian base-2^32 big integer -> nearest-ish double-double (error ~2^-105 rel).
+void big_to_dd(const std::vector<uint32_t> &x, double &hi, double &lo)
+{
+    hi = 0.0;
+    lo = 0.0;
+    for (int i = (int)x.size() - 1; i >= 0; --i) {
+        double term = std::ldexp((double)x[i], 32 * i);
+        double s, e;
+        host_two_sum(hi, term, s, e);
+        e += lo;
+        host_two_sum(s, e, hi, lo);
+    }
+}
+
+std::mutex g_plan_mu;
+int g_target_bits = -1;
+int g_forced_nmod = -1;
+
+}  // namespace
+
+int ozaki_target_bits()
+{
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    if (g_target_bits < 0) {
+        const char *e = getenv("MWGEMM_OZAKI_BITS");
+        g_target_bits = e ? atoi(e) : 103;
+        if (g_target_bits < 8 || g_target_bits > 104) g_target_bits = 103;
+    }
+    return g_target_bits;
+}
+
+int ozaki_forced_nmod()
+{
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    if (g_forced_nmod < 0) {
+        const char *e = getenv("MWGEMM_OZAKI_NMOD");
+        g_forced_nmod = e ? atoi(e) : 0;
+        if (g_forced_nmod < 0 || g_forced_nmod > OZ_MAX_MOD) g_forced_nmod = 0;
+    }
+    return g_forced_nmod;
+}
+
+void ozaki_set_params(int bits, int nmod)
+{
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    if (bits >= 8 && bits <= 104) g_target_bits = bits;
+    if (nmod >= 0 && nmod <= OZ_MAX_MOD) g_forced_nmod = nmod;
+}
+
+bool ozaki_make_plan(int64_t k_eff, int target_bits, int force_nmod, OzPlan *p)
+{
+    if (k_eff <= 0 || k_eff > OZ_MAX_KEFF) return false;
+    const double lk = std::log2((double)k_eff);
+    int n = 0;
+    double lg = 0.0;
+    if (force_nmod > 0) {
+        n = force_nmod;
+        for (int i = 0; i < n; ++i) lg += std::log2((double)kModuli[i]);
+    } else {
+        // M/2 > k * 2^(2b)  <=>  log2 M > 2b + log2 k + 1; the 1e-6 margin keeps |C'|/M
+        // clear of 1/2 by far more than the 2^-116 error of the fixed-point reconstruction.
+        const double need = 2.0 * target_bits + lk + 1.0;
+        while (n < OZ_MAX_MOD && !(lg > need + 1e-6)) lg += std::log2((double)kModuli[n++]);
+        if (!(lg > need + 1e-6)) return false;
+    }
+    int b = (int)std::floor((lg - lk - 1.0) / 2.0);
+    while (b > 0 && !(lg > 2.0 * b + lk + 1.0 + 1e-6)) --b;
+    if (b > 104) b = 104;          // the split keeps every intermediate below 2^53 up to b = 104
+    if (b < 1) return false;
+    p->nmod = n;
+    p->bits = b;
+    p->log2M = lg;
+    std::vector<uint32_t> M(1, 1u);
+    for (int t = 0; t < n; ++t) {
+        uint64_t carry = 0;
+        for (auto &limb : M) {
+            uint64_t v = (uint64_t)limb * (uint64_t)kModuli[t] + carry;
+            limb = (uint32_t)v;
+            carry = v >> 32;
+        }
+        if (carry) M.push_back((uint32_t)carry);
+    }
+    big_to_dd(M, p->Mhi, p->Mlo);
+    for (int t = 0; t < n; ++t) {
+        const int64_t m = kModuli[t];
+        int64_t others = 1 % m;
+        for (int s = 0; s < n; ++s)
+            if (s != t) others = (others * kModuli[s]) % m;
+        ModConst &c = p->mc[t];
+        c.m = (double)m;
+        c.inv_m = 1.0 / (double)m;
+        c.y = (double)mod_inverse(others, m);
+        c.p1 = (double)mod_pow2(35, m);
+        c.p2 = (double)mod_pow2(70, m);
+        // round(2^128 / m) by long division of the 5-limb number 2^128.
+        uint32_t num[5] = {0, 0, 0, 0, 1};
+        uint32_t q[5] = {0, 0, 0, 0, 0};
+        uint64_t rem = 0;
+        for (int i = 4; i >= 0; --i) {
+            uint64_t cur = (rem << 32) | num[i];
+            q[i] = (uint32_t)(cur / (uint64_t)m);
+            rem = cur % (uint64_t)m;
+        }
+        if (2 * rem >= (uint64_t)m) {
+            for (int i = 0; i < 5; ++i) {
+                if (++q[i] != 0) break;
+            }
+        }
+        for (int l = 0; l < 4; ++l) c.w[l] = (double)q[l];
+    }
+    return true;
+}
+
+bool ozaki_supported(int64_t k) { return k >= 1 && k <= OZ_MAX_KEFF; }
+bool ozaki_supported_cplx(int64_t k) { return k >= 1 && 2 * k <= OZ_MAX_KEFF; }
+
+// ==================================================================================
+// Device kernels.
+// ==================================================================================
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+// Maximum that keeps a NaN (fmax would drop it): a NaN entry must poison its row.
+DDI double nan_max(double a, double b) { return (b > a || b != b) ? b : a; }
+DDI double mag(const ddv &x) { return fabs(x.hi) + fabs(x.lo); }
+DDI double mag(const zddv &x) { return nan_max(mag(x.re), mag(x.im)); }
+
+__global__ __launch_bounds__(256)
+void k_oz_fill_u64(unsigned long long *p, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// Row maxima of |hi|+|lo| (as the bit pattern of a non-negative double, so atomicMax on the
+// integer orders them; a NaN or Inf dominates).  RC: entry (r,k) at X[k*ld + r];
+// KC: entry (r,k) at X[r*ld + k].
+template <typename T>
+__global__ __launch_bounds__(256)
+void k_oz_rowmax_rc(const T *__restrict__ X, int64_t ld, int64_t R, int64_t K,
+                    unsigned long long *__restrict__ rmax)
+{
+    const int64_t r = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+    const int64_t k0 = (int64_t)blockIdx.y * 256 + (threadIdx.x >> 6) * 64;
+    if (r >= R) return;
+    double mx = 0.0;
+    const int64_t kend = (k0 + 64 < K) ? k0 + 64 : K;
+    for (int64_t k = k0; k < kend; ++k) mx = nan_max(mx, mag(X[k * ld + r]));
+    if (mx != 0.0 || mx != mx) atomicMax(&rmax[r], (unsigned long long)__double_as_longlong(mx));
+}
+
+template <typename T>
+__global__ __launch_bounds__(256)
+void k_oz_rowmax_kc(const T *__restrict__ X, int64_t ld, int64_t R, int64_t K,
+                    unsigned long long *__restrict__ rmax)
+{
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int64_t k0 = (int64_t)blockIdx.y * 1024;
+    double mx = 0.0;
+    if (r < R) {
+#pragma unroll 4
+        for (int s = 0; s < 16; ++s) {
+            int64_t k = k0 + lane + 64 * s;
+            if (k < K) mx = nan_max(mx, mag(X[r * ld + k]));
+        }
+    }
+    unsigned long long bits = (unsigned long long)__double_as_longlong(mx);
+    // NaN-aware wave maximum on the bit patterns.
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(bits, off, 64);
+        bits = o > bits ? o : bits;
+    }
+    if (lane == 0 && r < R && bits != 0ull) atomicMax(&rmax[r], bits);
+}
+
+// Scale exponent of a row from its maximum: largest entry lands in [2^(b-1), 2^b).
+DDI int row_sigma(unsigned long long bits, int b, bool &bad)
+{
+    double mx = __longlong_as_double((long long)bits);
+    bad = !(mx <= 1.79769313486231570815e308);   // NaN or Inf
+    if (bad || mx == 0.0) return 0;
+    int e;
+    (void)frexp(mx, &e);
+    return b - e;
+}
+
+// The integer round(x * 2^sigma) of a DD value as three exact FP64 chunks with
+// value = c0 + c1*2^35 + c2*2^70 (|c0| < 2^52, |c1| < 2^35, |c2| <= 2^34 for b <= 104).
+// The chunks are cut from |xh| and then signed: the low bits of a 53-bit significand are
+// always representable, whereas xh mod 2^70 of a negative xh generally is not.
+DDI void int_chunks(double hi, double lo, int sigma, double &c0, double &c1, double &c2)
+{
+    const double xh_s = ldexp(hi, sigma), xl_s = ldexp(lo, sigma);
+    const double xh = rint(xh_s);
+    const double xl = rint((xh_s - xh) + xl_s);
+    const double ax = fabs(xh);
+    const double a2 = floor(ax * 0x1p-70);
+    const double rem = __builtin_fma(-a2, 0x1p70, ax);
+    const double a1 = floor(rem * 0x1p-35);
+    const double a0 = __builtin_fma(-a1, 0x1p35, rem);
+    const double sg = xh < 0.0 ? -1.0 : 1.0;
+    c2 = sg * a2;
+    c1 = sg * a1;
+    c0 = __builtin_fma(sg, a0, xl);
+}
+
+// Symmetric residue of c0 + c1*2^35 + c2*2^70 modulo m, in [-(m-1)/2, m/2] (int8 range).
+DDI int sym_residue(double c0, double c1, double c2, const ModConst &mc)
+{
+    const double m = mc.m;
+    const double s = __builtin_fma(c2, mc.p2, __builtin_fma(c1, mc.p1, c0));
+    const double q = floor(s * mc.inv_m);
+    double r = __builtin_fma(-q, m, s);
+    r = (r < 0.0) ? r + m : r;
+    r = (r >= m) ? r - m : r;
+    r = (r > 0.5 * (m - 1.0)) ? r - m : r;
+    return (int)r;
+}
+
+DDI void comp(const ddv &x, int part, double &hi, double &lo)
+{
+    (void)part;
+    hi = x.hi;
+    lo = x.lo;
+}
+DDI void comp(const zddv &x, int part, double &hi, double &lo)
+{
+    const ddv &c = part ? x.im : x.re;
+    hi = c.hi;
+    lo = c.lo;
+}
+
+// Residue split: row r of the operand, 16 consecutive k per lane; all moduli.
+// out1[t*mod_stride + r*kp + k] = s1 * (residue of entry (r,k)), likewise out2 (optional).
+template <typename T, bool KC>
+__global__ __launch_bounds__(256)
+void k_oz_split(const T *__restrict__ X, int64_t ld, int64_t R, int64_t K, int64_t kgroups,
+                const unsigned long long *__restrict__ rmax, int bits, int part,
+                const ModConst *__restrict__ mcs, int nmod, int8_t *__restrict__ out1, int s1,
+                int8_t *__restrict__ out2, int s2, int64_t kp, int64_t mod_stride)
+{
+    int64_t r, kg;
+    if (KC) {
+        kg = (int64_t)blockIdx.y * 64 + (threadIdx.x & 63);
+        r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    } else {
+        r = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+        kg = (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
+    }
+    if (r >= R || kg >= kgroups) return;
+    bool bad;
+    const int sigma = row_sigma(rmax[r], bits, bad);
+    double c0[16], c1[16], c2[16];
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+        const int64_t k = kg * 16 + s;
+        double hi = 0.0, lo = 0.0;
+        if (k < K && !bad) comp(KC ? X[r * ld + k] : X[k * ld + r], part, hi, lo);
+        int_chunks(hi, lo, sigma, c0[s], c1[s], c2[s]);
+    }
+    for (int t = 0; t < nmod; ++t) {
+        const ModConst mc = mcs[t];
+        uint32_t w1[4] = {0, 0, 0, 0}, w2[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            const int res = sym_residue(c0[s], c1[s], c2[s], mc);
+            w1[s >> 2] |= ((uint32_t)(res * s1) & 0xffu) << (8 * (s & 3));
+            w2[s >> 2] |= ((uint32_t)(res * s2) & 0xffu) << (8 * (s & 3));
+        }
+        const int64_t off = t * mod_stride + r * kp + kg * 16;
+        *reinterpret_cast<uint4 *>(out1 + off) = make_uint4(w1[0], w1[1], w1[2], w1[3]);
+        if (out2) *reinterpret_cast<uint4 *>(out2 + off) = make_uint4(w2[0], w2[1], w2[2], w2[3]);
+    }
+}
+
+// u = ((v mod m) * y) mod m for an exact INT32 dot product v.
+DDI uint32_t crt_weight_residue(int v, double m, double inv_m, double y)
+{
+    double d = (double)v;
+    double q = floor(d * inv_m);
+    double r = __builtin_fma(-q, m, d);
+    r = (r < 0.0) ? r + m : r;
+    r = (r >= m) ? r - m : r;
+    double w = r * y;
+    q = floor(w * inv_m);
+    double u = __builtin_fma(-q, m, w);
+    u = (u < 0.0) ? u + m : u;
+    u = (u >= m) ? u - m : u;
+    return (uint32_t)u;
+}
+
+// Batched residue GEMM: U_b = ((A_b B_b^T) mod m * y) mod m, A_b: [Mp][kp] int8,
+// B_b: [Np][kp] int8, U_b column-major [Np][ldu] uint8.  Four wave64 per workgroup in a
+// 2 x 2 arrangement, each wave owns TM x TN MFMA tiles of 32 x 32 (v_mfma_i32_32x32x32_i8);
+// 64-deep K slabs double-buffered in LDS with 80-byte row pitch (conflict-free b128 reads).
+template <int TM, int TN>
+__global__ __launch_bounds__(256)
+void k_oz_gemm(const int8_t *__restrict__ A, const int8_t *__restrict__ B,
+               uint8_t *__restrict__ U, int64_t kp, int64_t ldu, int64_t sA, int64_t sB,
+               int64_t sU, const ModConst *__restrict__ mcs, int per, int tiles_m)
+{
+    constexpr int BM = 64 * TM, BN = 64 * TN, PITCH = 80;
+    __shared__ __attribute__((aligned(16))) int8_t sa[2][BM * PITCH];
+    __shared__ __attribute__((aligned(16))) int8_t sb[2][BN * PITCH];
+
+    const int batch = blockIdx.y;
+    const int64_t i0 = (int64_t)(blockIdx.x % tiles_m) * BM;
+    const int64_t j0 = (int64_t)(blockIdx.x / tiles_m) * BN;
+    const int8_t *Ab = A + batch * sA + i0 * kp;
+    const int8_t *Bb = B + batch * sB + j0 * kp;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wm = w & 1, wn = w >> 1;
+
+    uint4 ra[TM], rb[TN];
+    auto gload = [&](int64_t k0) {
+#pragma unroll
+        for (int q = 0; q < TM; ++q) {
+            const int c = tid + 256 * q;
+            ra[q] = *reinterpret_cast<const uint4 *>(Ab + (int64_t)(c >> 2) * kp + k0 + (c & 3) * 16);
+        }
+#pragma unroll
+        for (int q = 0; q < TN; ++q) {
+            const int c = tid + 256 * q;
+            rb[q] = *reinterpret_cast<const uint4 *>(Bb + (int64_t)(c >> 2) * kp + k0 + (c & 3) * 16);
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < TM; ++q) {
+            const int c = tid + 256 * q;
+            *reinterpret_cast<uint4 *>(&sa[buf][(c >> 2) * PITCH + (c & 3) * 16]) = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < TN; ++q) {
+            const int c = tid + 256 * q;
+            *reinterpret_cast<uint4 *>(&sb[buf][(c >> 2) * PITCH + (c & 3) * 16]) = rb[q];
+        }
+    };
+
+    v16i acc[TM][TN];
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b) acc[a][b] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+
+    const int64_t nk = kp / 64;
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int64_t kt = 0; kt < nk; ++kt) {
+        const int cur = (int)(kt & 1);
+        if (kt + 1 < nk) gload((kt + 1) * 64);
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            v4i af[TM], bf[TN];
+#pragma unroll
+            for (int a = 0; a < TM; ++a) {
+                const int row = wm * 32 * TM + a * 32 + (lane & 31);
+                af[a] = *reinterpret_cast<const v4i *>(&sa[cur][row * PITCH + ks * 32 + (lane >> 5) * 16]);
+            }
+#pragma unroll
+            for (int b = 0; b < TN; ++b) {
+                const int col = wn * 32 * TN + b * 32 + (lane & 31);
+                bf[b] = *reinterpret_cast<const v4i *>(&sb[cur][col * PITCH + ks * 32 + (lane >> 5) * 16]);
+            }
+#pragma unroll
+            for (int a = 0; a < TM; ++a)
+#pragma unroll
+                for (int b = 0; b < TN; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[a], bf[b], acc[a][b], 0, 0, 0);
+        }
+        if (kt + 1 < nk) lstore(cur ^ 1);
+        __syncthreads();
+    }
+
+    const ModConst &mc = mcs[batch / per];
+    const double m = mc.m, inv_m = mc.inv_m, y = mc.y;
+    uint8_t *Ub = U + batch * sU;
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b) {
+            const int64_t col = j0 + wn * 32 * TN + b * 32 + (lane & 31);
+            const int64_t rowb = i0 + wm * 32 * TM + a * 32 + 4 * (lane >> 5);
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                uint32_t word = 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    word |= crt_weight_residue(acc[a][b][4 * g + q], m, inv_m, y) << (8 * q);
+                *reinterpret_cast<uint32_t *>(Ub + col * ldu + rowb + 8 * g) = word;
+            }
+        }
+}
+
+// 128-bit fixed-point fraction sum_t u_t * round(2^128/m_t) mod 2^128, from four exact FP64
+// limb accumulators (each < 2^45), as a signed double-double value F (|F| <= 2^127).
+DDI ddv fixed128_to_dd(double a0, double a1, double a2, double a3)
+{
+    double c = floor(a0 * 0x1p-32);
+    const double f0 = __builtin_fma(-c, 0x1p32, a0);
+    a1 += c;
+    c = floor(a1 * 0x1p-32);
+    const double f1 = __builtin_fma(-c, 0x1p32, a1);
+    a2 += c;
+    c = floor(a2 * 0x1p-32);
+    const double f2 = __builtin_fma(-c, 0x1p32, a2);
+    a3 += c;
+    c = floor(a3 * 0x1p-32);
+    double f3 = __builtin_fma(-c, 0x1p32, a3);
+    if (f3 >= 0x1p31) f3 -= 0x1p32;              // symmetric range: F - 2^128
+    ddv v;
+    dd_two_sum(f3 * 0x1p96, f2 * 0x1p64, v.hi, v.lo);
+    v = dd_add_exact(v, ddv{f1 * 0x1p32, 0.0});
+    v = dd_add_exact(v, ddv{f0, 0.0});
+    return v;
+}
+
+DDI ddv reconstruct(const double acc[4], double Mhi, double Mlo, int e, bool bad)
+{
+    if (bad) return ddv{__builtin_nan(""), __builtin_nan("")};
+    const ddv f = fixed128_to_dd(acc[0], acc[1], acc[2], acc[3]);
+    const ddv c = dd_mul_exact(f, ddv{Mhi, Mlo});
+    return ddv{ldexp(c.hi, e), ldexp(c.lo, e)};
+}
+
+// Reconstruction + alpha/beta update.  Each lane owns 4 consecutive rows of one column.
+template <bool CPLX>
+__global__ __launch_bounds__(256)
+void k_oz_crt(const uint8_t *__restrict__ U, int64_t ldu, int64_t sU, int nmod,
+              const ModConst *__restrict__ mcs, double Mhi, double Mlo,
+              const unsigned long long *__restrict__ rmaxA,
+              const unsigned long long *__restrict__ rmaxB, int bits, int64_t m, int64_t n,
+              int64_t blocks_per_col, typename std::conditional<CPLX, zddv, ddv>::type alpha,
+              typename std::conditional<CPLX, zddv, ddv>::type beta, int beta_mode,
+              typename std::conditional<CPLX, zddv, ddv>::type *__restrict__ C, int64_t ldc)
+{
+    constexpr int P = CPLX ? 2 : 1;
+    const int64_t j = (int64_t)blockIdx.x / blocks_per_col;
+    const int64_t i0 = (((int64_t)blockIdx.x % blocks_per_col) * 256 + threadIdx.x) * 4;
+    if (j >= n || i0 >= m) return;
+    double acc[4][P][4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+#pragma unroll
+            for (int l = 0; l < 4; ++l) acc[e][p][l] = 0.0;
+    for (int t = 0; t < nmod; ++t) {
+        const double w0 = mcs[t].w[0], w1 = mcs[t].w[1], w2 = mcs[t].w[2], w3 = mcs[t].w[3];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const uint32_t word =
+                *reinterpret_cast<const uint32_t *>(U + (int64_t)(t * P + p) * sU + j * ldu + i0);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const double u = (double)((word >> (8 * e)) & 0xffu);
+                acc[e][p][0] = __builtin_fma(u, w0, acc[e][p][0]);
+                acc[e][p][1] = __builtin_fma(u, w1, acc[e][p][1]);
+                acc[e][p][2] = __builtin_fma(u, w2, acc[e][p][2]);
+                acc[e][p][3] = __builtin_fma(u, w3, acc[e][p][3]);
+            }
+        }
+    }
+    bool badj;
+    const int tau = row_sigma(rmaxB[j], bits, badj);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int64_t i = i0 + e;
+        if (i >= m) break;
+        bool badi;
+        const int sigma = row_sigma(rmaxA[i], bits, badi);
+        const int ex = -128 - sigma - tau;
+        const bool bad = badi || badj;
+        if constexpr (!CPLX) {
+            const ddv cab = reconstruct(acc[e][0], Mhi, Mlo, ex, bad);
+            ddv *cp = &C[j * ldc + i];
+            const ddv c0 = dd_beta_prepass(*cp, beta, beta_mode);
+            *cp = dd_add_exact(c0, dd_mul_exact(alpha, cab));
+        } else {
+            const zddv cab{reconstruct(acc[e][0], Mhi, Mlo, ex, bad),
+                           reconstruct(acc[e][1], Mhi, Mlo, ex, bad)};
+            zddv *cp = &C[j * ldc + i];
+            const zddv c0 = zdd_beta_prepass(*cp, beta, beta_mode);
+            *cp = zdd_add_exact(c0, zdd_mul_exact(alpha, cab));
+        }
+    }
+}
+
+// ==================================================================================
+// Host orchestration.
+// ==================================================================================
+namespace {
+
+#define OZ_TRY(expr)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return record_error((int)e_, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+struct OzDevice {
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+    std::map<int, ModConst *> tables;   // keyed by modulus count
+};
+std::mutex g_oz_mu;
+std::vector<OzDevice> g_oz;
+
+int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+int oz_device(OzDevice **out)
+{
+    int dev = 0;
+    OZ_TRY(hipGetDevice(&dev));
+    if ((int)g_oz.size() <= dev) g_oz.resize(dev + 1);
+    *out = &g_oz[dev];
+    return 0;
+}
+
+int oz_workspace(OzDevice *d, size_t bytes, hipStream_t s, char **base)
+{
+    if (d->ws_bytes < bytes) {
+        if (d->ws) {
+            OZ_TRY(hipStreamSynchronize(s));
+            OZ_TRY(hipFree(d->ws));
+            d->ws = nullptr;
+            d->ws_bytes = 0;
+        }
+        OZ_TRY(hipMalloc(&d->ws, bytes));
+        d->ws_bytes = bytes;
+    }
+    *base = static_cast<char *>(d->ws);
+    return 0;
+}
+
+int oz_table(OzDevice *d, const OzPlan &p, ModConst **out)
+{
+    auto it = d->tables.find(p.nmod);
+    if (it != d->tables.end()) {
+        *out = it->second;
+        return 0;
+    }
+    ModConst *t = nullptr;
+    OZ_TRY(hipMalloc(&t, sizeof(ModConst) * OZ_MAX_MOD));
+    OZ_TRY(hipMemcpy(t, p.mc, sizeof(ModConst) * p.nmod, hipMemcpyHostToDevice));
+    d->tables[p.nmod] = t;
+    *out = t;
+    return 0;
+}
+
+// Launch the row-maximum scan for an operand whose rows are contiguous in k (KC) or not.
+template <typename T>
+hipError_t launch_rowmax(const T *X, int64_t ld, int64_t R, int64_t K, bool kc,
+                         unsigned long long *rmax, hipStream_t s)
+{
+    if (R <= 0 || K <= 0) return hipSuccess;
+    if (kc) {
+        dim3 grid((unsigned)((R + 3) / 4), (unsigned)((K + 1023) / 1024));
+        hipLaunchKernelGGL(k_oz_rowmax_kc<T>, grid, dim3(256), 0, s, X, ld, R, K, rmax);
+    } else {
+        dim3 grid((unsigned)((R + 63) / 64), (unsigned)((K + 255) / 256));
+        hipLaunchKernelGGL(k_oz_rowmax_rc<T>, grid, dim3(256), 0, s, X, ld, R, K, rmax);
+    }
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_split(const T *X, int64_t ld, int64_t R, int64_t K, int64_t kc_len, bool kc,
+                        const unsigned long long *rmax, int bits, int part, const ModConst *mcs,
+                        int nmod, int8_t *out1, int s1, int8_t *out2, int s2, int64_t kp,
+                        int64_t mod_stride, hipStream_t s)
+{
+    const int64_t kgroups = kc_len / 16;
+    if (kc) {
+        dim3 grid((unsigned)((R + 3) / 4), (unsigned)((kgroups + 63) / 64));
+        hipLaunchKernelGGL((k_oz_split<T, true>), grid, dim3(256), 0, s, X, ld, R, K, kgroups,
+                           rmax, bits, part, mcs, nmod, out1, s1, out2, s2, kp, mod_stride);
+    } else {
+        dim3 grid((unsigned)((R + 63) / 64), (unsigned)((kgroups + 3) / 4));
+        hipLaunchKernelGGL((k_oz_split<T, false>), grid, dim3(256), 0, s, X, ld, R, K, kgroups,
+                           rmax, bits, part, mcs, nmod, out1, s1, out2, s2, kp, mod_stride);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_gemm(const int8_t *A, const int8_t *B, uint8_t *U, int64_t mp, int64_t np,
+                       int64_t kp, int64_t sA, int64_t sB, int64_t sU, const ModConst *mcs,
+                       int per, int batches, hipStream_t s)
+{
+    constexpr int TM = 2, TN = 2;
+    const int tiles_m = (int)(mp / (64 * TM));
+    const int64_t tiles = (int64_t)tiles_m * (np / (64 * TN));
+    dim3 grid((unsigned)tiles, (unsigned)batches);
+    hipLaunchKernelGGL((k_oz_gemm<TM, TN>), grid, dim3(256), 0, s, A, B, U, kp, mp, sA, sB, sU,
+                       mcs, per, tiles_m);
+    return hipGetLastError();
+}
+
+hipError_t fill_zero(unsigned long long *p, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_oz_fill_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+int ozaki_gemm_real(const GemmArgsR &g, hipStream_t s)
+{
+    std::lock_guard<std::mutex> lk(g_oz_mu);
+    OzPlan plan;
+    if (!ozaki_make_plan(g.k, ozaki_target_bits(), ozaki_forced_nmod(), &plan))
+        return record_error(-5, "ozaki: contraction depth %lld out of range", (long long)g.k);
+    OzDevice *d;
+    if (int rc = oz_device(&d)) return rc;
+    ModConst *mcs;
+    if (int rc = oz_table(d, plan, &mcs)) return rc;
+
+    const int64_t Mp = round_up(g.m, 256), Np = round_up(g.n, 256), Kc = round_up(g.k, 64);
+    const int nmod = plan.nmod;
+    const size_t szA = (size_t)nmod * Mp * Kc, szB = (size_t)nmod * Np * Kc;
+    const size_t szU = (size_t)nmod * Np * Mp, szR = (size_t)(Mp + Np) * 8;
+    char *base;
+    if (int rc = oz_workspace(d, szR + szA + szB + szU + 1024, s, &base)) return rc;
+    unsigned long long *rA = reinterpret_cast<unsigned long long *>(base);
+    unsigned long long *rB = rA + Mp;
+    int8_t *resA = reinterpret_cast<int8_t *>(base + round_up(szR, 256));
+    int8_t *resB = resA + round_up(szA, 256);
+    uint8_t *Ub = reinterpret_cast<uint8_t *>(resB + round_up(szB, 256));
+
+    const bool a_kc = g.opa != OP_N;   // op(A) row i contiguous in k when A is transposed
+    const bool b_kc = g.opb == OP_N;   // op(B) column j contiguous in k when B is not
+    {
+        ScopedTimer tm(KC_OZ_EXPONENT, s);
+        OZ_TRY(fill_zero(rA, Mp + Np, s));
+        OZ_TRY(launch_rowmax(g.A, g.lda, g.m, g.k, a_kc, rA, s));
+        OZ_TRY(launch_rowmax(g.B, g.ldb, g.n, g.k, b_kc, rB, s));
+    }
+    {
+        ScopedTimer tm(KC_OZ_SPLIT, s);
+        OZ_TRY(launch_split(g.A, g.lda, g.m, g.k, Kc, a_kc, rA, plan.bits, 0, mcs, nmod, resA, 1,
+                            (int8_t *)nullptr, 0, Kc, Mp * Kc, s));
+        OZ_TRY(launch_split(g.B, g.ldb, g.n, g.k, Kc, b_kc, rB, plan.bits, 0, mcs, nmod, resB, 1,
+                            (int8_t *)nullptr, 0, Kc, Np * Kc, s));
+    }
+    {
+        ScopedTimer tm(KC_OZ_GEMM, s);
+        OZ_TRY(launch_gemm(resA, resB, Ub, Mp, Np, Kc, Mp * Kc, Np * Kc, Np * Mp, mcs, 1, nmod, s));
+    }
+    {
+        ScopedTimer tm(KC_OZ_CRT, s);
+        const int64_t bpc = (g.m + 1023) / 1024;
+        hipLaunchKernelGGL((k_oz_crt<false>), dim3((unsigned)(bpc * g.n)), dim3(256), 0, s, Ub, Mp,
+                           Np * Mp, nmod, mcs, plan.Mhi, plan.Mlo, rA, rB, plan.bits, g.m, g.n,
+                           bpc, g.alpha, g.beta, g.beta_mode, g.C, g.ldc);
+        OZ_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+int ozaki_gemm_cplx(const GemmArgsZ &g, hipStream_t s)
+{
+    std::lock_guard<std::mutex> lk(g_oz_mu);
+    OzPlan plan;
+    if (!ozaki_make_plan(2 * g.k, ozaki_target_bits(), ozaki_forced_nmod(), &plan))
+        return record_error(-5, "ozaki: contraction depth %lld out of range", (long long)g.k);
+    OzDevice *d;
+    if (int rc = oz_device(&d)) return rc;
+    ModConst *mcs;
+    if (int rc = oz_table(d, plan, &mcs)) return rc;
+
+    const int64_t Mp = round_up(g.m, 256), Np = round_up(g.n, 256), Kc = round_up(g.k, 64);
+    const int64_t Kp = 2 * Kc;
+    const int nmod = plan.nmod;
+    // batch b = 2t + part (part 0: real part of C, 1: imaginary part)
+    const size_t szA = (size_t)2 * nmod * Mp * Kp, szB = (size_t)2 * nmod * Np * Kp;
+    const size_t szU = (size_t)2 * nmod * Np * Mp, szR = (size_t)(Mp + Np) * 8;
+    char *base;
+    if (int rc = oz_workspace(d, szR + szA + szB + szU + 1024, s, &base)) return rc;
+    unsigned long long *rA = reinterpret_cast<unsigned long long *>(base);
+    unsigned long long *rB = rA + Mp;
+    int8_t *resA = reinterpret_cast<int8_t *>(base + round_up(szR, 256));
+    int8_t *resB = resA + round_up(szA, 256);
+    uint8_t *Ub = reinterpret_cast<uint8_t *>(resB + round_up(szB, 256));
+    const int64_t bsA = Mp * Kp, bsB = Np * Kp;   // per-batch strides
+
+    const bool a_kc = g.opa != OP_N, b_kc = g.opb == OP_N;
+    const int sa = g.opa == OP_C ? -1 : 1, sb = g.opb == OP_C ? -1 : 1;
+    {
+        ScopedTimer tm(KC_OZ_EXPONENT, s);
+        OZ_TRY(fill_zero(rA, Mp + Np, s));
+        OZ_TRY(launch_rowmax(g.A, g.lda, g.m, g.k, a_kc, rA, s));
+        OZ_TRY(launch_rowmax(g.B, g.ldb, g.n, g.k, b_kc, rB, s));
+    }
+    {
+        ScopedTimer tm(KC_OZ_SPLIT, s);
+        // A: part 0 row = [Ar | -Ai], part 1 row = [Ar | Ai]
+        OZ_TRY(launch_split(g.A, g.lda, g.m, g.k, Kc, a_kc, rA, plan.bits, 0, mcs, nmod,
+                            resA, 1, resA + bsA, 1, Kp, 2 * bsA, s));
+        OZ_TRY(launch_split(g.A, g.lda, g.m, g.k, Kc, a_kc, rA, plan.bits, 1, mcs, nmod,
+                            resA + Kc, -sa, resA + bsA + Kc, sa, Kp, 2 * bsA, s));
+        // B: part 0 column = [Br ; Bi], part 1 column = [Bi ; Br]
+        OZ_TRY(launch_split(g.B, g.ldb, g.n, g.k, Kc, b_kc, rB, plan.bits, 0, mcs, nmod,
+                            resB, 1, resB + bsB + Kc, 1, Kp, 2 * bsB, s));
+        OZ_TRY(launch_split(g.B, g.ldb, g.n, g.k, Kc, b_kc, rB, plan.bits, 1, mcs, nmod,
+                            resB + Kc, sb, resB + bsB, sb, Kp, 2 * bsB, s));
+    }
+    {
+        ScopedTimer tm(KC_OZ_GEMM, s);
+        OZ_TRY(launch_gemm(resA, resB, Ub, Mp, Np, Kp, bsA, bsB, Np * Mp, mcs, 2, 2 * nmod, s));
+    }
+    {
+        ScopedTimer tm(KC_OZ_CRT, s);
+        const int64_t bpc = (g.m + 1023) / 1024;
+        hipLaunchKernelGGL((k_oz_crt<true>), dim3((unsigned)(bpc * g.n)), dim3(256), 0, s, Ub, Mp,
+                           Np * Mp, nmod, mcs, plan.Mhi, plan.Mlo, rA, rB, plan.bits, g.m, g.n,
+                           bpc, g.alpha, g.beta, g.beta_mode, g.C, g.ldc);
+        OZ_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+int ozaki_residue_gemm_test(const int8_t *dA, const int8_t *dB, uint8_t *dU, int64_t mp,
+                            int64_t np, int64_t kp, int mod_index, hipStream_t s)
+{
+    std::lock_guard<std::mutex> lk(g_oz_mu);
+    if (mp % 256 || np % 256 || kp % 64 || kp <= 0 || kp > OZ_MAX_KEFF || mod_index < 0 ||
+        mod_index >= OZ_MAX_MOD)
+        return record_error(-3, "residue gemm test: bad shape");
+    OzPlan plan;
+    if (!ozaki_make_plan(1, 8, OZ_MAX_MOD, &plan)) return record_error(-5, "plan");
+    OzDevice *d;
+    if (int rc = oz_device(&d)) return rc;
+    ModConst *mcs;
+    if (int rc = oz_table(d, plan, &mcs)) return rc;
+    OZ_TRY(launch_gemm(dA, dB, dU, mp, np, kp, 0, 0, 0, mcs + mod_index, 1, 1, s));
+    return 0;
+}
+
+}  // namespace mw

@@ -1,4 +1,5 @@
-// ozaki.hpp -- the Ozaki-scheme-II double-double GEMM (INT8 residues on the matrix cores).
+// ozaki.hpp -- the exact-product double-double GEMM (Ozaki scheme II: integer residues on
+// the INT8 matrix cores, Chinese-remainder reconstruction in FP64).
 #pragma once
 #include "mw_internal.hpp"
 
@@ -6,10 +7,48 @@ namespace mw {
 
 int record_error(int code, const char *fmt, ...);
 
-// True when the exact-product path handles a contraction depth of k.
+constexpr int OZ_MAX_MOD = 40;
+
+// Per-modulus constants, read by every Ozaki kernel (uniform across a wave).
+struct ModConst {
+    double m, inv_m;      // modulus and 1/m (rounded)
+    double y;             // ((M/m)^-1 mod m): CRT weight numerator
+    double p1, p2;        // 2^35 mod m, 2^70 mod m: chunk weights for the residue split
+    double w[4];          // round(2^128/m) as four 32-bit limbs (w[0] least significant)
+};
+
+// One reconstruction configuration: the first nmod moduli of the table, and the integer
+// width b every scaled operand row/column is rounded to (|A'|, |B'| <= 2^b).
+struct OzPlan {
+    int nmod;
+    int bits;
+    double log2M;
+    double Mhi, Mlo;      // product of the moduli as a double-double
+    ModConst mc[OZ_MAX_MOD];
+};
+
+// Choose the plan for a contraction depth k (k_eff = k for real, 2k for complex): the
+// fewest moduli whose product M satisfies M/2 > k_eff * 2^(2b) with b >= target bits.
+// Returns false when k is out of range.
+bool ozaki_make_plan(int64_t k_eff, int target_bits, int force_nmod, OzPlan *plan);
+
+// Largest contraction depth the INT32 accumulation keeps exact (k * 128^2 < 2^31).
+constexpr int64_t OZ_MAX_KEFF = 65536;
+
 bool ozaki_supported(int64_t k);
-// C := alpha*op(A)*op(B) + beta*C through the residue pipeline; returns 0 or an error code.
+bool ozaki_supported_cplx(int64_t k);
 int ozaki_gemm_real(const GemmArgsR &g, hipStream_t s);
 int ozaki_gemm_cplx(const GemmArgsZ &g, hipStream_t s);
+
+// Test hook: U[j*ldu + i] = ((sum_k A[i*kp + k] * B[j*kp + k]) mod m * y) mod m for one
+// modulus of the table, computed by the production INT8 MFMA kernel.  mp and np must be
+// multiples of 256, kp a multiple of 64.
+int ozaki_residue_gemm_test(const int8_t *dA, const int8_t *dB, uint8_t *dU, int64_t mp,
+                            int64_t np, int64_t kp, int mod_index, hipStream_t s);
+
+// Current target bit width and a forced modulus count (0 = automatic).
+int ozaki_target_bits();
+int ozaki_forced_nmod();
+void ozaki_set_params(int bits, int nmod);
 
 }  // namespace mw

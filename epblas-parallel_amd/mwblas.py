@@ -209,3 +209,48 @@ def timer_read():
     n = lib().mwgemm_timer_read(_ptr(ms), _ptr(cnt), cap, names, 2048)
     labels = names.raw.split(b"\0")[:n]
     return {labels[i].decode(): (float(ms[i]), int(cnt[i])) for i in range(n) if cnt[i]}
+
+
+def ozaki_config(k, cplx=False):
+    """(number of moduli, integer width b) the exact-product path uses at depth k."""
+    L = lib()
+    f = L.mwgemm_ozaki_config
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int64, ctypes.c_int, _vp, _vp]
+    nm, b = ctypes.c_int(), ctypes.c_int()
+    if f(k, 1 if cplx else 0, ctypes.byref(nm), ctypes.byref(b)) != 0:
+        raise MwgemmError(f"no Ozaki plan for k={k}")
+    return nm.value, b.value
+
+
+def ozaki_set(bits=0, nmod=0):
+    """Target integer width (0 = default 103) and forced modulus count (0 = automatic)."""
+    f = lib().mwgemm_ozaki_set
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int, ctypes.c_int]
+    if f(bits, nmod) != 0:
+        raise ValueError("bad Ozaki parameters")
+
+
+def ozaki_moduli(nmod):
+    """The first nmod moduli and their CRT weights y_t for that set."""
+    f = lib().mwgemm_ozaki_moduli
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int, _vp, _vp]
+    m = np.zeros(nmod, dtype=np.int32)
+    y = np.zeros(nmod, dtype=np.int32)
+    if f(nmod, _ptr(m), _ptr(y)) != 0:
+        raise ValueError("bad modulus count")
+    return m, y
+
+
+def testing_residue_gemm(dA, dB, dU, mp, np_, kp, mod_index, stream=None):
+    """Run the production INT8 residue GEMM kernel once (test hook)."""
+    f = lib().mwgemm_testing_residue_gemm
+    f.restype = ctypes.c_int
+    f.argtypes = [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
+                  _vp]
+    rc = f(_ptr(dA), _ptr(dB), _ptr(dU), mp, np_, kp, mod_index,
+           None if stream is None else ctypes.c_void_p(stream))
+    if rc != 0:
+        _check(rc)

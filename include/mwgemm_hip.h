@@ -112,6 +112,23 @@ MWGEMM_API int mwgemm_wgemm_device(char transa, char transb, int64_t m, int64_t 
                         const mw_complex64x2 *dB, int64_t ldb, const mw_complex64x2 *beta,
                         mw_complex64x2 *dC, int64_t ldc, int path, void *stream);
 
+/* Ozaki path parameters.  mwgemm_ozaki_config reports, for contraction depth k (real or
+ * complex), how many moduli the exact-product path uses and the integer width b every
+ * scaled row/column is rounded to (normwise error bound ~4*2^-b).  mwgemm_ozaki_set sets the
+ * target width (0 = default 103; also MWGEMM_OZAKI_BITS) and optionally forces the modulus
+ * count (0 = automatic; also MWGEMM_OZAKI_NMOD).  mwgemm_ozaki_moduli lists the first nmod
+ * moduli and their CRT weights y_t = ((M/m_t)^-1 mod m_t) for that set.  Return 0 or -1. */
+MWGEMM_API int mwgemm_ozaki_config(int64_t k, int cplx, int *nmod, int *bits);
+MWGEMM_API int mwgemm_ozaki_set(int bits, int nmod);
+MWGEMM_API int mwgemm_ozaki_moduli(int nmod, int *m, int *y);
+
+/* Test hook: runs the production INT8 MFMA residue kernel once on device buffers,
+ * U[j*mp + i] = ((sum_k A[i*kp + k] * B[j*kp + k]) mod m * y) mod m with m, y the
+ * mod_index-th modulus and weight of the full 40-modulus table.  mp, np multiples of 256,
+ * kp a multiple of 64.  Returns 0 or an error code. */
+MWGEMM_API int mwgemm_testing_residue_gemm(const void *dA, const void *dB, void *dU, int64_t mp,
+                                           int64_t np, int64_t kp, int mod_index, void *stream);
+
 /* Kernel timers: while enabled, every kernel the library launches is bracketed by HIP
  * events on its own stream.  mwgemm_timer_read synchronises, then writes up to cap
  * entries of {total milliseconds, launch count} per kernel class into ms/count and their

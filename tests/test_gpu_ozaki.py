@@ -1,0 +1,163 @@
+"""Exact-product (Ozaki scheme II, INT8 MFMA) GPU path.
+
+Bars: the INT8 residue kernel is exact (bitwise against integer arithmetic); every DD GEMM
+result is within normwise error 1e-30 of the binary128 oracle (and within the analytic
+bound 4*2^-b of the configured width b); integer operands give the exact integer product."""
+import numpy as np
+import pytest
+
+import ddgen
+import mwblas
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+BAR = 1e-30
+
+
+@pytest.fixture
+def ozaki_path():
+    prev = mwblas.set_path(mwblas.PATH_OZAKI)
+    yield
+    mwblas.set_path(prev)
+
+
+@pytest.mark.parametrize("mod_index", [0, 1, 5, 28, 39])
+def test_residue_gemm_kernel_is_exact(gpu, mod_index):
+    import torch
+    rng = np.random.default_rng(100 + mod_index)
+    mp, np_, kp = 256, 512, 320
+    m_all, y_all = mwblas.ozaki_moduli(40)
+    m, y = int(m_all[mod_index]), int(y_all[mod_index])
+    lo, hi = (-128, 127) if m == 256 else (-(m - 1) // 2, (m - 1) // 2)
+    A = rng.integers(lo, hi + 1, size=(mp, kp), dtype=np.int64)
+    B = rng.integers(lo, hi + 1, size=(np_, kp), dtype=np.int64)
+    dA = torch.from_numpy(A.astype(np.int8)).to(gpu)
+    dB = torch.from_numpy(B.astype(np.int8)).to(gpu)
+    dU = torch.zeros((np_, mp), dtype=torch.uint8, device=gpu)
+    mwblas.testing_residue_gemm(dA, dB, dU, mp, np_, kp, mod_index)
+    torch.cuda.synchronize()
+    expect = (((A @ B.T) % m) * y) % m            # (mp, np)
+    got = dU.cpu().numpy().astype(np.int64).T
+    assert np.array_equal(got, expect)
+
+
+def _check(cplx, ta, tb, m, n, k, kind, rng, ent=None):
+    gen = ddgen.cplx_matrix if cplx else ddgen.real_matrix
+    ar, ac = (m, k) if ta == "N" else (k, m)
+    br, bc = (k, n) if tb == "N" else (n, k)
+    lda, ldb, ldc = ar + 1, br + 2, m + 3
+    A, B = gen(rng, ar, ac, lda, kind=kind), gen(rng, br, bc, ldb, kind=kind)
+    C0 = gen(rng, m, n, ldc)
+    al = ddgen.cplx_scalar(rng, scale=2) if cplx else ddgen.real_scalar(rng, scale=2)
+    be = ddgen.cplx_scalar(rng, scale=2) if cplx else ddgen.real_scalar(rng, scale=2)
+    C = C0.copy()
+    (mwblas.wgemm if cplx else mwblas.mgemm)(ta, tb, m, n, k, al, A, lda, B, ldb, be, C, ldc)
+    assert np.array_equal(C[:, m:, :], C0[:, m:, :])       # padding rows untouched
+    r = oracle.q128_gemm_err(cplx, ta, tb, m, n, k, al, A, lda, B, ldb, be, C0, ldc, C, ldc,
+                             entries=ent)
+    _, bits = mwblas.ozaki_config(k, cplx)
+    print(f"{'z' if cplx else 'r'} {ta}{tb} {m}x{n}x{k} {kind}: err={r['err']:.3e}", flush=True)
+    assert r["err"] <= BAR
+    assert r["err"] <= 4 * 2.0 ** -bits
+    return r
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
+@pytest.mark.parametrize("kind", ["uniform", "wide"])
+def test_real_vs_binary128(gpu, ozaki_path, ta, tb, kind):
+    rng = np.random.default_rng(11)
+    _check(False, ta, tb, 300, 211, 517, kind, rng)
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("C", "N"), ("N", "C"), ("T", "C"), ("C", "T"),
+                                   ("C", "C")])
+def test_cplx_vs_binary128(gpu, ozaki_path, ta, tb):
+    rng = np.random.default_rng(12)
+    _check(True, ta, tb, 150, 133, 259, "uniform", rng)
+
+
+def test_small_and_ragged_shapes(gpu, ozaki_path):
+    rng = np.random.default_rng(13)
+    for (m, n, k) in [(1, 1, 1), (3, 257, 2), (255, 1, 65), (17, 33, 1000)]:
+        _check(False, "N", "N", m, n, k, "wide", rng)
+        _check(True, "C", "N", m, n, k, "uniform", rng)
+
+
+def test_integer_operands(gpu, ozaki_path):
+    """Integer operands: the high limbs carry the exact integer product, and the whole
+    result is within the normwise bar of it (the reconstruction rounds C' = A'B' through a
+    double-double multiply, so low limbs of ~2^-104 relative may remain)."""
+    rng = np.random.default_rng(14)
+    m, n, k = 70, 90, 600
+    A = ddgen.real_matrix(rng, m, k, kind="integer", bound=1000)
+    B = ddgen.real_matrix(rng, k, n, kind="integer", bound=1000)
+    C = np.zeros((n, m, 2))
+    mwblas.mgemm("N", "N", m, n, k, [1, 0], A, m, B, k, [0, 0], C, m)
+    exact = A[..., 0].T.astype(np.int64) @ B[..., 0].T.astype(np.int64)
+    assert np.array_equal(C[..., 0].T, exact.astype(np.float64))
+    opn = k * 1000.0 * 1000.0
+    assert np.max(np.abs(C[..., 1])) <= 1e-30 * opn
+
+
+def test_special_values(gpu, ozaki_path):
+    rng = np.random.default_rng(15)
+    m, n, k = 40, 30, 50
+    A, B = ddgen.real_matrix(rng, m, k), ddgen.real_matrix(rng, k, n)
+    A[:, 5, :] = 0.0                    # a zero row of A
+    A[7, 9, 0] = np.nan                 # row 9 of A carries a NaN
+    C0 = ddgen.real_matrix(rng, m, n)
+    C0[2, 3, 0] = np.nan
+    C = C0.copy()
+    mwblas.mgemm("N", "N", m, n, k, [1, 0], A, m, B, k, [0, 0], C, m)
+    assert np.all(np.isnan(C[:, 9, 0]))                      # NaN row propagates
+    assert np.all(C[:, 5, :] == 0.0)                          # zero row is exact zero
+    mask = np.ones(m, bool)
+    mask[9] = False
+    assert np.all(np.isfinite(C[:, mask, :]))                 # beta = 0 cleared C0's NaN
+
+
+def test_8192_wide_exponents_vs_binary128_and_plain(gpu):
+    """The headline workload: M=N=K=8192, hi = +-U[1,2)*2^U_int[-20,20], alpha=1, beta=0.
+    Ozaki and plain GPU results: 256 sampled entries each within 1e-30 of binary128."""
+    import torch
+    rng = np.random.default_rng(8192)
+    m = n = k = 8192
+    A = ddgen.real_matrix(rng, m, k, kind="wide")
+    B = ddgen.real_matrix(rng, k, n, kind="wide")
+    C0 = np.zeros((n, m, 2))
+    al, be = np.array([1.0, 0.0]), np.zeros(2)
+    dA, dB = torch.from_numpy(A).to(gpu), torch.from_numpy(B).to(gpu)
+    ent = (rng.integers(0, m, 256), rng.integers(0, n, 256))
+    for path in (mwblas.PATH_OZAKI, mwblas.PATH_PLAIN):
+        dC = torch.zeros((n, m, 2), dtype=torch.float64, device=gpu)
+        mwblas.gemm_device(False, "N", "N", m, n, k, al, dA, m, dB, k, be, dC, m, path=path)
+        torch.cuda.synchronize()
+        C = dC.cpu().numpy()
+        r = oracle.q128_gemm_err(False, "N", "N", m, n, k, al, A, m, B, k, be, C0, m, C, m,
+                                 entries=ent)
+        print(f"8192 path={path} err={r['err']:.3e}", flush=True)
+        assert r["err"] <= BAR
+
+
+def test_4096_complex_conj_vs_binary128(gpu):
+    """Complex workload: M=N=K=4096, op(A) = conjugate transpose, op(B) = N, random complex
+    DD alpha/beta; plain and Ozaki, 128 sampled entries against binary128."""
+    import torch
+    rng = np.random.default_rng(4097)
+    m = n = k = 4096
+    A = ddgen.cplx_matrix(rng, k, m)
+    B = ddgen.cplx_matrix(rng, k, n)
+    C0 = ddgen.cplx_matrix(rng, m, n)
+    al, be = ddgen.cplx_scalar(rng), ddgen.cplx_scalar(rng)
+    dA, dB = torch.from_numpy(A).to(gpu), torch.from_numpy(B).to(gpu)
+    ent = (rng.integers(0, m, 128), rng.integers(0, n, 128))
+    for path in (mwblas.PATH_OZAKI, mwblas.PATH_PLAIN):
+        dC = torch.from_numpy(C0).to(gpu)
+        mwblas.gemm_device(True, "C", "N", m, n, k, al, dA, k, dB, k, be, dC, m, path=path)
+        torch.cuda.synchronize()
+        C = dC.cpu().numpy()
+        r = oracle.q128_gemm_err(True, "C", "N", m, n, k, al, A, k, B, k, be, C0, m, C, m,
+                                 entries=ent)
+        print(f"4096 complex path={path} err={r['err']:.3e}", flush=True)
+        assert r["err"] <= BAR
