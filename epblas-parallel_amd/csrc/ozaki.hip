@@ -362,12 +362,30 @@ DDI uint32_t crt_weight_residue(int v, double m, double inv_m, double y)
     return (uint32_t)u;
 }
 
+// Workgroup -> output tile.  Workgroups are dispatched round-robin over the 8 XCDs (each
+// with its own L2), so ids are first regrouped so that one XCD receives a contiguous run;
+// runs are then walked in bands of GROUP tile-rows, column by column, so the workgroups
+// resident on one XCD share A row-slabs and B column-slabs in its L2.
+DDI void tile_order(int bid, int tiles_m, int tiles_n, int &tm, int &tn)
+{
+    constexpr int XCD = 8, GROUP = 4;
+    const int T = tiles_m * tiles_n;
+    int id = bid;
+    if (T % XCD == 0) id = (bid % XCD) * (T / XCD) + bid / XCD;
+    const int band = GROUP * tiles_n;
+    const int first_m = (id / band) * GROUP;
+    const int gm = (tiles_m - first_m < GROUP) ? tiles_m - first_m : GROUP;
+    const int r = id % band;
+    tm = first_m + r % gm;
+    tn = r / gm;
+}
+
 // Batched residue GEMM: U_b = ((A_b B_b^T) mod m * y) mod m, A_b: [Mp][kp] int8,
 // B_b: [Np][kp] int8, U_b column-major [Np][ldu] uint8.  Four wave64 per workgroup in a
 // 2 x 2 arrangement, each wave owns TM x TN MFMA tiles of 32 x 32 (v_mfma_i32_32x32x32_i8);
 // 64-deep K slabs double-buffered in LDS with 80-byte row pitch (conflict-free b128 reads).
 template <int TM, int TN>
-__global__ __launch_bounds__(256)
+__global__ __launch_bounds__(256, TM * TN >= 16 ? 1 : 2)
 void k_oz_gemm(const int8_t *__restrict__ A, const int8_t *__restrict__ B,
                uint8_t *__restrict__ U, int64_t kp, int64_t ldu, int64_t sA, int64_t sB,
                int64_t sU, const ModConst *__restrict__ mcs, int per, int tiles_m)
@@ -377,38 +395,38 @@ void k_oz_gemm(const int8_t *__restrict__ A, const int8_t *__restrict__ B,
     __shared__ __attribute__((aligned(16))) int8_t sb[2][BN * PITCH];
 
     const int batch = blockIdx.y;
-    const int64_t i0 = (int64_t)(blockIdx.x % tiles_m) * BM;
-    const int64_t j0 = (int64_t)(blockIdx.x / tiles_m) * BN;
+    int tm, tn;
+    tile_order(blockIdx.x, tiles_m, (int)gridDim.x / tiles_m, tm, tn);
+    const int64_t i0 = (int64_t)tm * BM;
+    const int64_t j0 = (int64_t)tn * BN;
     const int8_t *Ab = A + batch * sA + i0 * kp;
     const int8_t *Bb = B + batch * sB + j0 * kp;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wm = w & 1, wn = w >> 1;
 
-    uint4 ra[TM], rb[TN];
-    auto gload = [&](int64_t k0) {
-#pragma unroll
-        for (int q = 0; q < TM; ++q) {
-            const int c = tid + 256 * q;
-            ra[q] = *reinterpret_cast<const uint4 *>(Ab + (int64_t)(c >> 2) * kp + k0 + (c & 3) * 16);
-        }
-#pragma unroll
-        for (int q = 0; q < TN; ++q) {
-            const int c = tid + 256 * q;
-            rb[q] = *reinterpret_cast<const uint4 *>(Bb + (int64_t)(c >> 2) * kp + k0 + (c & 3) * 16);
-        }
-    };
-    auto lstore = [&](int buf) {
-#pragma unroll
-        for (int q = 0; q < TM; ++q) {
-            const int c = tid + 256 * q;
-            *reinterpret_cast<uint4 *>(&sa[buf][(c >> 2) * PITCH + (c & 3) * 16]) = ra[q];
-        }
-#pragma unroll
-        for (int q = 0; q < TN; ++q) {
-            const int c = tid + 256 * q;
-            *reinterpret_cast<uint4 *>(&sb[buf][(c >> 2) * PITCH + (c & 3) * 16]) = rb[q];
-        }
-    };
+    v4i ra[TM], rb[TN];
+#define OZ_GLOAD(K0)                                                                          \
+    do {                                                                                      \
+        _Pragma("unroll") for (int q = 0; q < TM; ++q) {                                      \
+            const int c = tid + 256 * q;                                                      \
+            ra[q] = *reinterpret_cast<const v4i *>(Ab + (int64_t)(c >> 2) * kp + (K0) + (c & 3) * 16); \
+        }                                                                                     \
+        _Pragma("unroll") for (int q = 0; q < TN; ++q) {                                      \
+            const int c = tid + 256 * q;                                                      \
+            rb[q] = *reinterpret_cast<const v4i *>(Bb + (int64_t)(c >> 2) * kp + (K0) + (c & 3) * 16); \
+        }                                                                                     \
+    } while (0)
+#define OZ_LSTORE(BUF)                                                                        \
+    do {                                                                                      \
+        _Pragma("unroll") for (int q = 0; q < TM; ++q) {                                      \
+            const int c = tid + 256 * q;                                                      \
+            *reinterpret_cast<v4i *>(&sa[BUF][(c >> 2) * PITCH + (c & 3) * 16]) = ra[q];     \
+        }                                                                                     \
+        _Pragma("unroll") for (int q = 0; q < TN; ++q) {                                      \
+            const int c = tid + 256 * q;                                                      \
+            *reinterpret_cast<v4i *>(&sb[BUF][(c >> 2) * PITCH + (c & 3) * 16]) = rb[q];     \
+        }                                                                                     \
+    } while (0)
 
     v16i acc[TM][TN];
 #pragma unroll
@@ -417,12 +435,12 @@ void k_oz_gemm(const int8_t *__restrict__ A, const int8_t *__restrict__ B,
         for (int b = 0; b < TN; ++b) acc[a][b] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 
     const int64_t nk = kp / 64;
-    gload(0);
-    lstore(0);
+    OZ_GLOAD(0);
+    OZ_LSTORE(0);
     __syncthreads();
     for (int64_t kt = 0; kt < nk; ++kt) {
         const int cur = (int)(kt & 1);
-        if (kt + 1 < nk) gload((kt + 1) * 64);
+        if (kt + 1 < nk) OZ_GLOAD((kt + 1) * 64);
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
             v4i af[TM], bf[TN];
@@ -442,10 +460,12 @@ void k_oz_gemm(const int8_t *__restrict__ A, const int8_t *__restrict__ B,
                 for (int b = 0; b < TN; ++b)
                     acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[a], bf[b], acc[a][b], 0, 0, 0);
         }
-        if (kt + 1 < nk) lstore(cur ^ 1);
+        if (kt + 1 < nk) OZ_LSTORE(cur ^ 1);
         __syncthreads();
     }
 
+#undef OZ_GLOAD
+#undef OZ_LSTORE
     const ModConst &mc = mcs[batch / per];
     const double m = mc.m, inv_m = mc.inv_m, y = mc.y;
     uint8_t *Ub = U + batch * sU;
@@ -661,12 +681,19 @@ hipError_t launch_gemm(const int8_t *A, const int8_t *B, uint8_t *U, int64_t mp,
                        int64_t kp, int64_t sA, int64_t sB, int64_t sU, const ModConst *mcs,
                        int per, int batches, hipStream_t s)
 {
-    constexpr int TM = 2, TN = 2;
-    const int tiles_m = (int)(mp / (64 * TM));
-    const int64_t tiles = (int64_t)tiles_m * (np / (64 * TN));
-    dim3 grid((unsigned)tiles, (unsigned)batches);
-    hipLaunchKernelGGL((k_oz_gemm<TM, TN>), grid, dim3(256), 0, s, A, B, U, kp, mp, sA, sB, sU,
-                       mcs, per, tiles_m);
+    // 256 x 256 tiles when there are enough of them to fill the chip, else 128 x 128.
+    const bool big = (mp / 256) * (np / 256) * (int64_t)batches >= 512 && getenv("MWGEMM_OZ_SMALL_TILE") == nullptr;
+    if (big) {
+        const int tiles_m = (int)(mp / 256);
+        dim3 grid((unsigned)(tiles_m * (np / 256)), (unsigned)batches);
+        hipLaunchKernelGGL((k_oz_gemm<4, 4>), grid, dim3(256), 0, s, A, B, U, kp, mp, sA, sB, sU,
+                           mcs, per, tiles_m);
+    } else {
+        const int tiles_m = (int)(mp / 128);
+        dim3 grid((unsigned)(tiles_m * (np / 128)), (unsigned)batches);
+        hipLaunchKernelGGL((k_oz_gemm<2, 2>), grid, dim3(256), 0, s, A, B, U, kp, mp, sA, sB, sU,
+                           mcs, per, tiles_m);
+    }
     return hipGetLastError();
 }
 

@@ -26,14 +26,22 @@ constexpr int KCHUNK = 128;   // reference KC: the accumulator flush period
 // Tile staging helpers (generic over element type T = ddv or zddv).
 // ----------------------------------------------------------------------------------
 
-DDI ddv conj_elem(ddv x) { return x; }
-DDI zddv conj_elem(zddv x) { return zdd_conj(x); }
+// Staging registers hold native FP64 vectors (a DD as double2, a complex DD as double4)
+// so they stay in VGPRs; arrays of the element structs end up in scratch memory.
+typedef double v2d __attribute__((ext_vector_type(2)));
+typedef double v4d __attribute__((ext_vector_type(4)));
+template <typename T> struct VecOf;
+template <> struct VecOf<ddv> { typedef v2d type; };
+template <> struct VecOf<zddv> { typedef v4d type; };
+DDI v2d conj_vec(v2d x) { return x; }
+DDI v4d conj_vec(v4d x) { return v4d{x.x, x.y, -x.z, -x.w}; }
 
 // op(A) slab [i0, i0+BM) x [k0, k0+BK) -> regs.  TRANS: A is stored K x M.
 template <typename T, int BM, int BK, bool TRANS, bool CONJ>
 struct SlabA {
+    typedef typename VecOf<T>::type V;
     static constexpr int PER = BM * BK / 256;
-    T v[PER];
+    V v[PER];
     DDI void load(const T *__restrict__ A, int64_t lda, int64_t m, int64_t k, int64_t i0,
                   int64_t k0, int tid)
     {
@@ -44,10 +52,10 @@ struct SlabA {
             if (!TRANS) { r = e % BM; c = e / BM; }
             else        { c = e % BK; r = e / BK; }
             int64_t gi = i0 + r, gk = k0 + c;
-            T x{};
+            V x = V(0.0);
             if (gi < m && gk < k) {
-                x = TRANS ? A[gi * lda + gk] : A[gk * lda + gi];
-                if constexpr (CONJ) x = conj_elem(x);
+                x = *reinterpret_cast<const V *>(TRANS ? &A[gi * lda + gk] : &A[gk * lda + gi]);
+                if constexpr (CONJ) x = conj_vec(x);
             }
             v[q] = x;
         }
@@ -61,7 +69,7 @@ struct SlabA {
             int r, c;
             if (!TRANS) { r = e % BM; c = e / BM; }
             else        { c = e % BK; r = e / BK; }
-            S[c][r] = v[q];
+            *reinterpret_cast<V *>(&S[c][r]) = v[q];
         }
     }
 };
@@ -69,8 +77,9 @@ struct SlabA {
 // op(B) slab [k0, k0+BK) x [j0, j0+BN) -> regs.  TRANS: B is stored N x K.
 template <typename T, int BN, int BK, bool TRANS, bool CONJ>
 struct SlabB {
+    typedef typename VecOf<T>::type V;
     static constexpr int PER = BN * BK / 256;
-    T v[PER];
+    V v[PER];
     DDI void load(const T *__restrict__ B, int64_t ldb, int64_t n, int64_t k, int64_t j0,
                   int64_t k0, int tid)
     {
@@ -81,10 +90,10 @@ struct SlabB {
             if (!TRANS) { c = e % BK; r = e / BK; }
             else        { r = e % BN; c = e / BN; }
             int64_t gj = j0 + r, gk = k0 + c;
-            T x{};
+            V x = V(0.0);
             if (gj < n && gk < k) {
-                x = TRANS ? B[gk * ldb + gj] : B[gj * ldb + gk];
-                if constexpr (CONJ) x = conj_elem(x);
+                x = *reinterpret_cast<const V *>(TRANS ? &B[gk * ldb + gj] : &B[gj * ldb + gk]);
+                if constexpr (CONJ) x = conj_vec(x);
             }
             v[q] = x;
         }
@@ -98,7 +107,7 @@ struct SlabB {
             int r, c;
             if (!TRANS) { c = e % BK; r = e / BK; }
             else        { r = e % BN; c = e / BN; }
-            S[c][r] = v[q];
+            *reinterpret_cast<V *>(&S[c][r]) = v[q];
         }
     }
 };
@@ -114,8 +123,8 @@ void k_mgemm_plain(int64_t m, int64_t n, int64_t k, ddv alpha, ddv beta, int bet
 {
     constexpr int BM = 16 * TM, BN = 16 * TN;
     static_assert(KCHUNK % BK == 0, "BK must divide the flush period");
-    __shared__ ddv As[BK][BM];
-    __shared__ ddv Bs[BK][BN];
+    __shared__ __attribute__((aligned(16))) ddv As[BK][BM];
+    __shared__ __attribute__((aligned(16))) ddv Bs[BK][BN];
 
     const int tid = threadIdx.x;
     const int tx = tid & 15, ty = tid >> 4;
@@ -212,8 +221,8 @@ void k_wgemm_plain(int64_t m, int64_t n, int64_t k, zddv alpha, zddv beta, int b
 {
     constexpr int BM = 16 * TM, BN = 16 * TN;
     static_assert(KCHUNK % BK == 0, "BK must divide the flush period");
-    __shared__ zddv As[BK][BM];
-    __shared__ zddv Bs[BK][BN];
+    __shared__ __attribute__((aligned(32))) zddv As[BK][BM];
+    __shared__ __attribute__((aligned(32))) zddv Bs[BK][BN];
 
     const int tid = threadIdx.x;
     const int tx = tid & 15, ty = tid >> 4;
