@@ -393,6 +393,11 @@ void drain_slots()
 int timer_begin(int cls, hipStream_t s)
 {
     if (!g_timer_on.load(std::memory_order_relaxed)) return -1;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) {
+        (void)hipGetLastError();
+        return -1;    // no event records inside a captured graph
+    }
     std::lock_guard<std::mutex> lk(g_timer_mu);
     TimerSlot sl{cls, nullptr, nullptr};
     if (hipEventCreate(&sl.a) != hipSuccess || hipEventCreate(&sl.b) != hipSuccess) return -1;

@@ -1,0 +1,572 @@
+// batched.hip -- many independent, unevenly sized real double-double GEMMs in one
+// persistent launch (plain path) or one short fixed kernel sequence (Ozaki path), captured
+// once into a HIP graph by a plan and replayed with a single hipGraphLaunch.
+//
+// Plain: a persistent grid of (CUs x 2) workgroups pulls 64x64 output tiles from a global
+// work counter; the tile list is sorted by descending depth K (longest jobs first), so the
+// queue balances uneven problems.  Each tile runs exactly the single-GEMM tile routine, so
+// every problem's result is bit-identical to a separate mgemm_ call.
+//
+// Ozaki: the row/column maxima, residue split and reconstruction run over flattened unit
+// spaces covering all problems (a lane finds its problem by binary search over prefix
+// offsets); the residue GEMMs run as a persistent work queue over (problem, modulus, tile)
+// with 128x128 tiles.  All problems share one modulus plan sized for the deepest K.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/mwgemm_hip.h"
+#include "ozaki_device.hpp"
+#include "plain_tile.hpp"
+#include "timer.hpp"
+
+namespace mw {
+
+int record_error(int code, const char *fmt, ...);
+
+// Device-side descriptor of one problem of a batch.
+struct BatchProb {
+    const ddv *A, *B;
+    ddv *C;
+    int64_t lda, ldb, ldc;
+    int64_t m, n, k;
+    ddv alpha, beta;
+    int beta_mode, active;          // active: k > 0 and alpha != 0
+    // Ozaki workspace of this problem
+    int8_t *resA, *resB;
+    uint8_t *U;
+    unsigned long long *rA, *rB;
+    int64_t Mp, Np, Kc;
+};
+
+// Flattened unit spaces of the Ozaki phases: prefix offsets per problem.
+struct UnitSpace {
+    const int64_t *start;           // [count + 1], start[p] = first unit of problem p
+    int count;
+};
+
+DDI int find_problem(const int64_t *__restrict__ start, int count, int64_t u)
+{
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= u) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256)
+void k_batch_fill(unsigned long long *p, int64_t n, int *counters, int ncounters)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+    if (i < ncounters) counters[i] = 0;
+}
+
+// ---------------------------------------------------------------------------------
+// Plain persistent kernel.
+// ---------------------------------------------------------------------------------
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256)
+void k_batch_plain(const BatchProb *__restrict__ probs, const int4 *__restrict__ tiles,
+                   int ntiles, int *__restrict__ counter)
+{
+    __shared__ __attribute__((aligned(16))) ddv As[16][64];
+    __shared__ __attribute__((aligned(16))) ddv Bs[16][64];
+    __shared__ int s_tile;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
+        __syncthreads();
+        const int t = s_tile;
+        __syncthreads();
+        if (t >= ntiles) return;          // every workgroup ends here once the queue drains
+        const int4 tr = tiles[t];
+        const BatchProb &p = probs[tr.x];
+        const int64_t i0 = (int64_t)tr.y * 64, j0 = (int64_t)tr.z * 64;
+        if (p.active) {
+            mgemm_plain_tile<4, 4, 16, TA, TB>(p.m, p.n, p.k, p.alpha, p.beta, p.beta_mode, p.A,
+                                               p.lda, p.B, p.ldb, p.C, p.ldc, i0, j0, As, Bs);
+        } else {
+            const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j) {
+                    const int64_t gi = i0 + tx + 16 * i, gj = j0 + ty + 16 * j;
+                    if (gi < p.m && gj < p.n) {
+                        ddv *cp = &p.C[gj * p.ldc + gi];
+                        *cp = dd_beta_prepass(*cp, p.beta, p.beta_mode);
+                    }
+                }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Ozaki phases over all problems.
+// ---------------------------------------------------------------------------------
+// Row maxima: unit = (row r, 64-deep k chunk); rows fastest so lanes touch adjacent rows.
+template <bool A_SIDE, bool KC>
+__global__ __launch_bounds__(256)
+void k_batch_rowmax(const BatchProb *__restrict__ probs, UnitSpace us)
+{
+    const int64_t u = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (u >= us.start[us.count]) return;
+    const int pi = find_problem(us.start, us.count, u);
+    const BatchProb &p = probs[pi];
+    const int64_t R = A_SIDE ? p.m : p.n;
+    const int64_t local = u - us.start[pi];
+    const int64_t r = local % R, kc = local / R;
+    const ddv *X = A_SIDE ? p.A : p.B;
+    const int64_t ld = A_SIDE ? p.lda : p.ldb;
+    unsigned long long *rmax = A_SIDE ? p.rA : p.rB;
+    double mx = 0.0;
+    const int64_t kend = (kc * 64 + 64 < p.k) ? kc * 64 + 64 : p.k;
+    for (int64_t kk = kc * 64; kk < kend; ++kk) mx = nan_max(mx, mag(KC ? X[r * ld + kk] : X[kk * ld + r]));
+    if (mx != 0.0 || mx != mx) atomicMax(&rmax[r], (unsigned long long)__double_as_longlong(mx));
+}
+
+// Residue split: unit = (row r, 16-wide k group), ordered for coalescing.
+template <bool A_SIDE, bool KC>
+__global__ __launch_bounds__(256)
+void k_batch_split(const BatchProb *__restrict__ probs, UnitSpace us, int bits,
+                   const ModConst *__restrict__ mcs, int nmod)
+{
+    const int64_t u = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (u >= us.start[us.count]) return;
+    const int pi = find_problem(us.start, us.count, u);
+    const BatchProb &p = probs[pi];
+    const int64_t R = A_SIDE ? p.m : p.n;
+    const int64_t G = p.Kc / 16;
+    const int64_t local = u - us.start[pi];
+    int64_t r, kg;
+    if (KC) { kg = local % G; r = local / G; }
+    else    { r = local % R; kg = local / R; }
+    const int64_t rows = A_SIDE ? p.Mp : p.Np;
+    oz_split_unit<ddv, KC>(A_SIDE ? p.A : p.B, A_SIDE ? p.lda : p.ldb, p.k,
+                           A_SIDE ? p.rA : p.rB, bits, 0, mcs, nmod,
+                           A_SIDE ? p.resA : p.resB, 1, (int8_t *)nullptr, 0, p.Kc,
+                           rows * p.Kc, r, kg);
+}
+
+// Residue GEMMs: persistent queue over (problem, modulus, tm, tn) tiles of 128 x 128.
+__global__ __launch_bounds__(256, 2)
+void k_batch_oz_gemm(const BatchProb *__restrict__ probs, const int4 *__restrict__ tiles,
+                     int ntiles, int *__restrict__ counter, const ModConst *__restrict__ mcs)
+{
+    __shared__ __attribute__((aligned(16))) int8_t sa[2][128 * 80];
+    __shared__ __attribute__((aligned(16))) int8_t sb[2][128 * 80];
+    __shared__ int s_tile;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
+        __syncthreads();
+        const int t = s_tile;
+        __syncthreads();
+        if (t >= ntiles) return;
+        const int4 tr = tiles[t];      // (problem, modulus, tm, tn)
+        const BatchProb &p = probs[tr.x];
+        const int64_t i0 = (int64_t)tr.z * 128, j0 = (int64_t)tr.w * 128;
+        const int8_t *Ab = p.resA + (int64_t)tr.y * p.Mp * p.Kc + i0 * p.Kc;
+        const int8_t *Bb = p.resB + (int64_t)tr.y * p.Np * p.Kc + j0 * p.Kc;
+        uint8_t *Ub = p.U + (int64_t)tr.y * p.Np * p.Mp;
+        oz_gemm_tile<2, 2>(Ab, Bb, Ub, p.Kc, p.Mp, mcs[tr.y], i0, j0, sa, sb);
+    }
+}
+
+// Reconstruction: unit = (column j, 4-row group); inactive problems get the pre-pass only.
+__global__ __launch_bounds__(256)
+void k_batch_crt(const BatchProb *__restrict__ probs, UnitSpace us, int nmod,
+                 const ModConst *__restrict__ mcs, double Mhi, double Mlo, int bits)
+{
+    const int64_t u = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (u >= us.start[us.count]) return;
+    const int pi = find_problem(us.start, us.count, u);
+    const BatchProb &p = probs[pi];
+    const int64_t groups = (p.m + 3) / 4;
+    const int64_t local = u - us.start[pi];
+    const int64_t j = local / groups, i0 = (local % groups) * 4;
+    if (!p.active) {
+        for (int e = 0; e < 4 && i0 + e < p.m; ++e) {
+            ddv *cp = &p.C[j * p.ldc + i0 + e];
+            *cp = dd_beta_prepass(*cp, p.beta, p.beta_mode);
+        }
+        return;
+    }
+    oz_crt_unit<false>(p.U, p.Mp, p.Np * p.Mp, nmod, mcs, Mhi, Mlo, p.rA, p.rB, bits, p.m,
+                       p.alpha, p.beta, p.beta_mode, p.C, p.ldc, j, i0);
+}
+
+// ---------------------------------------------------------------------------------
+// Host side: plans.
+// ---------------------------------------------------------------------------------
+namespace {
+
+#define B_TRY(expr)                                                                    \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return record_error((int)e_, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+int64_t rup(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+unsigned blocks_for(int64_t units) { return (unsigned)((units + 255) / 256); }
+
+}  // namespace
+
+struct BatchPlan {
+    int path = 0;
+    int ta = 0, tb = 0;                 // 1 = transposed
+    int count = 0;
+    int device = 0;
+    void *dmem = nullptr;               // one allocation: table, lists, workspace
+    BatchProb *probs = nullptr;
+    int4 *tiles = nullptr;
+    int ntiles = 0;
+    int *counters = nullptr;            // [0]: plain / gemm queue
+    int64_t *us_ra = nullptr, *us_rb = nullptr, *us_sa = nullptr, *us_sb = nullptr, *us_crt = nullptr;
+    int64_t n_ra = 0, n_rb = 0, n_sa = 0, n_sb = 0, n_crt = 0;
+    unsigned long long *rmax_base = nullptr;
+    int64_t rmax_count = 0;
+    int grid_persist = 0;
+    OzPlan oz{};
+    ModConst *mcs = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double flops = 0.0;
+};
+
+// Enqueue the batch's kernels on s (used directly, and under stream capture for the graph).
+static int batch_enqueue(BatchPlan *P, hipStream_t s)
+{
+    const bool timed = true;
+    (void)timed;
+    if (P->path == MWGEMM_PATH_PLAIN) {
+        ScopedTimer tm(KC_BATCH_PLAIN, s);
+        hipLaunchKernelGGL(k_batch_fill, dim3(1), dim3(256), 0, s, (unsigned long long *)nullptr,
+                           (int64_t)0, P->counters, 1);
+        B_TRY(hipGetLastError());
+        if (P->ntiles == 0) return 0;
+        const dim3 g((unsigned)P->grid_persist), b(256);
+        if (!P->ta && !P->tb)
+            hipLaunchKernelGGL((k_batch_plain<false, false>), g, b, 0, s, P->probs, P->tiles, P->ntiles, P->counters);
+        else if (P->ta && !P->tb)
+            hipLaunchKernelGGL((k_batch_plain<true, false>), g, b, 0, s, P->probs, P->tiles, P->ntiles, P->counters);
+        else if (!P->ta && P->tb)
+            hipLaunchKernelGGL((k_batch_plain<false, true>), g, b, 0, s, P->probs, P->tiles, P->ntiles, P->counters);
+        else
+            hipLaunchKernelGGL((k_batch_plain<true, true>), g, b, 0, s, P->probs, P->tiles, P->ntiles, P->counters);
+        B_TRY(hipGetLastError());
+        return 0;
+    }
+    const bool a_kc = P->ta != 0, b_kc = P->tb == 0;
+    {
+        ScopedTimer tm(KC_BATCH_OZ_SPLIT, s);
+        const int64_t nfill = P->rmax_count > 1 ? P->rmax_count : 1;
+        hipLaunchKernelGGL(k_batch_fill, dim3(blocks_for(nfill)), dim3(256), 0, s, P->rmax_base,
+                           P->rmax_count, P->counters, 1);
+        B_TRY(hipGetLastError());
+        UnitSpace ra{P->us_ra, P->count}, rb{P->us_rb, P->count};
+        UnitSpace sa{P->us_sa, P->count}, sb{P->us_sb, P->count};
+        if (P->n_ra) {
+            if (a_kc) hipLaunchKernelGGL((k_batch_rowmax<true, true>), dim3(blocks_for(P->n_ra)), dim3(256), 0, s, P->probs, ra);
+            else      hipLaunchKernelGGL((k_batch_rowmax<true, false>), dim3(blocks_for(P->n_ra)), dim3(256), 0, s, P->probs, ra);
+            B_TRY(hipGetLastError());
+        }
+        if (P->n_rb) {
+            if (b_kc) hipLaunchKernelGGL((k_batch_rowmax<false, true>), dim3(blocks_for(P->n_rb)), dim3(256), 0, s, P->probs, rb);
+            else      hipLaunchKernelGGL((k_batch_rowmax<false, false>), dim3(blocks_for(P->n_rb)), dim3(256), 0, s, P->probs, rb);
+            B_TRY(hipGetLastError());
+        }
+        if (P->n_sa) {
+            if (a_kc) hipLaunchKernelGGL((k_batch_split<true, true>), dim3(blocks_for(P->n_sa)), dim3(256), 0, s, P->probs, sa, P->oz.bits, P->mcs, P->oz.nmod);
+            else      hipLaunchKernelGGL((k_batch_split<true, false>), dim3(blocks_for(P->n_sa)), dim3(256), 0, s, P->probs, sa, P->oz.bits, P->mcs, P->oz.nmod);
+            B_TRY(hipGetLastError());
+        }
+        if (P->n_sb) {
+            if (b_kc) hipLaunchKernelGGL((k_batch_split<false, true>), dim3(blocks_for(P->n_sb)), dim3(256), 0, s, P->probs, sb, P->oz.bits, P->mcs, P->oz.nmod);
+            else      hipLaunchKernelGGL((k_batch_split<false, false>), dim3(blocks_for(P->n_sb)), dim3(256), 0, s, P->probs, sb, P->oz.bits, P->mcs, P->oz.nmod);
+            B_TRY(hipGetLastError());
+        }
+    }
+    if (P->ntiles) {
+        ScopedTimer tm(KC_BATCH_OZ_GEMM, s);
+        hipLaunchKernelGGL(k_batch_oz_gemm, dim3((unsigned)P->grid_persist), dim3(256), 0, s,
+                           P->probs, P->tiles, P->ntiles, P->counters, P->mcs);
+        B_TRY(hipGetLastError());
+    }
+    if (P->n_crt) {
+        ScopedTimer tm(KC_BATCH_OZ_CRT, s);
+        UnitSpace cu{P->us_crt, P->count};
+        hipLaunchKernelGGL(k_batch_crt, dim3(blocks_for(P->n_crt)), dim3(256), 0, s, P->probs, cu,
+                           P->oz.nmod, P->mcs, P->oz.Mhi, P->oz.Mlo, P->oz.bits);
+        B_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+static void plan_free(BatchPlan *P)
+{
+    if (!P) return;
+    if (P->exec) (void)hipGraphExecDestroy(P->exec);
+    if (P->graph) (void)hipGraphDestroy(P->graph);
+    if (P->dmem) (void)hipFree(P->dmem);
+    if (P->mcs) (void)hipFree(P->mcs);
+    delete P;
+}
+
+static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, const int64_t *m,
+                      const int64_t *n, const int64_t *k, const mw_float64x2 *alpha,
+                      const mw_float64x2 *const *dA, const int64_t *lda,
+                      const mw_float64x2 *const *dB, const int64_t *ldb,
+                      const mw_float64x2 *beta, mw_float64x2 *const *dC, const int64_t *ldc,
+                      int path)
+{
+    *out = nullptr;
+    auto letter = [](char c) { c = (char)toupper((unsigned char)c); return c == 'N' ? 0 : (c == 'T' || c == 'C') ? 1 : -1; };
+    const int ta = letter(transa), tb = letter(transb);
+    if (ta < 0 || tb < 0) return record_error(-2, "batched mgemm: invalid trans letter");
+    if (count < 0 || count > (1 << 24)) return record_error(-3, "batched mgemm: bad count %lld", (long long)count);
+    if (path != MWGEMM_PATH_PLAIN && path != MWGEMM_PATH_OZAKI && path != MWGEMM_PATH_AUTO)
+        return record_error(-3, "batched mgemm: bad path");
+    int64_t kmax = 0;
+    double flops = 0.0;
+    for (int64_t p = 0; p < count; ++p) {
+        const int64_t ar = ta ? k[p] : m[p], br = tb ? n[p] : k[p];
+        const bool act = k[p] > 0 && !(alpha[p].limbs[0] == 0.0 && alpha[p].limbs[1] == 0.0);
+        if (m[p] < 0 || n[p] < 0 || ldc[p] < (m[p] > 1 ? m[p] : 1) ||
+            (act && m[p] > 0 && n[p] > 0 && (lda[p] < (ar > 1 ? ar : 1) || ldb[p] < (br > 1 ? br : 1))))
+            return record_error(-3, "batched mgemm: bad shape or leading dimension in problem %lld", (long long)p);
+        if (act) kmax = std::max(kmax, k[p]);
+        flops += 2.0 * (double)m[p] * (double)n[p] * (double)std::max<int64_t>(k[p], 0);
+    }
+    if (path == MWGEMM_PATH_AUTO) path = (kmax > 0 && ozaki_supported(kmax) && flops >= 2e9) ? MWGEMM_PATH_OZAKI : MWGEMM_PATH_PLAIN;
+    if (path == MWGEMM_PATH_OZAKI && kmax > 0 && !ozaki_supported(kmax)) path = MWGEMM_PATH_PLAIN;
+
+    BatchPlan *P = new BatchPlan();
+    P->path = path;
+    P->ta = ta;
+    P->tb = tb;
+    P->count = (int)count;
+    P->flops = flops;
+    if (hipGetDevice(&P->device) != hipSuccess) { plan_free(P); return record_error(-1, "hipGetDevice"); }
+    int cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, P->device) == hipSuccess) cus = prop.multiProcessorCount;
+
+    std::vector<BatchProb> host(count);
+    std::vector<int4> tl;
+    std::vector<int64_t> ra(count + 1, 0), rb(count + 1, 0), sa(count + 1, 0), sb(count + 1, 0),
+        cr(count + 1, 0);
+    const bool oz = path == MWGEMM_PATH_OZAKI;
+    if (oz && kmax > 0 && !ozaki_make_plan(kmax, ozaki_target_bits(), ozaki_forced_nmod(), &P->oz)) {
+        plan_free(P);
+        return record_error(-5, "batched ozaki: no plan for k=%lld", (long long)kmax);
+    }
+    const int nmod = oz ? P->oz.nmod : 0;
+    size_t ws = 0;                      // Ozaki workspace bytes (offsets relative to ws start)
+    int64_t rmax_n = 0;
+    std::vector<size_t> offA(count), offB(count), offU(count);
+    std::vector<int64_t> offR(count);
+    for (int64_t p = 0; p < count; ++p) {
+        BatchProb &q = host[p];
+        memset(&q, 0, sizeof q);
+        q.A = reinterpret_cast<const ddv *>(dA ? dA[p] : nullptr);
+        q.B = reinterpret_cast<const ddv *>(dB ? dB[p] : nullptr);
+        q.C = reinterpret_cast<ddv *>(dC[p]);
+        q.lda = lda[p]; q.ldb = ldb[p]; q.ldc = ldc[p];
+        q.m = m[p]; q.n = n[p]; q.k = std::max<int64_t>(k[p], 0);
+        q.alpha = ddv{alpha[p].limbs[0], alpha[p].limbs[1]};
+        q.beta = ddv{beta[p].limbs[0], beta[p].limbs[1]};
+        const bool b0 = q.beta.hi == 0.0 && q.beta.lo == 0.0, b1 = q.beta.hi == 1.0 && q.beta.lo == 0.0;
+        q.beta_mode = b0 ? BETA_ZERO : b1 ? BETA_ONE : BETA_SCALE;
+        q.active = (q.k > 0 && !(q.alpha.hi == 0.0 && q.alpha.lo == 0.0)) ? 1 : 0;
+        const bool empty = q.m <= 0 || q.n <= 0;
+        if (!oz) {
+            if (!empty && (q.active || q.beta_mode != BETA_ONE))
+                for (int64_t tn = 0; tn < (q.n + 63) / 64; ++tn)
+                    for (int64_t tm = 0; tm < (q.m + 63) / 64; ++tm) tl.push_back(int4{(int)p, (int)tm, (int)tn, (int)(q.active ? q.k : 0)});
+            continue;
+        }
+        cr[p + 1] = empty ? 0 : q.n * ((q.m + 3) / 4);
+        if (empty || !q.active) continue;
+        q.Mp = rup(q.m, 128); q.Np = rup(q.n, 128); q.Kc = rup(q.k, 64);
+        offR[p] = rmax_n;
+        rmax_n += q.Mp + q.Np;
+        offA[p] = ws; ws += rup((int64_t)nmod * q.Mp * q.Kc, 256);
+        offB[p] = ws; ws += rup((int64_t)nmod * q.Np * q.Kc, 256);
+        offU[p] = ws; ws += rup((int64_t)nmod * q.Np * q.Mp, 256);
+        ra[p + 1] = q.m * ((q.k + 63) / 64);
+        rb[p + 1] = q.n * ((q.k + 63) / 64);
+        sa[p + 1] = q.m * (q.Kc / 16);
+        sb[p + 1] = q.n * (q.Kc / 16);
+        for (int t = 0; t < nmod; ++t)
+            for (int64_t tn = 0; tn < q.Np / 128; ++tn)
+                for (int64_t tm = 0; tm < q.Mp / 128; ++tm) tl.push_back(int4{(int)p, t, (int)tm, (int)tn});
+    }
+    // Longest jobs first: plain tiles cost ~K, residue tiles ~Kc.
+    if (!oz)
+        std::stable_sort(tl.begin(), tl.end(), [](const int4 &a, const int4 &b) { return a.w > b.w; });
+    else
+        std::stable_sort(tl.begin(), tl.end(), [&](const int4 &a, const int4 &b) { return host[a.x].Kc > host[b.x].Kc; });
+    for (int64_t p = 0; p < count; ++p) {
+        ra[p + 1] += ra[p]; rb[p + 1] += rb[p]; sa[p + 1] += sa[p]; sb[p + 1] += sb[p]; cr[p + 1] += cr[p];
+    }
+    if (tl.size() > (size_t)INT32_MAX) { plan_free(P); return record_error(-3, "batched mgemm: too many tiles"); }
+    P->ntiles = (int)tl.size();
+    P->n_ra = ra[count]; P->n_rb = rb[count]; P->n_sa = sa[count]; P->n_sb = sb[count]; P->n_crt = cr[count];
+    P->rmax_count = rmax_n;
+    P->grid_persist = std::max(1, std::min<int>(P->ntiles, cus * 2));
+
+    // One device allocation: [probs | tiles | counters | 5 prefix arrays | rmax | workspace]
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~size_t(255); return o; };
+    const size_t o_probs = take(sizeof(BatchProb) * std::max<int64_t>(count, 1));
+    const size_t o_tiles = take(sizeof(int4) * std::max<size_t>(tl.size(), 1));
+    const size_t o_cnt = take(sizeof(int) * 64);
+    const size_t pre = sizeof(int64_t) * (count + 1);
+    const size_t o_ra = take(pre), o_rb = take(pre), o_sa = take(pre), o_sb = take(pre), o_cr = take(pre);
+    const size_t o_rmax = take(sizeof(unsigned long long) * std::max<int64_t>(rmax_n, 1));
+    const size_t o_ws = take(std::max<size_t>(ws, 1));
+    if (hipMalloc(&P->dmem, off) != hipSuccess) {
+        (void)hipGetLastError();
+        size_t bytes = off;
+        plan_free(P);
+        return record_error(-6, "batched mgemm: cannot allocate %zu bytes of plan memory", bytes);
+    }
+    char *base = static_cast<char *>(P->dmem);
+    P->probs = reinterpret_cast<BatchProb *>(base + o_probs);
+    P->tiles = reinterpret_cast<int4 *>(base + o_tiles);
+    P->counters = reinterpret_cast<int *>(base + o_cnt);
+    P->us_ra = reinterpret_cast<int64_t *>(base + o_ra);
+    P->us_rb = reinterpret_cast<int64_t *>(base + o_rb);
+    P->us_sa = reinterpret_cast<int64_t *>(base + o_sa);
+    P->us_sb = reinterpret_cast<int64_t *>(base + o_sb);
+    P->us_crt = reinterpret_cast<int64_t *>(base + o_cr);
+    P->rmax_base = reinterpret_cast<unsigned long long *>(base + o_rmax);
+    char *wsb = base + o_ws;
+    for (int64_t p = 0; p < count; ++p) {
+        BatchProb &q = host[p];
+        if (!oz || q.Kc == 0) continue;
+        q.rA = P->rmax_base + offR[p];
+        q.rB = q.rA + q.Mp;
+        q.resA = reinterpret_cast<int8_t *>(wsb + offA[p]);
+        q.resB = reinterpret_cast<int8_t *>(wsb + offB[p]);
+        q.U = reinterpret_cast<uint8_t *>(wsb + offU[p]);
+    }
+    int rc = 0;
+    if (hipMemcpy(P->probs, host.data(), sizeof(BatchProb) * count, hipMemcpyHostToDevice) != hipSuccess ||
+        (!tl.empty() && hipMemcpy(P->tiles, tl.data(), sizeof(int4) * tl.size(), hipMemcpyHostToDevice) != hipSuccess) ||
+        hipMemcpy(P->us_ra, ra.data(), pre, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(P->us_rb, rb.data(), pre, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(P->us_sa, sa.data(), pre, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(P->us_sb, sb.data(), pre, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(P->us_crt, cr.data(), pre, hipMemcpyHostToDevice) != hipSuccess)
+        rc = record_error(-1, "batched mgemm: plan upload failed");
+    if (!rc && oz && nmod > 0) {
+        if (hipMalloc(&P->mcs, sizeof(ModConst) * OZ_MAX_MOD) != hipSuccess ||
+            hipMemcpy(P->mcs, P->oz.mc, sizeof(ModConst) * nmod, hipMemcpyHostToDevice) != hipSuccess)
+            rc = record_error(-1, "batched mgemm: modulus table upload failed");
+    }
+    if (rc) { plan_free(P); return rc; }
+    *out = P;
+    return 0;
+}
+
+}  // namespace mw
+
+using namespace mw;
+
+struct mwgemm_batched_plan_s {
+    BatchPlan *p;
+};
+
+extern "C" {
+
+int mwgemm_batched_plan_create(mwgemm_batched_plan *plan, int64_t count, char transa, char transb,
+                               const int64_t *m, const int64_t *n, const int64_t *k,
+                               const mw_float64x2 *alpha, const mw_float64x2 *const *dA,
+                               const int64_t *lda, const mw_float64x2 *const *dB,
+                               const int64_t *ldb, const mw_float64x2 *beta,
+                               mw_float64x2 *const *dC, const int64_t *ldc, int path)
+{
+    if (!plan) return record_error(-3, "batched plan: null handle");
+    *plan = nullptr;
+    BatchPlan *P = nullptr;
+    if (int rc = plan_build(&P, count, transa, transb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc, path))
+        return rc;
+    // Capture the kernel sequence once; runs replay it with one graph launch.
+    hipStream_t cs = nullptr;
+    int rc = 0;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) rc = record_error(-1, "stream");
+    if (!rc && hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = record_error(-1, "begin capture");
+    if (!rc) {
+        int erc = batch_enqueue(P, cs);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(cs, &g);
+        if (erc) rc = erc;
+        else if (e != hipSuccess) rc = record_error((int)e, "end capture: %s", hipGetErrorString(e));
+        else {
+            P->graph = g;
+            e = hipGraphInstantiate(&P->exec, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) rc = record_error((int)e, "graph instantiate: %s", hipGetErrorString(e));
+        }
+    }
+    if (cs) (void)hipStreamDestroy(cs);
+    if (rc) { plan_free(P); return rc; }
+    auto *h = new mwgemm_batched_plan_s{P};
+    *plan = h;
+    return 0;
+}
+
+int mwgemm_batched_plan_run(mwgemm_batched_plan plan, void *stream)
+{
+    if (!plan || !plan->p || !plan->p->exec) return record_error(-3, "batched plan: invalid handle");
+    hipError_t e = hipGraphLaunch(plan->p->exec, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return record_error((int)e, "hipGraphLaunch: %s", hipGetErrorString(e));
+    return 0;
+}
+
+int mwgemm_batched_plan_info(mwgemm_batched_plan plan, int *path, int64_t *tiles, double *flops,
+                             int *nmod)
+{
+    if (!plan || !plan->p) return -1;
+    if (path) *path = plan->p->path;
+    if (tiles) *tiles = plan->p->ntiles;
+    if (flops) *flops = plan->p->flops;
+    if (nmod) *nmod = plan->p->path == MWGEMM_PATH_OZAKI ? plan->p->oz.nmod : 0;
+    return 0;
+}
+
+int mwgemm_batched_plan_destroy(mwgemm_batched_plan plan)
+{
+    if (!plan) return 0;
+    (void)hipDeviceSynchronize();
+    plan_free(plan->p);
+    delete plan;
+    return 0;
+}
+
+int mwgemm_mgemm_batched_device(int64_t count, char transa, char transb, const int64_t *m,
+                                const int64_t *n, const int64_t *k, const mw_float64x2 *alpha,
+                                const mw_float64x2 *const *dA, const int64_t *lda,
+                                const mw_float64x2 *const *dB, const int64_t *ldb,
+                                const mw_float64x2 *beta, mw_float64x2 *const *dC,
+                                const int64_t *ldc, int path, void *stream)
+{
+    BatchPlan *P = nullptr;
+    if (int rc = plan_build(&P, count, transa, transb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc, path))
+        return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int rc = batch_enqueue(P, s);
+    // The plan's memory is released once the stream has drained it.
+    hipError_t e = hipStreamSynchronize(s);
+    if (!rc && e != hipSuccess) rc = record_error((int)e, "batched mgemm: %s", hipGetErrorString(e));
+    plan_free(P);
+    return rc;
+}
+
+}  // extern "C"

@@ -19,7 +19,7 @@
 //
 // Complex: Re C' = [Ar | -Ai] [Br ; Bi], Im C' = [Ar | Ai] [Bi ; Br] -- two real products
 // of doubled depth per modulus (four real slice products), conjugation folded into signs.
-#include "ozaki.hpp"
+#include "ozaki_device.hpp"
 #include "timer.hpp"
 
 #include <cmath>
@@ -188,14 +188,6 @@ bool ozaki_supported_cplx(int64_t k) { return k >= 1 && 2 * k <= OZ_MAX_KEFF; }
 // ==================================================================================
 // Device kernels.
 // ==================================================================================
-typedef int v4i __attribute__((ext_vector_type(4)));
-typedef int v16i __attribute__((ext_vector_type(16)));
-
-// Maximum that keeps a NaN (fmax would drop it): a NaN entry must poison its row.
-DDI double nan_max(double a, double b) { return (b > a || b != b) ? b : a; }
-DDI double mag(const ddv &x) { return fabs(x.hi) + fabs(x.lo); }
-DDI double mag(const zddv &x) { return nan_max(mag(x.re), mag(x.im)); }
-
 __global__ __launch_bounds__(256)
 void k_oz_fill_u64(unsigned long long *p, int64_t n)
 {
@@ -246,65 +238,7 @@ void k_oz_rowmax_kc(const T *__restrict__ X, int64_t ld, int64_t R, int64_t K,
     if (lane == 0 && r < R && bits != 0ull) atomicMax(&rmax[r], bits);
 }
 
-// Scale exponent of a row from its maximum: largest entry lands in [2^(b-1), 2^b).
-DDI int row_sigma(unsigned long long bits, int b, bool &bad)
-{
-    double mx = __longlong_as_double((long long)bits);
-    bad = !(mx <= 1.79769313486231570815e308);   // NaN or Inf
-    if (bad || mx == 0.0) return 0;
-    int e;
-    (void)frexp(mx, &e);
-    return b - e;
-}
-
-// The integer round(x * 2^sigma) of a DD value as three exact FP64 chunks with
-// value = c0 + c1*2^35 + c2*2^70 (|c0| < 2^52, |c1| < 2^35, |c2| <= 2^34 for b <= 104).
-// The chunks are cut from |xh| and then signed: the low bits of a 53-bit significand are
-// always representable, whereas xh mod 2^70 of a negative xh generally is not.
-DDI void int_chunks(double hi, double lo, int sigma, double &c0, double &c1, double &c2)
-{
-    const double xh_s = ldexp(hi, sigma), xl_s = ldexp(lo, sigma);
-    const double xh = rint(xh_s);
-    const double xl = rint((xh_s - xh) + xl_s);
-    const double ax = fabs(xh);
-    const double a2 = floor(ax * 0x1p-70);
-    const double rem = __builtin_fma(-a2, 0x1p70, ax);
-    const double a1 = floor(rem * 0x1p-35);
-    const double a0 = __builtin_fma(-a1, 0x1p35, rem);
-    const double sg = xh < 0.0 ? -1.0 : 1.0;
-    c2 = sg * a2;
-    c1 = sg * a1;
-    c0 = __builtin_fma(sg, a0, xl);
-}
-
-// Symmetric residue of c0 + c1*2^35 + c2*2^70 modulo m, in [-(m-1)/2, m/2] (int8 range).
-DDI int sym_residue(double c0, double c1, double c2, const ModConst &mc)
-{
-    const double m = mc.m;
-    const double s = __builtin_fma(c2, mc.p2, __builtin_fma(c1, mc.p1, c0));
-    const double q = floor(s * mc.inv_m);
-    double r = __builtin_fma(-q, m, s);
-    r = (r < 0.0) ? r + m : r;
-    r = (r >= m) ? r - m : r;
-    r = (r > 0.5 * (m - 1.0)) ? r - m : r;
-    return (int)r;
-}
-
-DDI void comp(const ddv &x, int part, double &hi, double &lo)
-{
-    (void)part;
-    hi = x.hi;
-    lo = x.lo;
-}
-DDI void comp(const zddv &x, int part, double &hi, double &lo)
-{
-    const ddv &c = part ? x.im : x.re;
-    hi = c.hi;
-    lo = c.lo;
-}
-
-// Residue split: row r of the operand, 16 consecutive k per lane; all moduli.
-// out1[t*mod_stride + r*kp + k] = s1 * (residue of entry (r,k)), likewise out2 (optional).
+// Residue split kernel: one unit per lane (see oz_split_unit).
 template <typename T, bool KC>
 __global__ __launch_bounds__(256)
 void k_oz_split(const T *__restrict__ X, int64_t ld, int64_t R, int64_t K, int64_t kgroups,
@@ -321,203 +255,30 @@ void k_oz_split(const T *__restrict__ X, int64_t ld, int64_t R, int64_t K, int64
         kg = (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
     }
     if (r >= R || kg >= kgroups) return;
-    bool bad;
-    const int sigma = row_sigma(rmax[r], bits, bad);
-    double c0[16], c1[16], c2[16];
-#pragma unroll
-    for (int s = 0; s < 16; ++s) {
-        const int64_t k = kg * 16 + s;
-        double hi = 0.0, lo = 0.0;
-        if (k < K && !bad) comp(KC ? X[r * ld + k] : X[k * ld + r], part, hi, lo);
-        int_chunks(hi, lo, sigma, c0[s], c1[s], c2[s]);
-    }
-    for (int t = 0; t < nmod; ++t) {
-        const ModConst mc = mcs[t];
-        uint32_t w1[4] = {0, 0, 0, 0}, w2[4] = {0, 0, 0, 0};
-#pragma unroll
-        for (int s = 0; s < 16; ++s) {
-            const int res = sym_residue(c0[s], c1[s], c2[s], mc);
-            w1[s >> 2] |= ((uint32_t)(res * s1) & 0xffu) << (8 * (s & 3));
-            w2[s >> 2] |= ((uint32_t)(res * s2) & 0xffu) << (8 * (s & 3));
-        }
-        const int64_t off = t * mod_stride + r * kp + kg * 16;
-        *reinterpret_cast<uint4 *>(out1 + off) = make_uint4(w1[0], w1[1], w1[2], w1[3]);
-        if (out2) *reinterpret_cast<uint4 *>(out2 + off) = make_uint4(w2[0], w2[1], w2[2], w2[3]);
-    }
+    oz_split_unit<T, KC>(X, ld, K, rmax, bits, part, mcs, nmod, out1, s1, out2, s2, kp,
+                         mod_stride, r, kg);
 }
 
-// u = ((v mod m) * y) mod m for an exact INT32 dot product v.
-DDI uint32_t crt_weight_residue(int v, double m, double inv_m, double y)
-{
-    double d = (double)v;
-    double q = floor(d * inv_m);
-    double r = __builtin_fma(-q, m, d);
-    r = (r < 0.0) ? r + m : r;
-    r = (r >= m) ? r - m : r;
-    double w = r * y;
-    q = floor(w * inv_m);
-    double u = __builtin_fma(-q, m, w);
-    u = (u < 0.0) ? u + m : u;
-    u = (u >= m) ? u - m : u;
-    return (uint32_t)u;
-}
-
-// Workgroup -> output tile.  Workgroups are dispatched round-robin over the 8 XCDs (each
-// with its own L2), so ids are first regrouped so that one XCD receives a contiguous run;
-// runs are then walked in bands of GROUP tile-rows, column by column, so the workgroups
-// resident on one XCD share A row-slabs and B column-slabs in its L2.
-DDI void tile_order(int bid, int tiles_m, int tiles_n, int &tm, int &tn)
-{
-    constexpr int XCD = 8, GROUP = 4;
-    const int T = tiles_m * tiles_n;
-    int id = bid;
-    if (T % XCD == 0) id = (bid % XCD) * (T / XCD) + bid / XCD;
-    const int band = GROUP * tiles_n;
-    const int first_m = (id / band) * GROUP;
-    const int gm = (tiles_m - first_m < GROUP) ? tiles_m - first_m : GROUP;
-    const int r = id % band;
-    tm = first_m + r % gm;
-    tn = r / gm;
-}
-
-// Batched residue GEMM: U_b = ((A_b B_b^T) mod m * y) mod m, A_b: [Mp][kp] int8,
-// B_b: [Np][kp] int8, U_b column-major [Np][ldu] uint8.  Four wave64 per workgroup in a
-// 2 x 2 arrangement, each wave owns TM x TN MFMA tiles of 32 x 32 (v_mfma_i32_32x32x32_i8);
-// 64-deep K slabs double-buffered in LDS with 80-byte row pitch (conflict-free b128 reads).
+// Batched residue GEMM kernel: blockIdx.y = batch (modulus, or modulus x part for complex).
 template <int TM, int TN>
 __global__ __launch_bounds__(256, TM * TN >= 16 ? 1 : 2)
 void k_oz_gemm(const int8_t *__restrict__ A, const int8_t *__restrict__ B,
                uint8_t *__restrict__ U, int64_t kp, int64_t ldu, int64_t sA, int64_t sB,
                int64_t sU, const ModConst *__restrict__ mcs, int per, int tiles_m)
 {
-    constexpr int BM = 64 * TM, BN = 64 * TN, PITCH = 80;
-    __shared__ __attribute__((aligned(16))) int8_t sa[2][BM * PITCH];
-    __shared__ __attribute__((aligned(16))) int8_t sb[2][BN * PITCH];
-
+    constexpr int BM = 64 * TM, BN = 64 * TN;
+    __shared__ __attribute__((aligned(16))) int8_t sa[2][BM * 80];
+    __shared__ __attribute__((aligned(16))) int8_t sb[2][BN * 80];
     const int batch = blockIdx.y;
     int tm, tn;
     tile_order(blockIdx.x, tiles_m, (int)gridDim.x / tiles_m, tm, tn);
     const int64_t i0 = (int64_t)tm * BM;
     const int64_t j0 = (int64_t)tn * BN;
-    const int8_t *Ab = A + batch * sA + i0 * kp;
-    const int8_t *Bb = B + batch * sB + j0 * kp;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int wm = w & 1, wn = w >> 1;
-
-    v4i ra[TM], rb[TN];
-#define OZ_GLOAD(K0)                                                                          \
-    do {                                                                                      \
-        _Pragma("unroll") for (int q = 0; q < TM; ++q) {                                      \
-            const int c = tid + 256 * q;                                                      \
-            ra[q] = *reinterpret_cast<const v4i *>(Ab + (int64_t)(c >> 2) * kp + (K0) + (c & 3) * 16); \
-        }                                                                                     \
-        _Pragma("unroll") for (int q = 0; q < TN; ++q) {                                      \
-            const int c = tid + 256 * q;                                                      \
-            rb[q] = *reinterpret_cast<const v4i *>(Bb + (int64_t)(c >> 2) * kp + (K0) + (c & 3) * 16); \
-        }                                                                                     \
-    } while (0)
-#define OZ_LSTORE(BUF)                                                                        \
-    do {                                                                                      \
-        _Pragma("unroll") for (int q = 0; q < TM; ++q) {                                      \
-            const int c = tid + 256 * q;                                                      \
-            *reinterpret_cast<v4i *>(&sa[BUF][(c >> 2) * PITCH + (c & 3) * 16]) = ra[q];     \
-        }                                                                                     \
-        _Pragma("unroll") for (int q = 0; q < TN; ++q) {                                      \
-            const int c = tid + 256 * q;                                                      \
-            *reinterpret_cast<v4i *>(&sb[BUF][(c >> 2) * PITCH + (c & 3) * 16]) = rb[q];     \
-        }                                                                                     \
-    } while (0)
-
-    v16i acc[TM][TN];
-#pragma unroll
-    for (int a = 0; a < TM; ++a)
-#pragma unroll
-        for (int b = 0; b < TN; ++b) acc[a][b] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-
-    const int64_t nk = kp / 64;
-    OZ_GLOAD(0);
-    OZ_LSTORE(0);
-    __syncthreads();
-    for (int64_t kt = 0; kt < nk; ++kt) {
-        const int cur = (int)(kt & 1);
-        if (kt + 1 < nk) OZ_GLOAD((kt + 1) * 64);
-#pragma unroll
-        for (int ks = 0; ks < 2; ++ks) {
-            v4i af[TM], bf[TN];
-#pragma unroll
-            for (int a = 0; a < TM; ++a) {
-                const int row = wm * 32 * TM + a * 32 + (lane & 31);
-                af[a] = *reinterpret_cast<const v4i *>(&sa[cur][row * PITCH + ks * 32 + (lane >> 5) * 16]);
-            }
-#pragma unroll
-            for (int b = 0; b < TN; ++b) {
-                const int col = wn * 32 * TN + b * 32 + (lane & 31);
-                bf[b] = *reinterpret_cast<const v4i *>(&sb[cur][col * PITCH + ks * 32 + (lane >> 5) * 16]);
-            }
-#pragma unroll
-            for (int a = 0; a < TM; ++a)
-#pragma unroll
-                for (int b = 0; b < TN; ++b)
-                    acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[a], bf[b], acc[a][b], 0, 0, 0);
-        }
-        if (kt + 1 < nk) OZ_LSTORE(cur ^ 1);
-        __syncthreads();
-    }
-
-#undef OZ_GLOAD
-#undef OZ_LSTORE
-    const ModConst &mc = mcs[batch / per];
-    const double m = mc.m, inv_m = mc.inv_m, y = mc.y;
-    uint8_t *Ub = U + batch * sU;
-#pragma unroll
-    for (int a = 0; a < TM; ++a)
-#pragma unroll
-        for (int b = 0; b < TN; ++b) {
-            const int64_t col = j0 + wn * 32 * TN + b * 32 + (lane & 31);
-            const int64_t rowb = i0 + wm * 32 * TM + a * 32 + 4 * (lane >> 5);
-#pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                uint32_t word = 0;
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    word |= crt_weight_residue(acc[a][b][4 * g + q], m, inv_m, y) << (8 * q);
-                *reinterpret_cast<uint32_t *>(Ub + col * ldu + rowb + 8 * g) = word;
-            }
-        }
+    oz_gemm_tile<TM, TN>(A + batch * sA + i0 * kp, B + batch * sB + j0 * kp, U + batch * sU, kp,
+                         ldu, mcs[batch / per], i0, j0, sa, sb);
 }
 
-// 128-bit fixed-point fraction sum_t u_t * round(2^128/m_t) mod 2^128, from four exact FP64
-// limb accumulators (each < 2^45), as a signed double-double value F (|F| <= 2^127).
-DDI ddv fixed128_to_dd(double a0, double a1, double a2, double a3)
-{
-    double c = floor(a0 * 0x1p-32);
-    const double f0 = __builtin_fma(-c, 0x1p32, a0);
-    a1 += c;
-    c = floor(a1 * 0x1p-32);
-    const double f1 = __builtin_fma(-c, 0x1p32, a1);
-    a2 += c;
-    c = floor(a2 * 0x1p-32);
-    const double f2 = __builtin_fma(-c, 0x1p32, a2);
-    a3 += c;
-    c = floor(a3 * 0x1p-32);
-    double f3 = __builtin_fma(-c, 0x1p32, a3);
-    if (f3 >= 0x1p31) f3 -= 0x1p32;              // symmetric range: F - 2^128
-    ddv v;
-    dd_two_sum(f3 * 0x1p96, f2 * 0x1p64, v.hi, v.lo);
-    v = dd_add_exact(v, ddv{f1 * 0x1p32, 0.0});
-    v = dd_add_exact(v, ddv{f0, 0.0});
-    return v;
-}
-
-DDI ddv reconstruct(const double acc[4], double Mhi, double Mlo, int e, bool bad)
-{
-    if (bad) return ddv{__builtin_nan(""), __builtin_nan("")};
-    const ddv f = fixed128_to_dd(acc[0], acc[1], acc[2], acc[3]);
-    const ddv c = dd_mul_exact(f, ddv{Mhi, Mlo});
-    return ddv{ldexp(c.hi, e), ldexp(c.lo, e)};
-}
-
-// Reconstruction + alpha/beta update.  Each lane owns 4 consecutive rows of one column.
+// Reconstruction kernel.  Each lane owns 4 consecutive rows of one column.
 template <bool CPLX>
 __global__ __launch_bounds__(256)
 void k_oz_crt(const uint8_t *__restrict__ U, int64_t ldu, int64_t sU, int nmod,
@@ -528,56 +289,11 @@ void k_oz_crt(const uint8_t *__restrict__ U, int64_t ldu, int64_t sU, int nmod,
               typename std::conditional<CPLX, zddv, ddv>::type beta, int beta_mode,
               typename std::conditional<CPLX, zddv, ddv>::type *__restrict__ C, int64_t ldc)
 {
-    constexpr int P = CPLX ? 2 : 1;
     const int64_t j = (int64_t)blockIdx.x / blocks_per_col;
     const int64_t i0 = (((int64_t)blockIdx.x % blocks_per_col) * 256 + threadIdx.x) * 4;
     if (j >= n || i0 >= m) return;
-    double acc[4][P][4];
-#pragma unroll
-    for (int e = 0; e < 4; ++e)
-#pragma unroll
-        for (int p = 0; p < P; ++p)
-#pragma unroll
-            for (int l = 0; l < 4; ++l) acc[e][p][l] = 0.0;
-    for (int t = 0; t < nmod; ++t) {
-        const double w0 = mcs[t].w[0], w1 = mcs[t].w[1], w2 = mcs[t].w[2], w3 = mcs[t].w[3];
-#pragma unroll
-        for (int p = 0; p < P; ++p) {
-            const uint32_t word =
-                *reinterpret_cast<const uint32_t *>(U + (int64_t)(t * P + p) * sU + j * ldu + i0);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const double u = (double)((word >> (8 * e)) & 0xffu);
-                acc[e][p][0] = __builtin_fma(u, w0, acc[e][p][0]);
-                acc[e][p][1] = __builtin_fma(u, w1, acc[e][p][1]);
-                acc[e][p][2] = __builtin_fma(u, w2, acc[e][p][2]);
-                acc[e][p][3] = __builtin_fma(u, w3, acc[e][p][3]);
-            }
-        }
-    }
-    bool badj;
-    const int tau = row_sigma(rmaxB[j], bits, badj);
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        const int64_t i = i0 + e;
-        if (i >= m) break;
-        bool badi;
-        const int sigma = row_sigma(rmaxA[i], bits, badi);
-        const int ex = -128 - sigma - tau;
-        const bool bad = badi || badj;
-        if constexpr (!CPLX) {
-            const ddv cab = reconstruct(acc[e][0], Mhi, Mlo, ex, bad);
-            ddv *cp = &C[j * ldc + i];
-            const ddv c0 = dd_beta_prepass(*cp, beta, beta_mode);
-            *cp = dd_add_exact(c0, dd_mul_exact(alpha, cab));
-        } else {
-            const zddv cab{reconstruct(acc[e][0], Mhi, Mlo, ex, bad),
-                           reconstruct(acc[e][1], Mhi, Mlo, ex, bad)};
-            zddv *cp = &C[j * ldc + i];
-            const zddv c0 = zdd_beta_prepass(*cp, beta, beta_mode);
-            *cp = zdd_add_exact(c0, zdd_mul_exact(alpha, cab));
-        }
-    }
+    oz_crt_unit<CPLX>(U, ldu, sU, nmod, mcs, Mhi, Mlo, rmaxA, rmaxB, bits, m, alpha, beta,
+                      beta_mode, C, ldc, j, i0);
 }
 
 // ==================================================================================

@@ -15,102 +15,10 @@
 //       C := faithful_add(C, fast_mul(alpha, acc))   (mgemm_serial.cpp:173-191)
 // The running C lives in HBM between chunk flushes: one read-modify-write per 128-deep
 // chunk costs ~1 byte per CU-clock, far below what the FP64 ALU leaves idle.
-#include "mw_internal.hpp"
+#include "plain_tile.hpp"
 #include "timer.hpp"
 
 namespace mw {
-
-constexpr int KCHUNK = 128;   // reference KC: the accumulator flush period
-
-// ----------------------------------------------------------------------------------
-// Tile staging helpers (generic over element type T = ddv or zddv).
-// ----------------------------------------------------------------------------------
-
-// Staging registers hold native FP64 vectors (a DD as double2, a complex DD as double4)
-// so they stay in VGPRs; arrays of the element structs end up in scratch memory.
-typedef double v2d __attribute__((ext_vector_type(2)));
-typedef double v4d __attribute__((ext_vector_type(4)));
-template <typename T> struct VecOf;
-template <> struct VecOf<ddv> { typedef v2d type; };
-template <> struct VecOf<zddv> { typedef v4d type; };
-DDI v2d conj_vec(v2d x) { return x; }
-DDI v4d conj_vec(v4d x) { return v4d{x.x, x.y, -x.z, -x.w}; }
-
-// op(A) slab [i0, i0+BM) x [k0, k0+BK) -> regs.  TRANS: A is stored K x M.
-template <typename T, int BM, int BK, bool TRANS, bool CONJ>
-struct SlabA {
-    typedef typename VecOf<T>::type V;
-    static constexpr int PER = BM * BK / 256;
-    V v[PER];
-    DDI void load(const T *__restrict__ A, int64_t lda, int64_t m, int64_t k, int64_t i0,
-                  int64_t k0, int tid)
-    {
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            int e = tid + 256 * q;
-            int r, c;
-            if (!TRANS) { r = e % BM; c = e / BM; }
-            else        { c = e % BK; r = e / BK; }
-            int64_t gi = i0 + r, gk = k0 + c;
-            V x = V(0.0);
-            if (gi < m && gk < k) {
-                x = *reinterpret_cast<const V *>(TRANS ? &A[gi * lda + gk] : &A[gk * lda + gi]);
-                if constexpr (CONJ) x = conj_vec(x);
-            }
-            v[q] = x;
-        }
-    }
-    template <int LDW>
-    DDI void store(T (*S)[LDW], int tid) const
-    {
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            int e = tid + 256 * q;
-            int r, c;
-            if (!TRANS) { r = e % BM; c = e / BM; }
-            else        { c = e % BK; r = e / BK; }
-            *reinterpret_cast<V *>(&S[c][r]) = v[q];
-        }
-    }
-};
-
-// op(B) slab [k0, k0+BK) x [j0, j0+BN) -> regs.  TRANS: B is stored N x K.
-template <typename T, int BN, int BK, bool TRANS, bool CONJ>
-struct SlabB {
-    typedef typename VecOf<T>::type V;
-    static constexpr int PER = BN * BK / 256;
-    V v[PER];
-    DDI void load(const T *__restrict__ B, int64_t ldb, int64_t n, int64_t k, int64_t j0,
-                  int64_t k0, int tid)
-    {
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            int e = tid + 256 * q;
-            int r, c;   // r: column offset in [0,BN), c: k offset in [0,BK)
-            if (!TRANS) { c = e % BK; r = e / BK; }
-            else        { r = e % BN; c = e / BN; }
-            int64_t gj = j0 + r, gk = k0 + c;
-            V x = V(0.0);
-            if (gj < n && gk < k) {
-                x = *reinterpret_cast<const V *>(TRANS ? &B[gk * ldb + gj] : &B[gj * ldb + gk]);
-                if constexpr (CONJ) x = conj_vec(x);
-            }
-            v[q] = x;
-        }
-    }
-    template <int LDW>
-    DDI void store(T (*S)[LDW], int tid) const
-    {
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            int e = tid + 256 * q;
-            int r, c;
-            if (!TRANS) { c = e % BK; r = e / BK; }
-            else        { r = e % BN; c = e / BN; }
-            *reinterpret_cast<V *>(&S[c][r]) = v[q];
-        }
-    }
-};
 
 // ----------------------------------------------------------------------------------
 // Real kernel.
@@ -122,92 +30,12 @@ void k_mgemm_plain(int64_t m, int64_t n, int64_t k, ddv alpha, ddv beta, int bet
                    int64_t ldb, ddv *__restrict__ C, int64_t ldc, int tiles_m)
 {
     constexpr int BM = 16 * TM, BN = 16 * TN;
-    static_assert(KCHUNK % BK == 0, "BK must divide the flush period");
     __shared__ __attribute__((aligned(16))) ddv As[BK][BM];
     __shared__ __attribute__((aligned(16))) ddv Bs[BK][BN];
-
-    const int tid = threadIdx.x;
-    const int tx = tid & 15, ty = tid >> 4;
     const int64_t i0 = (int64_t)(blockIdx.x % tiles_m) * BM;
     const int64_t j0 = (int64_t)(blockIdx.x / tiles_m) * BN;
-
-    SlabA<ddv, BM, BK, TA, false> sa;
-    SlabB<ddv, BN, BK, TB, false> sb;
-
-    ddv acc[TM][TN];
-#pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-        for (int j = 0; j < TN; ++j) acc[i][j] = ddv{0.0, 0.0};
-
-    const int64_t ntiles = (k + BK - 1) / BK;
-    sa.load(A, lda, m, k, i0, 0, tid);
-    sb.load(B, ldb, n, k, j0, 0, tid);
-    sa.store(As, tid);
-    sb.store(Bs, tid);
-    __syncthreads();
-
-    for (int64_t t = 0; t < ntiles; ++t) {
-        const int64_t k0 = t * BK;
-        if (t + 1 < ntiles) {
-            sa.load(A, lda, m, k, i0, k0 + BK, tid);
-            sb.load(B, ldb, n, k, j0, k0 + BK, tid);
-        }
-        const int kmax = (k - k0 < BK) ? (int)(k - k0) : BK;
-        if (kmax == BK) {
-#pragma unroll 2
-            for (int kk = 0; kk < BK; ++kk) {
-                ddv a[TM], b[TN];
-#pragma unroll
-                for (int i = 0; i < TM; ++i) a[i] = As[kk][tx + 16 * i];
-#pragma unroll
-                for (int j = 0; j < TN; ++j) b[j] = Bs[kk][ty + 16 * j];
-#pragma unroll
-                for (int i = 0; i < TM; ++i)
-#pragma unroll
-                    for (int j = 0; j < TN; ++j)
-                        acc[i][j] = dd_add_fast(acc[i][j], dd_mul_fast(a[i], b[j]));
-            }
-        } else {
-            for (int kk = 0; kk < kmax; ++kk) {
-                ddv a[TM], b[TN];
-#pragma unroll
-                for (int i = 0; i < TM; ++i) a[i] = As[kk][tx + 16 * i];
-#pragma unroll
-                for (int j = 0; j < TN; ++j) b[j] = Bs[kk][ty + 16 * j];
-#pragma unroll
-                for (int i = 0; i < TM; ++i)
-#pragma unroll
-                    for (int j = 0; j < TN; ++j)
-                        acc[i][j] = dd_add_fast(acc[i][j], dd_mul_fast(a[i], b[j]));
-            }
-        }
-        const int64_t kend = k0 + kmax;
-        if (kend % KCHUNK == 0 || kend == k) {
-            const bool first = (kend <= KCHUNK);
-#pragma unroll
-            for (int i = 0; i < TM; ++i) {
-                const int64_t gi = i0 + tx + 16 * i;
-#pragma unroll
-                for (int j = 0; j < TN; ++j) {
-                    const int64_t gj = j0 + ty + 16 * j;
-                    if (gi < m && gj < n) {
-                        ddv *cp = &C[gj * ldc + gi];
-                        ddv c = *cp;
-                        if (first) c = dd_beta_prepass(c, beta, beta_mode);
-                        *cp = dd_add_exact(c, dd_mul_fast(alpha, acc[i][j]));
-                    }
-                    acc[i][j] = ddv{0.0, 0.0};
-                }
-            }
-        }
-        __syncthreads();
-        if (t + 1 < ntiles) {
-            sa.store(As, tid);
-            sb.store(Bs, tid);
-            __syncthreads();
-        }
-    }
+    mgemm_plain_tile<TM, TN, BK, TA, TB>(m, n, k, alpha, beta, beta_mode, A, lda, B, ldb, C, ldc,
+                                         i0, j0, As, Bs);
 }
 
 // ----------------------------------------------------------------------------------

@@ -74,11 +74,6 @@ def lib():
     L.mwgemm_timer_reset.restype = ctypes.c_int
     L.mwgemm_timer_read.restype = ctypes.c_int
     L.mwgemm_timer_read.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
-    if hasattr(L, "mwgemm_mgemm_batched_device"):
-        f = L.mwgemm_mgemm_batched_device
-        f.restype = ctypes.c_int
-        f.argtypes = [ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                      _vp, _vp, ctypes.c_int, _vp]
     if hasattr(L, "mwgemm_ozaki_config"):
         L.mwgemm_ozaki_config.restype = ctypes.c_int
         L.mwgemm_ozaki_config.argtypes = [ctypes.c_int64, _vp, _vp]
@@ -252,5 +247,84 @@ def testing_residue_gemm(dA, dB, dU, mp, np_, kp, mod_index, stream=None):
                   _vp]
     rc = f(_ptr(dA), _ptr(dB), _ptr(dU), mp, np_, kp, mod_index,
            None if stream is None else ctypes.c_void_p(stream))
+    if rc != 0:
+        _check(rc)
+
+
+def _batched_args(ms, ns, ks, alphas, As, ldas, Bs, ldbs, betas, Cs, ldcs):
+    cnt = len(ms)
+    i64 = lambda v: np.ascontiguousarray(np.asarray(v, dtype=np.int64).reshape(cnt))
+    f2 = lambda v: np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(cnt, 2))
+    ptrs = lambda xs: np.ascontiguousarray(
+        np.array([0 if x is None else x.data_ptr() for x in xs], dtype=np.uint64))
+    return cnt, [i64(ms), i64(ns), i64(ks), f2(alphas), ptrs(As), i64(ldas), ptrs(Bs), i64(ldbs),
+                 f2(betas), ptrs(Cs), i64(ldcs)]
+
+
+def _batched_sig(f, with_plan):
+    f.restype = ctypes.c_int
+    head = [_vp] if with_plan else []
+    f.argtypes = head + [ctypes.c_int64, ctypes.c_char, ctypes.c_char] + [_vp] * 11 + [ctypes.c_int] + ([] if with_plan else [_vp])
+
+
+class BatchedPlan:
+    """A captured batched DD GEMM (see mwgemm_batched_plan_create).  Matrices are device
+    tensors in the (cols, ld, 2) layout; the plan is bound to them."""
+
+    def __init__(self, transa, transb, ms, ns, ks, alphas, As, ldas, Bs, ldbs, betas, Cs, ldcs,
+                 path=PATH_PLAIN):
+        L = lib()
+        _batched_sig(L.mwgemm_batched_plan_create, True)
+        cnt, arrs = _batched_args(ms, ns, ks, alphas, As, ldas, Bs, ldbs, betas, Cs, ldcs)
+        self._keep = arrs
+        self.handle = ctypes.c_void_p()
+        clear_error()
+        rc = L.mwgemm_batched_plan_create(ctypes.byref(self.handle), cnt, transa.encode(),
+                                          transb.encode(), *[_ptr(a) for a in arrs], path)
+        if rc != 0:
+            _check(rc)
+
+    def run(self, stream=None):
+        f = lib().mwgemm_batched_plan_run
+        f.restype = ctypes.c_int
+        f.argtypes = [_vp, _vp]
+        rc = f(self.handle, None if stream is None else ctypes.c_void_p(stream))
+        if rc != 0:
+            _check(rc)
+
+    def info(self):
+        f = lib().mwgemm_batched_plan_info
+        f.restype = ctypes.c_int
+        f.argtypes = [_vp, _vp, _vp, _vp, _vp]
+        p, nm = ctypes.c_int(), ctypes.c_int()
+        t, fl = ctypes.c_int64(), ctypes.c_double()
+        f(self.handle, ctypes.byref(p), ctypes.byref(t), ctypes.byref(fl), ctypes.byref(nm))
+        return {"path": p.value, "tiles": t.value, "flops": fl.value, "moduli": nm.value}
+
+    def close(self):
+        if self.handle:
+            f = lib().mwgemm_batched_plan_destroy
+            f.restype = ctypes.c_int
+            f.argtypes = [_vp]
+            f(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mgemm_batched_device(transa, transb, ms, ns, ks, alphas, As, ldas, Bs, ldbs, betas, Cs,
+                         ldcs, path=PATH_PLAIN, stream=None):
+    """Build, run (no graph) and free a batched plan; synchronises the stream."""
+    L = lib()
+    _batched_sig(L.mwgemm_mgemm_batched_device, False)
+    cnt, arrs = _batched_args(ms, ns, ks, alphas, As, ldas, Bs, ldbs, betas, Cs, ldcs)
+    clear_error()
+    rc = L.mwgemm_mgemm_batched_device(cnt, transa.encode(), transb.encode(),
+                                       *[_ptr(a) for a in arrs], path,
+                                       None if stream is None else ctypes.c_void_p(stream))
     if rc != 0:
         _check(rc)

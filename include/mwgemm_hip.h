@@ -112,6 +112,36 @@ MWGEMM_API int mwgemm_wgemm_device(char transa, char transb, int64_t m, int64_t 
                         const mw_complex64x2 *dB, int64_t ldb, const mw_complex64x2 *beta,
                         mw_complex64x2 *dC, int64_t ldc, int path, void *stream);
 
+/* Batched GEMMs: count independent real DD problems C_p := alpha_p op(A_p) op(B_p) +
+ * beta_p C_p (one op pair for the whole batch; per-problem sizes, leading dimensions,
+ * scalars and device pointers; all host arrays of length count).  A plan validates the
+ * batch, allocates its device table and workspace, and captures the kernel sequence into a
+ * HIP graph: the plain path is one persistent work-queue kernel over all output tiles
+ * (results bit-identical to separate mgemm_ calls), the Ozaki path a fixed sequence of
+ * flattened/persistent kernels.  plan_run replays the graph on a stream; the plan is bound
+ * to the pointers it was created with.  mwgemm_mgemm_batched_device builds, runs (without
+ * a graph) and frees a plan in one call and synchronises the stream.  Return 0 or an error
+ * code.  No counterpart in the reference (its mgemm_ is one problem per call). */
+typedef struct mwgemm_batched_plan_s *mwgemm_batched_plan;
+MWGEMM_API int mwgemm_batched_plan_create(mwgemm_batched_plan *plan, int64_t count, char transa,
+                                          char transb, const int64_t *m, const int64_t *n,
+                                          const int64_t *k, const mw_float64x2 *alpha,
+                                          const mw_float64x2 *const *dA, const int64_t *lda,
+                                          const mw_float64x2 *const *dB, const int64_t *ldb,
+                                          const mw_float64x2 *beta, mw_float64x2 *const *dC,
+                                          const int64_t *ldc, int path);
+MWGEMM_API int mwgemm_batched_plan_run(mwgemm_batched_plan plan, void *stream);
+MWGEMM_API int mwgemm_batched_plan_info(mwgemm_batched_plan plan, int *path, int64_t *tiles,
+                                        double *flops, int *nmod);
+MWGEMM_API int mwgemm_batched_plan_destroy(mwgemm_batched_plan plan);
+MWGEMM_API int mwgemm_mgemm_batched_device(int64_t count, char transa, char transb,
+                                           const int64_t *m, const int64_t *n, const int64_t *k,
+                                           const mw_float64x2 *alpha,
+                                           const mw_float64x2 *const *dA, const int64_t *lda,
+                                           const mw_float64x2 *const *dB, const int64_t *ldb,
+                                           const mw_float64x2 *beta, mw_float64x2 *const *dC,
+                                           const int64_t *ldc, int path, void *stream);
+
 /* Ozaki path parameters.  mwgemm_ozaki_config reports, for contraction depth k (real or
  * complex), how many moduli the exact-product path uses and the integer width b every
  * scaled row/column is rounded to (normwise error bound ~4*2^-b).  mwgemm_ozaki_set sets the
