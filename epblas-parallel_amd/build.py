@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libmwgemm_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["api.cpp", "plain_gemm.hip", "ozaki.hip", "batched.hip"]
+SOURCES = ["api.cpp", "plain_gemm.hip", "ozaki.hip", "batched.hip", "autotune.cpp"]
 
 CFLAGS = [
     "-O3", f"--offload-arch={ARCH}", "-std=c++17", "-fPIC", "-ffp-contract=off",

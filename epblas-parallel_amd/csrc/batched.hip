@@ -152,13 +152,15 @@ void k_batch_split(const BatchProb *__restrict__ probs, UnitSpace us, int bits,
                            rows * p.Kc, r, kg);
 }
 
-// Residue GEMMs: persistent queue over (problem, modulus, tm, tn) tiles of 128 x 128.
+// Residue GEMMs: persistent queue over (problem, modulus, tm, tn) tiles of 64*T x 64*T.
+template <int T>
 __global__ __launch_bounds__(256, 2)
 void k_batch_oz_gemm(const BatchProb *__restrict__ probs, const int4 *__restrict__ tiles,
                      int ntiles, int *__restrict__ counter, const ModConst *__restrict__ mcs)
 {
-    __shared__ __attribute__((aligned(16))) int8_t sa[2][128 * 80];
-    __shared__ __attribute__((aligned(16))) int8_t sb[2][128 * 80];
+    constexpr int BT = 64 * T;
+    __shared__ __attribute__((aligned(16))) int8_t sa[2][BT * 80];
+    __shared__ __attribute__((aligned(16))) int8_t sb[2][BT * 80];
     __shared__ int s_tile;
     for (;;) {
         if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
@@ -168,11 +170,11 @@ void k_batch_oz_gemm(const BatchProb *__restrict__ probs, const int4 *__restrict
         if (t >= ntiles) return;
         const int4 tr = tiles[t];      // (problem, modulus, tm, tn)
         const BatchProb &p = probs[tr.x];
-        const int64_t i0 = (int64_t)tr.z * 128, j0 = (int64_t)tr.w * 128;
+        const int64_t i0 = (int64_t)tr.z * BT, j0 = (int64_t)tr.w * BT;
         const int8_t *Ab = p.resA + (int64_t)tr.y * p.Mp * p.Kc + i0 * p.Kc;
         const int8_t *Bb = p.resB + (int64_t)tr.y * p.Np * p.Kc + j0 * p.Kc;
         uint8_t *Ub = p.U + (int64_t)tr.y * p.Np * p.Mp;
-        oz_gemm_tile<2, 2>(Ab, Bb, Ub, p.Kc, p.Mp, mcs[tr.y], i0, j0, sa, sb);
+        oz_gemm_tile<T, T>(Ab, Bb, Ub, p.Kc, p.Mp, mcs[tr.y], i0, j0, sa, sb);
     }
 }
 
@@ -232,6 +234,7 @@ struct BatchPlan {
     unsigned long long *rmax_base = nullptr;
     int64_t rmax_count = 0;
     int grid_persist = 0;
+    int oz_tile = 128;                  // residue-GEMM tile edge (128 or 64)
     OzPlan oz{};
     ModConst *mcs = nullptr;
     hipGraph_t graph = nullptr;
@@ -294,8 +297,12 @@ static int batch_enqueue(BatchPlan *P, hipStream_t s)
     }
     if (P->ntiles) {
         ScopedTimer tm(KC_BATCH_OZ_GEMM, s);
-        hipLaunchKernelGGL(k_batch_oz_gemm, dim3((unsigned)P->grid_persist), dim3(256), 0, s,
-                           P->probs, P->tiles, P->ntiles, P->counters, P->mcs);
+        if (P->oz_tile == 128)
+            hipLaunchKernelGGL((k_batch_oz_gemm<2>), dim3((unsigned)P->grid_persist), dim3(256), 0, s,
+                               P->probs, P->tiles, P->ntiles, P->counters, P->mcs);
+        else
+            hipLaunchKernelGGL((k_batch_oz_gemm<1>), dim3((unsigned)P->grid_persist), dim3(256), 0, s,
+                               P->probs, P->tiles, P->ntiles, P->counters, P->mcs);
         B_TRY(hipGetLastError());
     }
     if (P->n_crt) {
@@ -323,7 +330,7 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
                       const mw_float64x2 *const *dA, const int64_t *lda,
                       const mw_float64x2 *const *dB, const int64_t *ldb,
                       const mw_float64x2 *beta, mw_float64x2 *const *dC, const int64_t *ldc,
-                      int path)
+                      int path, int oz_variant = -1)
 {
     *out = nullptr;
     auto letter = [](char c) { c = (char)toupper((unsigned char)c); return c == 'N' ? 0 : (c == 'T' || c == 'C') ? 1 : -1; };
@@ -343,7 +350,18 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
         if (act) kmax = std::max(kmax, k[p]);
         flops += 2.0 * (double)m[p] * (double)n[p] * (double)std::max<int64_t>(k[p], 0);
     }
-    if (path == MWGEMM_PATH_AUTO) path = (kmax > 0 && ozaki_supported(kmax) && flops >= 2e9) ? MWGEMM_PATH_OZAKI : MWGEMM_PATH_PLAIN;
+    // AUTO: the autotuned choice for this batch class if one was recorded, else plain (the
+    // padded residue GEMMs do not pay off for problems of a few hundred).
+    const int64_t vol = count > 0 ? (int64_t)std::cbrt(flops / 2.0 / (double)count) : 0;
+    if (path == MWGEMM_PATH_AUTO) {
+        const int tv = tune_lookup(TK_BATCH, count, vol, kmax);
+        if (tv >= 0) {
+            path = tv / 16 ? MWGEMM_PATH_OZAKI : MWGEMM_PATH_PLAIN;
+            if (oz_variant < 0) oz_variant = tv % 16;
+        } else {
+            path = MWGEMM_PATH_PLAIN;
+        }
+    }
     if (path == MWGEMM_PATH_OZAKI && kmax > 0 && !ozaki_supported(kmax)) path = MWGEMM_PATH_PLAIN;
 
     BatchPlan *P = new BatchPlan();
@@ -362,6 +380,9 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
     std::vector<int64_t> ra(count + 1, 0), rb(count + 1, 0), sa(count + 1, 0), sb(count + 1, 0),
         cr(count + 1, 0);
     const bool oz = path == MWGEMM_PATH_OZAKI;
+    if (oz_variant < 0) oz_variant = tune_lookup(TK_BATCH, count, vol, kmax) % 16;
+    P->oz_tile = (oz_variant == 1) ? 64 : 128;
+    const int64_t OT = P->oz_tile;
     if (oz && kmax > 0 && !ozaki_make_plan(kmax, ozaki_target_bits(), ozaki_forced_nmod(), &P->oz)) {
         plan_free(P);
         return record_error(-5, "batched ozaki: no plan for k=%lld", (long long)kmax);
@@ -393,7 +414,7 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
         }
         cr[p + 1] = empty ? 0 : q.n * ((q.m + 3) / 4);
         if (empty || !q.active) continue;
-        q.Mp = rup(q.m, 128); q.Np = rup(q.n, 128); q.Kc = rup(q.k, 64);
+        q.Mp = rup(q.m, OT); q.Np = rup(q.n, OT); q.Kc = rup(q.k, 64);
         offR[p] = rmax_n;
         rmax_n += q.Mp + q.Np;
         offA[p] = ws; ws += rup((int64_t)nmod * q.Mp * q.Kc, 256);
@@ -404,8 +425,8 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
         sa[p + 1] = q.m * (q.Kc / 16);
         sb[p + 1] = q.n * (q.Kc / 16);
         for (int t = 0; t < nmod; ++t)
-            for (int64_t tn = 0; tn < q.Np / 128; ++tn)
-                for (int64_t tm = 0; tm < q.Mp / 128; ++tm) tl.push_back(int4{(int)p, t, (int)tm, (int)tn});
+            for (int64_t tn = 0; tn < q.Np / OT; ++tn)
+                for (int64_t tm = 0; tm < q.Mp / OT; ++tm) tl.push_back(int4{(int)p, t, (int)tm, (int)tn});
     }
     // Longest jobs first: plain tiles cost ~K, residue tiles ~Kc.
     if (!oz)
@@ -474,6 +495,39 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
     if (rc) { plan_free(P); return rc; }
     *out = P;
     return 0;
+}
+
+// Autotuner hook: build a plan (no graph) with an explicit path / residue-tile variant,
+// run it reps times on s and return the best time in milliseconds (or a negative code).
+double batch_time_variant(int64_t count, char ta, char tb, const int64_t *m, const int64_t *n,
+                          const int64_t *k, const mw_float64x2 *alpha,
+                          const mw_float64x2 *const *dA, const int64_t *lda,
+                          const mw_float64x2 *const *dB, const int64_t *ldb,
+                          const mw_float64x2 *beta, mw_float64x2 *const *dC, const int64_t *ldc,
+                          int path, int oz_variant, int reps, hipStream_t s)
+{
+    BatchPlan *P = nullptr;
+    if (int rc = plan_build(&P, count, ta, tb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc, path, oz_variant))
+        return (double)(rc < 0 ? rc : -rc);
+    hipEvent_t e0, e1;
+    double best = -1.0;
+    if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+        if (batch_enqueue(P, s) == 0 && hipStreamSynchronize(s) == hipSuccess) {
+            for (int r = 0; r < reps; ++r) {
+                float ms = 0.f;
+                if (hipEventRecord(e0, s) != hipSuccess || batch_enqueue(P, s) != 0 ||
+                    hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                    break;
+                if (best < 0 || ms < best) best = ms;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    (void)hipStreamSynchronize(s);
+    plan_free(P);
+    return best;
 }
 
 }  // namespace mw

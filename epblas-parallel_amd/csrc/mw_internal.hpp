@@ -18,6 +18,7 @@ struct GemmArgsR {
     const ddv *A; int64_t lda;
     const ddv *B; int64_t ldb;
     ddv *C; int64_t ldc;
+    int variant = -1;        // tile variant; -1: autotuned choice or heuristic
 };
 
 struct GemmArgsZ {
@@ -28,7 +29,23 @@ struct GemmArgsZ {
     const zddv *A; int64_t lda;
     const zddv *B; int64_t ldb;
     zddv *C; int64_t ldc;
+    int variant = -1;
 };
+
+// Autotuner table (autotune.cpp).  Kinds of tuned decisions:
+enum TuneKind {
+    TK_PLAIN_REAL = 0,   // variant: 0 = 64x64 tile, 1 = 32x64 tile
+    TK_PLAIN_CPLX = 1,   // variant: 0 = 32x32 tile
+    TK_OZ_GEMM = 2,      // variant: 0 = 256x256 residue tile, 1 = 128x128
+    TK_BATCH = 3,        // variant: path * 16 + residue-tile variant (0 = 128, 1 = 64)
+    TK_COUNT
+};
+// Tuned variant for a shape class, or -1 when none has been recorded.
+int tune_lookup(int kind, int64_t a, int64_t b, int64_t c);
+void tune_record(int kind, int64_t a, int64_t b, int64_t c, int variant, double ms);
+constexpr int PLAIN_REAL_VARIANTS = 2;
+constexpr int OZ_GEMM_VARIANTS = 2;
+constexpr int BATCH_OZ_VARIANTS = 2;
 
 // plain_gemm.hip
 hipError_t plain_gemm_real(const GemmArgsR &g, hipStream_t s);

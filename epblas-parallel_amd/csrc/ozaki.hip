@@ -160,6 +160,10 @@ bool ozaki_make_plan(int64_t k_eff, int target_bits, int force_nmod, OzPlan *p)
         ModConst &c = p->mc[t];
         c.m = (double)m;
         c.inv_m = 1.0 / (double)m;
+        c.inv_up = std::nextafter(c.inv_m, 1.0);
+        if (c.inv_up * (double)m < 1.0) c.inv_up = std::nextafter(c.inv_up, 1.0);
+        c.off_acc = std::ldexp((double)m, 24);
+        c.off_split = std::ldexp((double)m, 39);
         c.y = (double)mod_inverse(others, m);
         c.p1 = (double)mod_pow2(35, m);
         c.p2 = (double)mod_pow2(70, m);
@@ -266,16 +270,16 @@ void k_oz_gemm(const int8_t *__restrict__ A, const int8_t *__restrict__ B,
                uint8_t *__restrict__ U, int64_t kp, int64_t ldu, int64_t sA, int64_t sB,
                int64_t sU, const ModConst *__restrict__ mcs, int per, int tiles_m)
 {
-    constexpr int BM = 64 * TM, BN = 64 * TN;
-    __shared__ __attribute__((aligned(16))) int8_t sa[2][BM * 80];
-    __shared__ __attribute__((aligned(16))) int8_t sb[2][BN * 80];
+    constexpr int BM = 64 * TM, BN = 64 * TN, BK = (TM * TN >= 16) ? 128 : 64;
+    __shared__ __attribute__((aligned(16))) int8_t sa[2][BM * (BK + 16)];
+    __shared__ __attribute__((aligned(16))) int8_t sb[2][BN * (BK + 16)];
     const int batch = blockIdx.y;
     int tm, tn;
     tile_order(blockIdx.x, tiles_m, (int)gridDim.x / tiles_m, tm, tn);
     const int64_t i0 = (int64_t)tm * BM;
     const int64_t j0 = (int64_t)tn * BN;
-    oz_gemm_tile<TM, TN>(A + batch * sA + i0 * kp, B + batch * sB + j0 * kp, U + batch * sU, kp,
-                         ldu, mcs[batch / per], i0, j0, sa, sb);
+    oz_gemm_tile<TM, TN, BK>(A + batch * sA + i0 * kp, B + batch * sB + j0 * kp, U + batch * sU,
+                             kp, ldu, mcs[batch / per], i0, j0, sa, sb);
 }
 
 // Reconstruction kernel.  Each lane owns 4 consecutive rows of one column.
@@ -393,13 +397,16 @@ hipError_t launch_split(const T *X, int64_t ld, int64_t R, int64_t K, int64_t kc
     return hipGetLastError();
 }
 
+// Variant 0: 256 x 256 tiles, 1: 128 x 128.  Without a tuned choice, 256 x 256 when there
+// are enough tiles to give every CU two.
 hipError_t launch_gemm(const int8_t *A, const int8_t *B, uint8_t *U, int64_t mp, int64_t np,
                        int64_t kp, int64_t sA, int64_t sB, int64_t sU, const ModConst *mcs,
-                       int per, int batches, hipStream_t s)
+                       int per, int batches, int variant, hipStream_t s)
 {
-    // 256 x 256 tiles when there are enough of them to fill the chip, else 128 x 128.
-    const bool big = (mp / 256) * (np / 256) * (int64_t)batches >= 512 && getenv("MWGEMM_OZ_SMALL_TILE") == nullptr;
-    if (big) {
+    int v = variant;
+    if (v < 0) v = tune_lookup(TK_OZ_GEMM, mp, np, kp);
+    if (v < 0) v = (mp / 256) * (np / 256) * (int64_t)batches >= 512 ? 0 : 1;
+    if (v == 0) {
         const int tiles_m = (int)(mp / 256);
         dim3 grid((unsigned)(tiles_m * (np / 256)), (unsigned)batches);
         hipLaunchKernelGGL((k_oz_gemm<4, 4>), grid, dim3(256), 0, s, A, B, U, kp, mp, sA, sB, sU,
@@ -432,7 +439,8 @@ int ozaki_gemm_real(const GemmArgsR &g, hipStream_t s)
     ModConst *mcs;
     if (int rc = oz_table(d, plan, &mcs)) return rc;
 
-    const int64_t Mp = round_up(g.m, 256), Np = round_up(g.n, 256), Kc = round_up(g.k, 64);
+    // K padded to 128: the 256 x 256 residue tile consumes 128-deep slabs.
+    const int64_t Mp = round_up(g.m, 256), Np = round_up(g.n, 256), Kc = round_up(g.k, 128);
     const int nmod = plan.nmod;
     const size_t szA = (size_t)nmod * Mp * Kc, szB = (size_t)nmod * Np * Kc;
     const size_t szU = (size_t)nmod * Np * Mp, szR = (size_t)(Mp + Np) * 8;
@@ -461,7 +469,7 @@ int ozaki_gemm_real(const GemmArgsR &g, hipStream_t s)
     }
     {
         ScopedTimer tm(KC_OZ_GEMM, s);
-        OZ_TRY(launch_gemm(resA, resB, Ub, Mp, Np, Kc, Mp * Kc, Np * Kc, Np * Mp, mcs, 1, nmod, s));
+        OZ_TRY(launch_gemm(resA, resB, Ub, Mp, Np, Kc, Mp * Kc, Np * Kc, Np * Mp, mcs, 1, nmod, g.variant, s));
     }
     {
         ScopedTimer tm(KC_OZ_CRT, s);
@@ -523,7 +531,7 @@ int ozaki_gemm_cplx(const GemmArgsZ &g, hipStream_t s)
     }
     {
         ScopedTimer tm(KC_OZ_GEMM, s);
-        OZ_TRY(launch_gemm(resA, resB, Ub, Mp, Np, Kp, bsA, bsB, Np * Mp, mcs, 2, 2 * nmod, s));
+        OZ_TRY(launch_gemm(resA, resB, Ub, Mp, Np, Kp, bsA, bsB, Np * Mp, mcs, 2, 2 * nmod, g.variant, s));
     }
     {
         ScopedTimer tm(KC_OZ_CRT, s);
@@ -549,7 +557,7 @@ int ozaki_residue_gemm_test(const int8_t *dA, const int8_t *dB, uint8_t *dU, int
     if (int rc = oz_device(&d)) return rc;
     ModConst *mcs;
     if (int rc = oz_table(d, plan, &mcs)) return rc;
-    OZ_TRY(launch_gemm(dA, dB, dU, mp, np, kp, 0, 0, 0, mcs + mod_index, 1, 1, s));
+    OZ_TRY(launch_gemm(dA, dB, dU, mp, np, kp, 0, 0, 0, mcs + mod_index, 1, 1, 1, s));
     return 0;
 }
 

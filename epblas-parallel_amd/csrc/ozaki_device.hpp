@@ -28,9 +28,10 @@ DDI int row_sigma(unsigned long long bits, int b, bool &bad)
 }
 
 // The integer round(x * 2^sigma) of a DD value as three exact FP64 chunks with
-// value = c0 + c1*2^35 + c2*2^70 (|c0| < 2^52, |c1| < 2^35, |c2| <= 2^34 for b <= 104).
-// The chunks are cut from |xh| and then signed: the low bits of a 53-bit significand are
-// always representable, whereas xh mod 2^70 of a negative xh generally is not.
+// value = c0 + c1*2^35 + c2*2^70, 0 <= c0 < 2^35, |c1| < 2^36, |c2| <= 2^34 (b <= 104).
+// The chunks are cut from |xh| and then signed (the low bits of a 53-bit significand are
+// always representable, whereas xh mod 2^70 of a negative xh generally is not); the low
+// integer xl is then folded into c0 with a carry into c1.
 DDI void int_chunks(double hi, double lo, int sigma, double &c0, double &c1, double &c2)
 {
     const double xh_s = ldexp(hi, sigma), xl_s = ldexp(lo, sigma);
@@ -42,20 +43,22 @@ DDI void int_chunks(double hi, double lo, int sigma, double &c0, double &c1, dou
     const double a1 = floor(rem * 0x1p-35);
     const double a0 = __builtin_fma(-a1, 0x1p35, rem);
     const double sg = xh < 0.0 ? -1.0 : 1.0;
+    const double t0 = __builtin_fma(sg, a0, xl);          // |t0| < 2^52, exact
+    const double carry = floor(t0 * 0x1p-35);
+    c0 = __builtin_fma(-carry, 0x1p35, t0);
+    c1 = __builtin_fma(sg, a1, carry);
     c2 = sg * a2;
-    c1 = sg * a1;
-    c0 = __builtin_fma(sg, a0, xl);
 }
 
 // Symmetric residue of c0 + c1*2^35 + c2*2^70 modulo m, in [-(m-1)/2, m/2] (int8 range).
+// The combination is shifted by m*2^39 (> its magnitude) so that it is a non-negative
+// integer below 2^48, for which floor(s * inv_up) is the exact quotient: no fix-ups.
 DDI int sym_residue(double c0, double c1, double c2, const ModConst &mc)
 {
     const double m = mc.m;
-    const double s = __builtin_fma(c2, mc.p2, __builtin_fma(c1, mc.p1, c0));
-    const double q = floor(s * mc.inv_m);
+    const double s = __builtin_fma(c2, mc.p2, __builtin_fma(c1, mc.p1, c0 + mc.off_split));
+    const double q = floor(s * mc.inv_up);
     double r = __builtin_fma(-q, m, s);
-    r = (r < 0.0) ? r + m : r;
-    r = (r >= m) ? r - m : r;
     r = (r > 0.5 * (m - 1.0)) ? r - m : r;
     return (int)r;
 }
@@ -108,20 +111,15 @@ DDI void oz_split_unit(const T *__restrict__ X, int64_t ld, int64_t K,
     }
 }
 
-// u = ((v mod m) * y) mod m for an exact INT32 dot product v.
-DDI uint32_t crt_weight_residue(int v, double m, double inv_m, double y)
+// u = ((v mod m) * y) mod m for an exact INT32 dot product v (|v| <= 2^30): shifted by
+// m*2^24 to a non-negative integer below 2^33, both quotients are exact with inv_up.
+DDI uint32_t crt_weight_residue(int v, const ModConst &mc)
 {
-    double d = (double)v;
-    double q = floor(d * inv_m);
-    double r = __builtin_fma(-q, m, d);
-    r = (r < 0.0) ? r + m : r;
-    r = (r >= m) ? r - m : r;
-    double w = r * y;
-    q = floor(w * inv_m);
-    double u = __builtin_fma(-q, m, w);
-    u = (u < 0.0) ? u + m : u;
-    u = (u >= m) ? u - m : u;
-    return (uint32_t)u;
+    const double m = mc.m;
+    const double d = (double)v + mc.off_acc;
+    const double r = __builtin_fma(-floor(d * mc.inv_up), m, d);
+    const double w = r * mc.y;
+    return (uint32_t)__builtin_fma(-floor(w * mc.inv_up), m, w);
 }
 
 // Workgroup -> output tile.  Workgroups are dispatched round-robin over the 8 XCDs (each
@@ -146,37 +144,40 @@ DDI void tile_order(int bid, int tiles_m, int tiles_n, int &tm, int &tn)
 // B_b: [Np][kp] int8, U_b column-major [Np][ldu] uint8.  Four wave64 per workgroup in a
 // 2 x 2 arrangement, each wave owns TM x TN MFMA tiles of 32 x 32 (v_mfma_i32_32x32x32_i8);
 // 64-deep K slabs double-buffered in LDS with 80-byte row pitch (conflict-free b128 reads).
-template <int TM, int TN>
+template <int TM, int TN, int BK = 64>
 DDI void oz_gemm_tile(const int8_t *__restrict__ Ab, const int8_t *__restrict__ Bb,
                       uint8_t *__restrict__ Ub, int64_t kp, int64_t ldu, const ModConst &mc,
-                      int64_t i0, int64_t j0, int8_t (&sa)[2][64 * TM * 80],
-                      int8_t (&sb)[2][64 * TN * 80])
+                      int64_t i0, int64_t j0, int8_t (&sa)[2][64 * TM * (BK + 16)],
+                      int8_t (&sb)[2][64 * TN * (BK + 16)])
 {
-    constexpr int PITCH = 80;
+    // Row pitch BK+16 bytes: 16 consecutive rows land in 16 distinct 16-byte bank slots
+    // ((BK+16)/16 is odd), so the MFMA fragment reads are conflict-free.
+    constexpr int PITCH = BK + 16, CPR = BK / 16;     // 16-byte chunks per row
+    constexpr int LA = TM * BK / 64, LB = TN * BK / 64;   // 16-byte loads per lane and slab
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wm = w & 1, wn = w >> 1;
 
-    v4i ra[TM], rb[TN];
+    v4i ra[LA], rb[LB];
 #define OZ_GLOAD(K0)                                                                          \
     do {                                                                                      \
-        _Pragma("unroll") for (int q = 0; q < TM; ++q) {                                      \
+        _Pragma("unroll") for (int q = 0; q < LA; ++q) {                                      \
             const int c = tid + 256 * q;                                                      \
-            ra[q] = *reinterpret_cast<const v4i *>(Ab + (int64_t)(c >> 2) * kp + (K0) + (c & 3) * 16); \
+            ra[q] = *reinterpret_cast<const v4i *>(Ab + (int64_t)(c / CPR) * kp + (K0) + (c % CPR) * 16); \
         }                                                                                     \
-        _Pragma("unroll") for (int q = 0; q < TN; ++q) {                                      \
+        _Pragma("unroll") for (int q = 0; q < LB; ++q) {                                      \
             const int c = tid + 256 * q;                                                      \
-            rb[q] = *reinterpret_cast<const v4i *>(Bb + (int64_t)(c >> 2) * kp + (K0) + (c & 3) * 16); \
+            rb[q] = *reinterpret_cast<const v4i *>(Bb + (int64_t)(c / CPR) * kp + (K0) + (c % CPR) * 16); \
         }                                                                                     \
     } while (0)
 #define OZ_LSTORE(BUF)                                                                        \
     do {                                                                                      \
-        _Pragma("unroll") for (int q = 0; q < TM; ++q) {                                      \
+        _Pragma("unroll") for (int q = 0; q < LA; ++q) {                                      \
             const int c = tid + 256 * q;                                                      \
-            *reinterpret_cast<v4i *>(&sa[BUF][(c >> 2) * PITCH + (c & 3) * 16]) = ra[q];     \
+            *reinterpret_cast<v4i *>(&sa[BUF][(c / CPR) * PITCH + (c % CPR) * 16]) = ra[q];   \
         }                                                                                     \
-        _Pragma("unroll") for (int q = 0; q < TN; ++q) {                                      \
+        _Pragma("unroll") for (int q = 0; q < LB; ++q) {                                      \
             const int c = tid + 256 * q;                                                      \
-            *reinterpret_cast<v4i *>(&sb[BUF][(c >> 2) * PITCH + (c & 3) * 16]) = rb[q];     \
+            *reinterpret_cast<v4i *>(&sb[BUF][(c / CPR) * PITCH + (c % CPR) * 16]) = rb[q];   \
         }                                                                                     \
     } while (0)
 
@@ -186,15 +187,15 @@ DDI void oz_gemm_tile(const int8_t *__restrict__ Ab, const int8_t *__restrict__ 
 #pragma unroll
         for (int b = 0; b < TN; ++b) acc[a][b] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 
-    const int64_t nk = kp / 64;
+    const int64_t nk = kp / BK;
     OZ_GLOAD(0);
     OZ_LSTORE(0);
     __syncthreads();
     for (int64_t kt = 0; kt < nk; ++kt) {
         const int cur = (int)(kt & 1);
-        if (kt + 1 < nk) OZ_GLOAD((kt + 1) * 64);
+        if (kt + 1 < nk) OZ_GLOAD((kt + 1) * BK);
 #pragma unroll
-        for (int ks = 0; ks < 2; ++ks) {
+        for (int ks = 0; ks < BK / 32; ++ks) {
             v4i af[TM], bf[TN];
 #pragma unroll
             for (int a = 0; a < TM; ++a) {
@@ -218,7 +219,6 @@ DDI void oz_gemm_tile(const int8_t *__restrict__ Ab, const int8_t *__restrict__ 
 
 #undef OZ_GLOAD
 #undef OZ_LSTORE
-    const double m = mc.m, inv_m = mc.inv_m, y = mc.y;
 #pragma unroll
     for (int a = 0; a < TM; ++a)
 #pragma unroll
@@ -230,7 +230,7 @@ DDI void oz_gemm_tile(const int8_t *__restrict__ Ab, const int8_t *__restrict__ 
                 uint32_t word = 0;
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    word |= crt_weight_residue(acc[a][b][4 * g + q], m, inv_m, y) << (8 * q);
+                    word |= crt_weight_residue(acc[a][b][4 * g + q], mc) << (8 * q);
                 *reinterpret_cast<uint32_t *>(Ub + col * ldu + rowb + 8 * g) = word;
             }
         }

@@ -152,18 +152,31 @@ namespace {
 constexpr int RTM = 4, RTN = 4, RBK = 16;   // real tile: 64 x 64, 16-deep slabs
 constexpr int CTM = 2, CTN = 2, CBK = 16;   // complex tile: 32 x 32
 
-template <bool TA, bool TB>
-hipError_t launch_real(const GemmArgsR &g, hipStream_t s)
+template <int TM, int TN, bool TA, bool TB>
+hipError_t launch_real_tile(const GemmArgsR &g, hipStream_t s)
 {
-    constexpr int BM = 16 * RTM, BN = 16 * RTN;
+    constexpr int BM = 16 * TM, BN = 16 * TN;
     int tiles_m = (int)((g.m + BM - 1) / BM);
     int64_t tiles = (int64_t)tiles_m * ((g.n + BN - 1) / BN);
     if (tiles > INT32_MAX) return hipErrorInvalidValue;
     ScopedTimer tm(KC_PLAIN_REAL, s);
-    hipLaunchKernelGGL((k_mgemm_plain<RTM, RTN, RBK, TA, TB>), dim3((unsigned)tiles), dim3(256),
+    hipLaunchKernelGGL((k_mgemm_plain<TM, TN, RBK, TA, TB>), dim3((unsigned)tiles), dim3(256),
                        0, s, g.m, g.n, g.k, g.alpha, g.beta, g.beta_mode, g.A, g.lda, g.B,
                        g.ldb, g.C, g.ldc, tiles_m);
     return hipGetLastError();
+}
+
+// Variant 0: 64 x 64 tiles (4 x 4 per lane); variant 1: 32 x 64 (2 x 4 per lane), which
+// keeps more workgroups in flight on small products.  Without a tuned choice, variant 1 is
+// used when 64 x 64 tiles would not give every CU two workgroups.
+template <bool TA, bool TB>
+hipError_t launch_real(const GemmArgsR &g, hipStream_t s)
+{
+    int v = g.variant;
+    if (v < 0) v = tune_lookup(TK_PLAIN_REAL, g.m, g.n, g.k);
+    if (v < 0) v = ((g.m + 63) / 64) * ((g.n + 63) / 64) < 512 ? 1 : 0;
+    if (v == 1) return launch_real_tile<2, 4, TA, TB>(g, s);
+    return launch_real_tile<RTM, RTN, TA, TB>(g, s);
 }
 
 template <int OA, int OB>

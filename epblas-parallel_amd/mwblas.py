@@ -328,3 +328,49 @@ def mgemm_batched_device(transa, transb, ms, ns, ks, alphas, As, ldas, Bs, ldbs,
                                        None if stream is None else ctypes.c_void_p(stream))
     if rc != 0:
         _check(rc)
+
+
+TUNE_PLAIN_REAL, TUNE_PLAIN_CPLX, TUNE_OZ_GEMM, TUNE_BATCH = 0, 1, 2, 3
+
+
+def autotune(m, n, k, cplx=False, path=PATH_PLAIN):
+    """Time the tile variants of a path on synthetic m x n x k operands; returns
+    (winning variant, its milliseconds) and records the decision."""
+    f = lib().mwgemm_autotune
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                  _vp, _vp]
+    v, ms = ctypes.c_int(), ctypes.c_double()
+    clear_error()
+    rc = f(m, n, k, 1 if cplx else 0, path, ctypes.byref(v), ctypes.byref(ms))
+    if rc != 0:
+        _check(rc)
+    return v.value, ms.value
+
+
+def batched_autotune(transa, transb, ms, ns, ks):
+    """Tune a batch class (plain vs Ozaki residue tiles); returns (choice, milliseconds) with
+    choice = path * 16 + residue-tile variant."""
+    f = lib().mwgemm_batched_autotune
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int64, ctypes.c_char, ctypes.c_char, _vp, _vp, _vp, _vp, _vp]
+    cnt = len(ms)
+    arr = [np.ascontiguousarray(np.asarray(x, dtype=np.int64)) for x in (ms, ns, ks)]
+    c, t = ctypes.c_int(), ctypes.c_double()
+    clear_error()
+    rc = f(cnt, transa.encode(), transb.encode(), *[_ptr(a) for a in arr], ctypes.byref(c),
+           ctypes.byref(t))
+    if rc != 0:
+        _check(rc)
+    return c.value, t.value
+
+
+def autotune_lookup(kind, a, b, c):
+    f = lib().mwgemm_autotune_lookup
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]
+    return f(kind, a, b, c)
+
+
+def autotune_clear():
+    lib().mwgemm_autotune_clear()

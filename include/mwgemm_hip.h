@@ -142,6 +142,24 @@ MWGEMM_API int mwgemm_mgemm_batched_device(int64_t count, char transa, char tran
                                            const mw_float64x2 *beta, mw_float64x2 *const *dC,
                                            const int64_t *ldc, int path, void *stream);
 
+/* Autotuner.  mwgemm_autotune times every tile variant of the given path (PLAIN or OZAKI)
+ * on synthetic device operands of shape m x n x k (allocated and freed here) and records the
+ * fastest for that shape class; mwgemm_batched_autotune does the same for a batch (plain
+ * vs Ozaki with 128 or 64 residue tiles), and plans created with path AUTO then use the
+ * recorded choice.  Decisions persist across processes through the file named by
+ * MWGEMM_TUNE_FILE (appended on every record, read on first use).  *variant / *choice
+ * receive the winner (batch: path * 16 + residue-tile variant), *best_ms its time.
+ * mwgemm_autotune_lookup returns the recorded variant for a kind (0 plain real, 1 plain
+ * complex, 2 Ozaki residue GEMM, 3 batch) and shape (a, b, c), or -1; mwgemm_autotune_clear
+ * forgets all decisions.  Return 0 or an error code. */
+MWGEMM_API int mwgemm_autotune(int64_t m, int64_t n, int64_t k, int cplx, int path, int *variant,
+                               double *best_ms);
+MWGEMM_API int mwgemm_batched_autotune(int64_t count, char transa, char transb, const int64_t *m,
+                                       const int64_t *n, const int64_t *k, int *choice,
+                                       double *best_ms);
+MWGEMM_API int mwgemm_autotune_lookup(int kind, int64_t a, int64_t b, int64_t c);
+MWGEMM_API int mwgemm_autotune_clear(void);
+
 /* Ozaki path parameters.  mwgemm_ozaki_config reports, for contraction depth k (real or
  * complex), how many moduli the exact-product path uses and the integer width b every
  * scaled row/column is rounded to (normwise error bound ~4*2^-b).  mwgemm_ozaki_set sets the

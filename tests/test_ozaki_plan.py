@@ -68,3 +68,64 @@ def test_fixed_point_crt_reconstruction_identity():
             F -= 2 ** 128
         est = F * M / 2 ** 128       # exact rational
         assert abs(est - X) <= M / 2 ** 115
+
+
+def _inv_up(m):
+    import numpy as np
+    return float(np.nextafter(1.0 / m, 2.0))
+
+
+def test_rounded_up_reciprocal_gives_exact_quotients():
+    """The residue kernels take floor(d * RU(1/m)) as the exact quotient floor(d/m) for
+    non-negative integers d below 2^33 (GEMM epilogue) and 2^48 (split); check it on random
+    and worst-case (multiples of m, one below a multiple) integers for every modulus."""
+    m_all, _ = mwblas.ozaki_moduli(40)
+    rng = random.Random(9)
+    for m in (int(v) for v in m_all):
+        inv = _inv_up(m)
+        for lim in (2 ** 33, 2 ** 48):
+            cases = [rng.randrange(0, lim) for _ in range(3000)]
+            top = lim // m
+            cases += [q * m for q in (0, 1, 2, top - 1, top)] + [q * m - 1 for q in (1, 2, top)]
+            cases += [rng.randrange(0, top) * m + d for d in (0, m - 1) for _ in range(200)]
+            for d in cases:
+                if d < 0 or d >= lim:
+                    continue
+                assert math.floor(float(d) * inv) == d // m, (m, d)
+
+
+def _chunks(hi, lo, sigma):
+    """The split's integer chunking, step by step in binary64 (every step is exact)."""
+    import numpy as np
+    xh_s, xl_s = math.ldexp(hi, sigma), math.ldexp(lo, sigma)
+    xh = float(np.rint(xh_s))
+    xl = float(np.rint((xh_s - xh) + xl_s))
+    ax = abs(xh)
+    a2 = math.floor(ax * 2.0 ** -70)
+    rem = ax - a2 * 2.0 ** 70
+    a1 = math.floor(rem * 2.0 ** -35)
+    a0 = rem - a1 * 2.0 ** 35
+    sg = -1.0 if xh < 0 else 1.0
+    t0 = sg * a0 + xl
+    carry = math.floor(t0 * 2.0 ** -35)
+    return t0 - carry * 2.0 ** 35, sg * a1 + carry, sg * a2, xh_s, xl_s
+
+
+def test_split_chunks_are_exact_and_bounded():
+    from fractions import Fraction
+    import numpy as np
+    import ddgen
+    rng = np.random.default_rng(10)
+    b = 103
+    for kind in ("uniform", "wide"):
+        vals = ddgen.real_values(rng, (4000,), kind=kind)
+        mx = float(np.max(np.abs(vals[:, 0]) + np.abs(vals[:, 1])))
+        sigma = b - math.frexp(mx)[1]
+        for hi, lo in vals:
+            c0, c1, c2, xh_s, xl_s = _chunks(float(hi), float(lo), sigma)
+            assert 0 <= c0 < 2 ** 35 and abs(c1) < 2 ** 36 and abs(c2) <= 2 ** 34
+            X = int(c0) + int(c1) * 2 ** 35 + int(c2) * 2 ** 70
+            assert abs(X) <= 2 ** b
+            assert abs(Fraction(X) - Fraction(xh_s) - Fraction(xl_s)) <= Fraction(5, 8)
+            # the shifted combination stays below 2^48 for every modulus weight < 256
+            assert abs(c0) + abs(c1) * 255 + abs(c2) * 255 < 2 ** 45
