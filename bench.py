@@ -156,6 +156,8 @@ def main():
     alpha, beta = [1.0, 0.0], [0.0, 0.0]
     stream = torch.cuda.current_stream().cuda_stream
     nmod, bits = mwblas.ozaki_config(k)
+    # the library's autotuner picks the residue-GEMM tile for this shape class
+    tile_variant, _, tile_ms = mwblas.autotune(m, n, k, path=mwblas.PATH_OZAKI, all_times=True)
 
     def step(path, out):
         mwblas.gemm_device(False, "N", "N", m, n, k, alpha, A, m, B, k, beta, out, m,
@@ -226,7 +228,9 @@ def main():
         "config": {"workload": f"ozaki_int8_real_dd_gemm_{m}", "m": m, "n": n, "k": k,
                    "trans": "NN", "alpha": alpha, "beta": beta,
                    "inputs": "hi=+-U[1,2)*2^U_int[-20,20], lo=hi*2^-53*U[-1/2,1/2]",
-                   "moduli": nmod, "bits": bits, "parallelism": "single-gpu"},
+                   "moduli": nmod, "bits": bits, "parallelism": "single-gpu",
+                   "residue_tile_variant": tile_variant,
+                   "autotune_ms_per_variant": [round(x, 3) for x in tile_ms]},
         "plain_fp64": None if not t_plain else {
             "value": round(plain_gflops, 2), "unit": "DD-GFLOP/s",
             "ms_per_step": round(t_plain * 1e3, 2),

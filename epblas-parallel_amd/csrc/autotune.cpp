@@ -125,7 +125,7 @@ using namespace mw;
 extern "C" {
 
 int mwgemm_autotune(int64_t m, int64_t n, int64_t k, int cplx, int path, int *variant,
-                    double *best_ms)
+                    double *best_ms, double *all_ms, int all_cap)
 {
     if (m <= 0 || n <= 0 || k <= 0 || (path != MWGEMM_PATH_PLAIN && path != MWGEMM_PATH_OZAKI))
         return record_error(-3, "autotune: bad shape or path");
@@ -178,6 +178,7 @@ int mwgemm_autotune(int64_t m, int64_t n, int64_t k, int cplx, int path, int *va
             }
             if (rep > 0 && (t_v < 0 || ms < t_v)) t_v = ms;   // rep 0 warms up
         }
+        if (!rc && all_ms && v < all_cap) all_ms[v] = t_v;
         if (!rc && (best < 0 || t_v < best_t)) { best = v; best_t = t_v; }
     }
     if (!rc && best >= 0) {

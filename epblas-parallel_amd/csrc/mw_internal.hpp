@@ -36,7 +36,7 @@ struct GemmArgsZ {
 enum TuneKind {
     TK_PLAIN_REAL = 0,   // variant: 0 = 64x64 tile, 1 = 32x64 tile
     TK_PLAIN_CPLX = 1,   // variant: 0 = 32x32 tile
-    TK_OZ_GEMM = 2,      // variant: 0 = 256x256 residue tile, 1 = 128x128
+    TK_OZ_GEMM = 2,      // variant: residue tile 0 = 256x256, 1 = 128x128, 2 = 256x128, 3 = 128x256
     TK_BATCH = 3,        // variant: path * 16 + residue-tile variant (0 = 128, 1 = 64)
     TK_COUNT
 };
@@ -44,7 +44,7 @@ enum TuneKind {
 int tune_lookup(int kind, int64_t a, int64_t b, int64_t c);
 void tune_record(int kind, int64_t a, int64_t b, int64_t c, int variant, double ms);
 constexpr int PLAIN_REAL_VARIANTS = 2;
-constexpr int OZ_GEMM_VARIANTS = 2;
+constexpr int OZ_GEMM_VARIANTS = 4;
 constexpr int BATCH_OZ_VARIANTS = 2;
 
 // plain_gemm.hip

@@ -397,8 +397,22 @@ hipError_t launch_split(const T *X, int64_t ld, int64_t R, int64_t K, int64_t kc
     return hipGetLastError();
 }
 
-// Variant 0: 256 x 256 tiles, 1: 128 x 128.  Without a tuned choice, 256 x 256 when there
-// are enough tiles to give every CU two.
+template <int TM, int TN>
+hipError_t launch_gemm_tile(const int8_t *A, const int8_t *B, uint8_t *U, int64_t mp, int64_t np,
+                            int64_t kp, int64_t sA, int64_t sB, int64_t sU, const ModConst *mcs,
+                            int per, int batches, hipStream_t s)
+{
+    const int tiles_m = (int)(mp / (64 * TM));
+    dim3 grid((unsigned)(tiles_m * (np / (64 * TN))), (unsigned)batches);
+    hipLaunchKernelGGL((k_oz_gemm<TM, TN>), grid, dim3(256), 0, s, A, B, U, kp, mp, sA, sB, sU,
+                       mcs, per, tiles_m);
+    return hipGetLastError();
+}
+
+// Variants: 0 = 256 x 256 tiles with 128-deep slabs (one workgroup per CU), 1 = 128 x 128,
+// 2 = 256 x 128 and 3 = 128 x 256 (64-deep slabs, two workgroups per CU so one fills the
+// other's barrier bubbles).  kp must be a multiple of 128 for variant 0.  Without a tuned
+// choice, 256 x 256 when there are enough tiles to give every CU two, else 128 x 128.
 hipError_t launch_gemm(const int8_t *A, const int8_t *B, uint8_t *U, int64_t mp, int64_t np,
                        int64_t kp, int64_t sA, int64_t sB, int64_t sU, const ModConst *mcs,
                        int per, int batches, int variant, hipStream_t s)
@@ -406,18 +420,13 @@ hipError_t launch_gemm(const int8_t *A, const int8_t *B, uint8_t *U, int64_t mp,
     int v = variant;
     if (v < 0) v = tune_lookup(TK_OZ_GEMM, mp, np, kp);
     if (v < 0) v = (mp / 256) * (np / 256) * (int64_t)batches >= 512 ? 0 : 1;
-    if (v == 0) {
-        const int tiles_m = (int)(mp / 256);
-        dim3 grid((unsigned)(tiles_m * (np / 256)), (unsigned)batches);
-        hipLaunchKernelGGL((k_oz_gemm<4, 4>), grid, dim3(256), 0, s, A, B, U, kp, mp, sA, sB, sU,
-                           mcs, per, tiles_m);
-    } else {
-        const int tiles_m = (int)(mp / 128);
-        dim3 grid((unsigned)(tiles_m * (np / 128)), (unsigned)batches);
-        hipLaunchKernelGGL((k_oz_gemm<2, 2>), grid, dim3(256), 0, s, A, B, U, kp, mp, sA, sB, sU,
-                           mcs, per, tiles_m);
+    if (v == 0 && kp % 128) v = 2;
+    switch (v) {
+    case 0: return launch_gemm_tile<4, 4>(A, B, U, mp, np, kp, sA, sB, sU, mcs, per, batches, s);
+    case 2: return launch_gemm_tile<4, 2>(A, B, U, mp, np, kp, sA, sB, sU, mcs, per, batches, s);
+    case 3: return launch_gemm_tile<2, 4>(A, B, U, mp, np, kp, sA, sB, sU, mcs, per, batches, s);
+    default: return launch_gemm_tile<2, 2>(A, B, U, mp, np, kp, sA, sB, sU, mcs, per, batches, s);
     }
-    return hipGetLastError();
 }
 
 hipError_t fill_zero(unsigned long long *p, int64_t n, hipStream_t s)

@@ -194,24 +194,43 @@ DDI void oz_gemm_tile(const int8_t *__restrict__ Ab, const int8_t *__restrict__ 
     for (int64_t kt = 0; kt < nk; ++kt) {
         const int cur = (int)(kt & 1);
         if (kt + 1 < nk) OZ_GLOAD((kt + 1) * BK);
+        // Fragments are double-buffered: the LDS reads of step ks+1 are issued before the
+        // MFMAs of step ks (scheduling barriers keep that order), so LDS latency overlaps
+        // the matrix-core work instead of stalling it.
+        constexpr int KS = BK / 32;
+        v4i af[2][TM], bf[2][TN];
 #pragma unroll
-        for (int ks = 0; ks < BK / 32; ++ks) {
-            v4i af[TM], bf[TN];
+        for (int a = 0; a < TM; ++a) {
+            const int row = wm * 32 * TM + a * 32 + (lane & 31);
+            af[0][a] = *reinterpret_cast<const v4i *>(&sa[cur][row * PITCH + (lane >> 5) * 16]);
+        }
 #pragma unroll
-            for (int a = 0; a < TM; ++a) {
-                const int row = wm * 32 * TM + a * 32 + (lane & 31);
-                af[a] = *reinterpret_cast<const v4i *>(&sa[cur][row * PITCH + ks * 32 + (lane >> 5) * 16]);
+        for (int b = 0; b < TN; ++b) {
+            const int col = wn * 32 * TN + b * 32 + (lane & 31);
+            bf[0][b] = *reinterpret_cast<const v4i *>(&sb[cur][col * PITCH + (lane >> 5) * 16]);
+        }
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const int cb = ks & 1, nb = cb ^ 1;
+            if (ks + 1 < KS) {
+#pragma unroll
+                for (int a = 0; a < TM; ++a) {
+                    const int row = wm * 32 * TM + a * 32 + (lane & 31);
+                    af[nb][a] = *reinterpret_cast<const v4i *>(&sa[cur][row * PITCH + (ks + 1) * 32 + (lane >> 5) * 16]);
+                }
+#pragma unroll
+                for (int b = 0; b < TN; ++b) {
+                    const int col = wn * 32 * TN + b * 32 + (lane & 31);
+                    bf[nb][b] = *reinterpret_cast<const v4i *>(&sb[cur][col * PITCH + (ks + 1) * 32 + (lane >> 5) * 16]);
+                }
             }
-#pragma unroll
-            for (int b = 0; b < TN; ++b) {
-                const int col = wn * 32 * TN + b * 32 + (lane & 31);
-                bf[b] = *reinterpret_cast<const v4i *>(&sb[cur][col * PITCH + ks * 32 + (lane >> 5) * 16]);
-            }
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int a = 0; a < TM; ++a)
 #pragma unroll
                 for (int b = 0; b < TN; ++b)
-                    acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[a], bf[b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[cb][a], bf[cb][b], acc[a][b], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
         }
         if (kt + 1 < nk) OZ_LSTORE(cur ^ 1);
         __syncthreads();

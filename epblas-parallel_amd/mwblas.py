@@ -333,18 +333,22 @@ def mgemm_batched_device(transa, transb, ms, ns, ks, alphas, As, ldas, Bs, ldbs,
 TUNE_PLAIN_REAL, TUNE_PLAIN_CPLX, TUNE_OZ_GEMM, TUNE_BATCH = 0, 1, 2, 3
 
 
-def autotune(m, n, k, cplx=False, path=PATH_PLAIN):
+def autotune(m, n, k, cplx=False, path=PATH_PLAIN, all_times=False):
     """Time the tile variants of a path on synthetic m x n x k operands; returns
-    (winning variant, its milliseconds) and records the decision."""
+    (winning variant, its milliseconds[, every variant's milliseconds]) and records the
+    decision."""
     f = lib().mwgemm_autotune
     f.restype = ctypes.c_int
     f.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
-                  _vp, _vp]
+                  _vp, _vp, _vp, ctypes.c_int]
     v, ms = ctypes.c_int(), ctypes.c_double()
+    allv = np.full(8, -1.0)
     clear_error()
-    rc = f(m, n, k, 1 if cplx else 0, path, ctypes.byref(v), ctypes.byref(ms))
+    rc = f(m, n, k, 1 if cplx else 0, path, ctypes.byref(v), ctypes.byref(ms), _ptr(allv), 8)
     if rc != 0:
         _check(rc)
+    if all_times:
+        return v.value, ms.value, [float(x) for x in allv if x >= 0]
     return v.value, ms.value
 
 

@@ -148,12 +148,13 @@ MWGEMM_API int mwgemm_mgemm_batched_device(int64_t count, char transa, char tran
  * vs Ozaki with 128 or 64 residue tiles), and plans created with path AUTO then use the
  * recorded choice.  Decisions persist across processes through the file named by
  * MWGEMM_TUNE_FILE (appended on every record, read on first use).  *variant / *choice
- * receive the winner (batch: path * 16 + residue-tile variant), *best_ms its time.
+ * receive the winner (batch: path * 16 + residue-tile variant), *best_ms its time, and
+ * all_ms (optional, all_cap entries) every variant's time.
  * mwgemm_autotune_lookup returns the recorded variant for a kind (0 plain real, 1 plain
  * complex, 2 Ozaki residue GEMM, 3 batch) and shape (a, b, c), or -1; mwgemm_autotune_clear
  * forgets all decisions.  Return 0 or an error code. */
 MWGEMM_API int mwgemm_autotune(int64_t m, int64_t n, int64_t k, int cplx, int path, int *variant,
-                               double *best_ms);
+                               double *best_ms, double *all_ms, int all_cap);
 MWGEMM_API int mwgemm_batched_autotune(int64_t count, char transa, char transb, const int64_t *m,
                                        const int64_t *n, const int64_t *k, int *choice,
                                        double *best_ms);
