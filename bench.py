@@ -246,7 +246,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(achieved, 1) if achieved else None,
                      "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / INT8_PEAK_TOPS, 4) if achieved else None,
-                     "traffic": traffic, "kernel": "k_oz_gemm (INT8 residue GEMM, int8 ops)"},
+                     "traffic": traffic, "kernel": "k_oz_gemm_ring (INT8 residue GEMM, int8 ops)"},
     }
     if not args.no_batched:
         result["batched"] = batched_bench(dev)

@@ -186,7 +186,7 @@ int mwgemm_autotune(int64_t m, int64_t n, int64_t k, int cplx, int path, int *va
             tune_record(TK_PLAIN_REAL, m, n, k, best, best_t);
         } else {
             const int64_t mp = (m + 255) / 256 * 256, np = (n + 255) / 256 * 256;
-            const int64_t kp = cplx ? 2 * ((k + 63) / 64 * 64) : (k + 127) / 128 * 128;
+            const int64_t kp = cplx ? 2 * ((k + 63) / 64 * 64) : (k + 63) / 64 * 64;
             tune_record(TK_OZ_GEMM, mp, np, kp, best, best_t);
         }
         if (variant) *variant = best;

@@ -265,12 +265,12 @@ void k_oz_split(const T *__restrict__ X, int64_t ld, int64_t R, int64_t K, int64
 
 // Batched residue GEMM kernel: blockIdx.y = batch (modulus, or modulus x part for complex).
 template <int TM, int TN>
-__global__ __launch_bounds__(256, TM * TN >= 16 ? 1 : 2)
+__global__ __launch_bounds__(256, 2)
 void k_oz_gemm(const int8_t *__restrict__ A, const int8_t *__restrict__ B,
                uint8_t *__restrict__ U, int64_t kp, int64_t ldu, int64_t sA, int64_t sB,
                int64_t sU, const ModConst *__restrict__ mcs, int per, int tiles_m)
 {
-    constexpr int BM = 64 * TM, BN = 64 * TN, BK = (TM * TN >= 16) ? 128 : 64;
+    constexpr int BM = 64 * TM, BN = 64 * TN, BK = 64;
     __shared__ __attribute__((aligned(16))) int8_t sa[2][BM * (BK + 16)];
     __shared__ __attribute__((aligned(16))) int8_t sb[2][BN * (BK + 16)];
     const int batch = blockIdx.y;
@@ -280,6 +280,22 @@ void k_oz_gemm(const int8_t *__restrict__ A, const int8_t *__restrict__ B,
     const int64_t j0 = (int64_t)tn * BN;
     oz_gemm_tile<TM, TN, BK>(A + batch * sA + i0 * kp, B + batch * sB + j0 * kp, U + batch * sU,
                              kp, ldu, mcs[batch / per], i0, j0, sa, sb);
+}
+
+// 256 x 256 residue tiles through the DMA-fed LDS ring (eight waves, one workgroup per CU).
+__global__ __launch_bounds__(RING_THREADS, 1)
+void k_oz_gemm_ring(const int8_t *__restrict__ A, const int8_t *__restrict__ B,
+                    uint8_t *__restrict__ U, int64_t kp, int64_t ldu, int64_t sA, int64_t sB,
+                    int64_t sU, const ModConst *__restrict__ mcs, int per, int tiles_m)
+{
+    __shared__ __attribute__((aligned(1024))) int8_t smem[RING_NST * RING_SLOT];
+    const int batch = blockIdx.y;
+    int tm, tn;
+    tile_order(blockIdx.x, tiles_m, (int)gridDim.x / tiles_m, tm, tn);
+    const int64_t i0 = (int64_t)tm * 256;
+    const int64_t j0 = (int64_t)tn * 256;
+    oz_gemm_tile_ring(A + batch * sA + i0 * kp, B + batch * sB + j0 * kp, U + batch * sU, kp, ldu,
+                      mcs[batch / per], i0, j0, smem);
 }
 
 // Reconstruction kernel.  Each lane owns 4 consecutive rows of one column.
@@ -409,10 +425,22 @@ hipError_t launch_gemm_tile(const int8_t *A, const int8_t *B, uint8_t *U, int64_
     return hipGetLastError();
 }
 
-// Variants: 0 = 256 x 256 tiles with 128-deep slabs (one workgroup per CU), 1 = 128 x 128,
-// 2 = 256 x 128 and 3 = 128 x 256 (64-deep slabs, two workgroups per CU so one fills the
-// other's barrier bubbles).  kp must be a multiple of 128 for variant 0.  Without a tuned
-// choice, 256 x 256 when there are enough tiles to give every CU two, else 128 x 128.
+hipError_t launch_gemm_ring(const int8_t *A, const int8_t *B, uint8_t *U, int64_t mp, int64_t np,
+                            int64_t kp, int64_t sA, int64_t sB, int64_t sU, const ModConst *mcs,
+                            int per, int batches, hipStream_t s)
+{
+    const int tiles_m = (int)(mp / 256);
+    dim3 grid((unsigned)(tiles_m * (np / 256)), (unsigned)batches);
+    hipLaunchKernelGGL(k_oz_gemm_ring, grid, dim3(RING_THREADS), 0, s, A, B, U, kp, mp, sA, sB,
+                       sU, mcs, per, tiles_m);
+    return hipGetLastError();
+}
+
+// Variants: 0 = 256 x 256 tiles through the DMA-fed LDS ring (eight waves, one workgroup per
+// CU), 1 = 128 x 128, 2 = 256 x 128 and 3 = 128 x 256 (register-staged 64-deep slabs, two
+// workgroups per CU so one fills the other's barrier bubbles).  Every variant needs kp to
+// be a multiple of 64.  Without a tuned choice, 256 x 256 when there are enough tiles to
+// give every CU two, else 128 x 128.
 hipError_t launch_gemm(const int8_t *A, const int8_t *B, uint8_t *U, int64_t mp, int64_t np,
                        int64_t kp, int64_t sA, int64_t sB, int64_t sU, const ModConst *mcs,
                        int per, int batches, int variant, hipStream_t s)
@@ -420,9 +448,8 @@ hipError_t launch_gemm(const int8_t *A, const int8_t *B, uint8_t *U, int64_t mp,
     int v = variant;
     if (v < 0) v = tune_lookup(TK_OZ_GEMM, mp, np, kp);
     if (v < 0) v = (mp / 256) * (np / 256) * (int64_t)batches >= 512 ? 0 : 1;
-    if (v == 0 && kp % 128) v = 2;
     switch (v) {
-    case 0: return launch_gemm_tile<4, 4>(A, B, U, mp, np, kp, sA, sB, sU, mcs, per, batches, s);
+    case 0: return launch_gemm_ring(A, B, U, mp, np, kp, sA, sB, sU, mcs, per, batches, s);
     case 2: return launch_gemm_tile<4, 2>(A, B, U, mp, np, kp, sA, sB, sU, mcs, per, batches, s);
     case 3: return launch_gemm_tile<2, 4>(A, B, U, mp, np, kp, sA, sB, sU, mcs, per, batches, s);
     default: return launch_gemm_tile<2, 2>(A, B, U, mp, np, kp, sA, sB, sU, mcs, per, batches, s);
@@ -449,7 +476,7 @@ int ozaki_gemm_real(const GemmArgsR &g, hipStream_t s)
     if (int rc = oz_table(d, plan, &mcs)) return rc;
 
     // K padded to 128: the 256 x 256 residue tile consumes 128-deep slabs.
-    const int64_t Mp = round_up(g.m, 256), Np = round_up(g.n, 256), Kc = round_up(g.k, 128);
+    const int64_t Mp = round_up(g.m, 256), Np = round_up(g.n, 256), Kc = round_up(g.k, 64);
     const int nmod = plan.nmod;
     const size_t szA = (size_t)nmod * Mp * Kc, szB = (size_t)nmod * Np * Kc;
     const size_t szU = (size_t)nmod * Np * Mp, szR = (size_t)(Mp + Np) * 8;

@@ -256,6 +256,177 @@ DDI void oz_gemm_tile(const int8_t *__restrict__ Ab, const int8_t *__restrict__ 
     __syncthreads();   // the LDS slabs may be refilled by the caller's next tile
 }
 
+// ---------------------------------------------------------------------------------------
+// 256 x 256 residue tile fed by direct global->LDS loads (global_load_lds_dwordx4).
+//
+// A ring of RING_NST LDS stages, each one 64-deep K slab of the A panel and of the B panel
+// (2 x 256 rows x 64 bytes = 32 KB), is filled by DMA that never passes through VGPRs, so
+// three stages are in flight while the fourth is consumed.  Eight waves (two per SIMD, so
+// one wave's barrier or LDS wait is covered by its partner's MFMAs) own 64 x 128 sub-tiles
+// (2 x 4 MFMA tiles of 32 x 32).  Within a stage row r, 16-byte chunk c is stored in slot
+// c ^ ((r >> 2) & 3): the 16 rows read together by a fragment load then cover all 16
+// distinct 16-byte slots of a 256-byte bank line (conflict-free), while each DMA
+// instruction still writes 1 KB of contiguous LDS (16 rows x 4 chunks).
+//
+// Synchronisation: a wave's own DMA writes have landed when its vector-memory counter says
+// so; the s_barrier then publishes them to the workgroup.  The barrier is bracketed by
+// compiler-only memory clobbers instead of a fence, because a workgroup fence would also
+// wait for the stages still in flight.
+constexpr int RING_NST = 4, RING_BK = 64, RING_SLOT = 2 * 256 * RING_BK;
+constexpr int RING_THREADS = 512;
+
+// Wait until at most N of this wave's vector-memory operations are outstanding
+// (gfx9 s_waitcnt encoding: vmcnt[3:0], expcnt[6:4], lgkmcnt[11:8], vmcnt[5:4] at [15:14]).
+template <int N> DDI void wait_vmem()
+{
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+}
+// Wait until at most n stages of DMA (Q instructions each) issued after the awaited one
+// are outstanding.
+template <int Q> DDI void wait_stages(int n)
+{
+    switch (n) {
+    case 0: wait_vmem<0>(); break;
+    case 1: wait_vmem<Q>(); break;
+    case 2: wait_vmem<2 * Q>(); break;
+    default: wait_vmem<3 * Q>(); break;
+    }
+}
+DDI void wait_lds() { __builtin_amdgcn_s_waitcnt(15 | (7 << 4) | (0 << 8) | (3 << 14)); }
+
+DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restrict__ Bb,
+                           uint8_t *__restrict__ Ub, int64_t kp, int64_t ldu, const ModConst &mc,
+                           int64_t i0, int64_t j0, int8_t *smem)
+{
+    constexpr int NST = RING_NST, BK = RING_BK, SLOT = RING_SLOT;
+    constexpr int WM = 4, TM = 2, TN = 4;       // waves along M; MFMA tiles per wave
+    constexpr int DQ = 2;                       // DMA instructions per operand, wave and stage
+    constexpr int NR = TM + TN, NM = TM * TN;   // fragment reads and MFMAs per 32-deep step
+    static_assert(RING_THREADS == 64 * WM * (256 / (32 * TN)), "wave grid");
+    static_assert(DQ * (RING_THREADS / 64) * 16 == 256, "DMA rows");
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wm = w % WM, wn = w / WM;
+    const int64_t nk = kp / BK;
+    const int drow = lane >> 2, dslot = lane & 3;
+
+    // DMA of stage kt into its ring slot: instruction q of wave w covers rows
+    // (2w + q) * 16 + lane / 4 of both panels.
+    auto issue = [&](int64_t kt) {
+        int8_t *slot = smem + (int)(kt % NST) * SLOT;
+#pragma unroll
+        for (int q = 0; q < DQ; ++q) {
+            const int blk = DQ * w + q;
+            const int r = blk * 16 + drow;
+            const int c = dslot ^ ((r >> 2) & 3);
+            __builtin_amdgcn_global_load_lds(
+                (const __attribute__((address_space(1))) void *)(Ab + (int64_t)r * kp + kt * BK + c * 16),
+                (__attribute__((address_space(3))) void *)(slot + blk * 1024), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(
+                (const __attribute__((address_space(1))) void *)(Bb + (int64_t)r * kp + kt * BK + c * 16),
+                (__attribute__((address_space(3))) void *)(slot + 256 * BK + blk * 1024), 16, 0, 0);
+        }
+    };
+    v4i af[2][TM], bf[2][TN];
+    // Fragments of 32-deep step ks (0 or 1) of stage kt into fragment buffer d.
+    auto frag = [&](int64_t kt, int ks, int d) {
+        const int8_t *sl = smem + (int)(kt % NST) * SLOT;
+        const int c = 2 * ks + (lane >> 5);
+#pragma unroll
+        for (int a = 0; a < TM; ++a) {
+            const int r = wm * 32 * TM + a * 32 + (lane & 31);
+            af[d][a] = *reinterpret_cast<const v4i *>(sl + r * 64 + ((c ^ ((r >> 2) & 3)) << 4));
+        }
+#pragma unroll
+        for (int b = 0; b < TN; ++b) {
+            const int r = wn * 32 * TN + b * 32 + (lane & 31);
+            bf[d][b] = *reinterpret_cast<const v4i *>(sl + 256 * BK + r * 64 + ((c ^ ((r >> 2) & 3)) << 4));
+        }
+    };
+    v16i acc[TM][TN];
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b) acc[a][b] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    auto mma = [&](int d) {
+#pragma unroll
+        for (int a = 0; a < TM; ++a)
+#pragma unroll
+            for (int b = 0; b < TN; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[d][a], bf[d][b], acc[a][b], 0, 0, 0);
+    };
+    // Scheduling groups: each MFMA followed by one DMA (when refilling) and one fragment read.
+    auto interleave = [&](bool with_dma) {
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            if (with_dma && i < 2 * DQ) __builtin_amdgcn_sched_group_barrier(0x010, 1, 0);
+            if (i < NR) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        }
+    };
+    // Stage barrier: this wave's DMA of the awaited stage has landed (at most `later` newer
+    // stages outstanding) and its fragment reads are complete; then all waves meet.
+    auto stage_barrier = [&](int later) {
+        wait_stages<2 * DQ>(later);
+        wait_lds();
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+    };
+
+    const int64_t npro = nk < NST ? nk : NST;
+    for (int64_t s = 0; s < npro; ++s) issue(s);
+    stage_barrier((int)(npro - 1));
+    frag(0, 0, 0);
+    int64_t kt = 0;
+    // Steady state.  Step 0 of stage kt runs while step 1's fragments are read; the barrier
+    // before step 1 frees slot kt (its last fragments are in registers), which is refilled
+    // with stage kt + NST while step 1 runs and stage kt + 1's first fragments are read.
+    for (; kt + NST < nk; ++kt) {
+        frag(kt, 1, 1);
+        mma(0);
+        interleave(false);
+        stage_barrier(NST - 2);
+        issue(kt + NST);
+        frag(kt + 1, 0, 0);
+        mma(1);
+        interleave(true);
+    }
+    // Drain: the last NST - 1 stage transitions issue no refills.
+    for (; kt + 1 < nk; ++kt) {
+        frag(kt, 1, 1);
+        mma(0);
+        interleave(false);
+        const int64_t later = nk - kt - 2;
+        stage_barrier((int)(later < NST - 2 ? later : NST - 2));
+        frag(kt + 1, 0, 0);
+        mma(1);
+        interleave(false);
+    }
+    frag(kt, 1, 1);
+    mma(0);
+    interleave(false);
+    wait_lds();
+    mma(1);
+
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b) {
+            const int64_t col = j0 + wn * 32 * TN + b * 32 + (lane & 31);
+            const int64_t rowb = i0 + wm * 32 * TM + a * 32 + 4 * (lane >> 5);
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                uint32_t word = 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    word |= crt_weight_residue(acc[a][b][4 * g + q], mc) << (8 * q);
+                *reinterpret_cast<uint32_t *>(Ub + col * ldu + rowb + 8 * g) = word;
+            }
+        }
+    __syncthreads();   // the ring may be refilled by the caller's next tile
+}
+
 // 128-bit fixed-point fraction sum_t u_t * round(2^128/m_t) mod 2^128, from four exact FP64
 // limb accumulators (each < 2^45), as a signed double-double value F (|F| <= 2^127).
 DDI ddv fixed128_to_dd(double a0, double a1, double a2, double a3)
