@@ -146,7 +146,7 @@ void k_batch_split(const BatchProb *__restrict__ probs, UnitSpace us, int bits,
     if (KC) { kg = local % G; r = local / G; }
     else    { r = local % R; kg = local / R; }
     const int64_t rows = A_SIDE ? p.Mp : p.Np;
-    oz_split_unit<ddv, KC>(A_SIDE ? p.A : p.B, A_SIDE ? p.lda : p.ldb, p.k,
+    oz_split_unit<ddv, KC, false>(A_SIDE ? p.A : p.B, A_SIDE ? p.lda : p.ldb, p.k,
                            A_SIDE ? p.rA : p.rB, bits, 0, mcs, nmod,
                            A_SIDE ? p.resA : p.resB, 1, (int8_t *)nullptr, 0, p.Kc,
                            rows * p.Kc, r, kg);

@@ -163,7 +163,6 @@ bool ozaki_make_plan(int64_t k_eff, int target_bits, int force_nmod, OzPlan *p)
         c.inv_up = std::nextafter(c.inv_m, 1.0);
         if (c.inv_up * (double)m < 1.0) c.inv_up = std::nextafter(c.inv_up, 1.0);
         c.off_acc = std::ldexp((double)m, 24);
-        c.off_split = std::ldexp((double)m, 39);
         c.y = (double)mod_inverse(others, m);
         c.p1 = (double)mod_pow2(35, m);
         c.p2 = (double)mod_pow2(70, m);
@@ -242,8 +241,11 @@ void k_oz_rowmax_kc(const T *__restrict__ X, int64_t ld, int64_t R, int64_t K,
     if (lane == 0 && r < R && bits != 0ull) atomicMax(&rmax[r], bits);
 }
 
-// Residue split kernel: one unit per lane (see oz_split_unit).
-template <typename T, bool KC>
+// Residue split kernel: one unit (row r, 16-deep k group kg) per lane.  Row-contiguous
+// input (KC): a wave covers 64 k groups of one row.  Column-contiguous input: a wave covers
+// 8 rows x 8 k groups, so the 16-byte residue chunks of one row written by a wave form a
+// whole 128-byte line, and each input load reads eight 128-byte segments.
+template <typename T, bool KC, bool DUAL>
 __global__ __launch_bounds__(256)
 void k_oz_split(const T *__restrict__ X, int64_t ld, int64_t R, int64_t K, int64_t kgroups,
                 const unsigned long long *__restrict__ rmax, int bits, int part,
@@ -251,16 +253,17 @@ void k_oz_split(const T *__restrict__ X, int64_t ld, int64_t R, int64_t K, int64
                 int8_t *__restrict__ out2, int s2, int64_t kp, int64_t mod_stride)
 {
     int64_t r, kg;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (KC) {
-        kg = (int64_t)blockIdx.y * 64 + (threadIdx.x & 63);
-        r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+        kg = (int64_t)blockIdx.y * 64 + lane;
+        r = (int64_t)blockIdx.x * 4 + w;
     } else {
-        r = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
-        kg = (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
+        r = (int64_t)blockIdx.x * 32 + w * 8 + (lane >> 3);
+        kg = (int64_t)blockIdx.y * 8 + (lane & 7);
     }
     if (r >= R || kg >= kgroups) return;
-    oz_split_unit<T, KC>(X, ld, K, rmax, bits, part, mcs, nmod, out1, s1, out2, s2, kp,
-                         mod_stride, r, kg);
+    oz_split_unit<T, KC, DUAL>(X, ld, K, rmax, bits, part, mcs, nmod, out1, s1, out2, s2, kp,
+                               mod_stride, r, kg);
 }
 
 // Batched residue GEMM kernel: blockIdx.y = batch (modulus, or modulus x part for complex).
@@ -298,7 +301,10 @@ void k_oz_gemm_ring(const int8_t *__restrict__ A, const int8_t *__restrict__ B,
                       mcs[batch / per], i0, j0, smem);
 }
 
-// Reconstruction kernel.  Each lane owns 4 consecutive rows of one column.
+// Reconstruction kernel: a workgroup owns 1024 consecutive rows of one column.  Each lane
+// accumulates 4 consecutive rows (one 32-bit load of residues per modulus), the products
+// are staged in LDS, and the C entries are then read (beta != 0) and written with
+// consecutive lanes on consecutive rows.
 template <bool CPLX>
 __global__ __launch_bounds__(256)
 void k_oz_crt(const uint8_t *__restrict__ U, int64_t ldu, int64_t sU, int nmod,
@@ -309,11 +315,31 @@ void k_oz_crt(const uint8_t *__restrict__ U, int64_t ldu, int64_t sU, int nmod,
               typename std::conditional<CPLX, zddv, ddv>::type beta, int beta_mode,
               typename std::conditional<CPLX, zddv, ddv>::type *__restrict__ C, int64_t ldc)
 {
+    using V = typename std::conditional<CPLX, zddv, ddv>::type;
+    __shared__ V stage[1024];
     const int64_t j = (int64_t)blockIdx.x / blocks_per_col;
-    const int64_t i0 = (((int64_t)blockIdx.x % blocks_per_col) * 256 + threadIdx.x) * 4;
-    if (j >= n || i0 >= m) return;
-    oz_crt_unit<CPLX>(U, ldu, sU, nmod, mcs, Mhi, Mlo, rmaxA, rmaxB, bits, m, alpha, beta,
-                      beta_mode, C, ldc, j, i0);
+    const int64_t r0 = ((int64_t)blockIdx.x % blocks_per_col) * 1024;
+    const int64_t i0 = r0 + threadIdx.x * 4;
+    if (i0 < m) {
+        double acc[4][CPLX ? 2 : 1][4];
+        oz_crt_accumulate<CPLX, 4>(U, ldu, sU, nmod, mcs, j, i0, acc);
+        bool badj;
+        const int tau = row_sigma(rmaxB[j], bits, badj);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int64_t i = i0 + e < m ? i0 + e : m - 1;
+            bool badi;
+            const int sigma = row_sigma(rmaxA[i], bits, badi);
+            stage[threadIdx.x * 4 + e] =
+                oz_crt_value(acc[e], alpha, Mhi, Mlo, -128 - sigma - tau, badi || badj);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int64_t i = r0 + threadIdx.x + 256 * e;
+        if (i < m) oz_crt_store(&C[j * ldc + i], stage[threadIdx.x + 256 * e], beta, beta_mode);
+    }
 }
 
 // ==================================================================================
@@ -394,23 +420,38 @@ hipError_t launch_rowmax(const T *X, int64_t ld, int64_t R, int64_t K, bool kc,
     return hipGetLastError();
 }
 
+template <typename T, bool DUAL>
+hipError_t launch_split_t(const T *X, int64_t ld, int64_t R, int64_t K, int64_t kc_len, bool kc,
+                          const unsigned long long *rmax, int bits, int part, const ModConst *mcs,
+                          int nmod, int8_t *out1, int s1, int8_t *out2, int s2, int64_t kp,
+                          int64_t mod_stride, hipStream_t s)
+{
+    const int64_t kgroups = kc_len / 16;
+    if (kc) {
+        dim3 grid((unsigned)((R + 3) / 4), (unsigned)((kgroups + 63) / 64));
+        hipLaunchKernelGGL((k_oz_split<T, true, DUAL>), grid, dim3(256), 0, s, X, ld, R, K,
+                           kgroups, rmax, bits, part, mcs, nmod, out1, s1, out2, s2, kp, mod_stride);
+    } else {
+        dim3 grid((unsigned)((R + 31) / 32), (unsigned)((kgroups + 7) / 8));
+        hipLaunchKernelGGL((k_oz_split<T, false, DUAL>), grid, dim3(256), 0, s, X, ld, R, K,
+                           kgroups, rmax, bits, part, mcs, nmod, out1, s1, out2, s2, kp, mod_stride);
+    }
+    return hipGetLastError();
+}
+
+// Residues of one operand (or one part of a complex operand) into out1 (signed by s1) and,
+// when out2 is given, also into out2 (signed by s2).
 template <typename T>
 hipError_t launch_split(const T *X, int64_t ld, int64_t R, int64_t K, int64_t kc_len, bool kc,
                         const unsigned long long *rmax, int bits, int part, const ModConst *mcs,
                         int nmod, int8_t *out1, int s1, int8_t *out2, int s2, int64_t kp,
                         int64_t mod_stride, hipStream_t s)
 {
-    const int64_t kgroups = kc_len / 16;
-    if (kc) {
-        dim3 grid((unsigned)((R + 3) / 4), (unsigned)((kgroups + 63) / 64));
-        hipLaunchKernelGGL((k_oz_split<T, true>), grid, dim3(256), 0, s, X, ld, R, K, kgroups,
-                           rmax, bits, part, mcs, nmod, out1, s1, out2, s2, kp, mod_stride);
-    } else {
-        dim3 grid((unsigned)((R + 63) / 64), (unsigned)((kgroups + 3) / 4));
-        hipLaunchKernelGGL((k_oz_split<T, false>), grid, dim3(256), 0, s, X, ld, R, K, kgroups,
-                           rmax, bits, part, mcs, nmod, out1, s1, out2, s2, kp, mod_stride);
-    }
-    return hipGetLastError();
+    if (out2)
+        return launch_split_t<T, true>(X, ld, R, K, kc_len, kc, rmax, bits, part, mcs, nmod, out1,
+                                       s1, out2, s2, kp, mod_stride, s);
+    return launch_split_t<T, false>(X, ld, R, K, kc_len, kc, rmax, bits, part, mcs, nmod, out1, s1,
+                                    out2, s2, kp, mod_stride, s);
 }
 
 template <int TM, int TN>

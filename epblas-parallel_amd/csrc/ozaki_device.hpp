@@ -50,17 +50,17 @@ DDI void int_chunks(double hi, double lo, int sigma, double &c0, double &c1, dou
     c2 = sg * a2;
 }
 
-// Symmetric residue of c0 + c1*2^35 + c2*2^70 modulo m, in [-(m-1)/2, m/2] (int8 range).
-// The combination is shifted by m*2^39 (> its magnitude) so that it is a non-negative
-// integer below 2^48, for which floor(s * inv_up) is the exact quotient: no fix-ups.
-DDI int sym_residue(double c0, double c1, double c2, const ModConst &mc)
+// Symmetric residue of c0 + c1*2^35 + c2*2^70 modulo m, in [-(m-1)/2, (m-1)/2] for odd m
+// and [-128, 127] for m = 256 (half = (m-1) >> 1).  s = c2*p2 + c1*p1 + c0 is an exact
+// integer with |s| < 2^45, and q = floor(s * inv_up) is floor(s/m) for s >= 0.  For s < 0
+// it is too, except when m divides s, where q is one lower and r = s - q*m comes out as m
+// instead of 0; r lies in [0, m] either way and the symmetric step maps m to 0.
+DDI int sym_residue(double c0, double c1, double c2, const ModConst &mc, int mi, int half)
 {
-    const double m = mc.m;
-    const double s = __builtin_fma(c2, mc.p2, __builtin_fma(c1, mc.p1, c0 + mc.off_split));
+    const double s = __builtin_fma(c2, mc.p2, __builtin_fma(c1, mc.p1, c0));
     const double q = floor(s * mc.inv_up);
-    double r = __builtin_fma(-q, m, s);
-    r = (r > 0.5 * (m - 1.0)) ? r - m : r;
-    return (int)r;
+    const int r = (int)__builtin_fma(-q, mc.m, s);
+    return r > half ? r - mi : r;
 }
 
 DDI void comp(const ddv &x, int part, double &hi, double &lo)
@@ -79,7 +79,7 @@ DDI void comp(const zddv &x, int part, double &hi, double &lo)
 // Residue split of one lane's unit: row r, k in [16*kg, 16*kg+16), all moduli.
 // out1[t*mod_stride + r*kp + k] = s1 * (residue of entry (r,k)), likewise out2 (optional).
 // KC: entry (r,k) at X[r*ld + k]; otherwise at X[k*ld + r].
-template <typename T, bool KC>
+template <typename T, bool KC, bool DUAL>
 DDI void oz_split_unit(const T *__restrict__ X, int64_t ld, int64_t K,
                        const unsigned long long *__restrict__ rmax, int bits, int part,
                        const ModConst *__restrict__ mcs, int nmod, int8_t *__restrict__ out1,
@@ -96,18 +96,31 @@ DDI void oz_split_unit(const T *__restrict__ X, int64_t ld, int64_t K,
         if (k < K && !bad) comp(KC ? X[r * ld + k] : X[k * ld + r], part, hi, lo);
         int_chunks(hi, lo, sigma, c0[s], c1[s], c2[s]);
     }
+    // The sign s1 is applied to the chunks once; out2 differs from out1 by the sign mask g
+    // ((v ^ g) - g negates when g is -1).  Every modulus but 256 is odd, so the symmetric
+    // residue of -x is minus that of x; for 256 both are -x mod 256 as a byte.
+    if (s1 < 0) {
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            c0[s] = -c0[s];
+            c1[s] = -c1[s];
+            c2[s] = -c2[s];
+        }
+    }
+    const int g = (s1 < 0) != (s2 < 0) ? -1 : 0;
     for (int t = 0; t < nmod; ++t) {
         const ModConst mc = mcs[t];
+        const int mi = (int)mc.m, half = (mi - 1) >> 1;
         uint32_t w1[4] = {0, 0, 0, 0}, w2[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int s = 0; s < 16; ++s) {
-            const int res = sym_residue(c0[s], c1[s], c2[s], mc);
-            w1[s >> 2] |= ((uint32_t)(res * s1) & 0xffu) << (8 * (s & 3));
-            w2[s >> 2] |= ((uint32_t)(res * s2) & 0xffu) << (8 * (s & 3));
+            const int res = sym_residue(c0[s], c1[s], c2[s], mc, mi, half);
+            w1[s >> 2] |= ((uint32_t)res & 0xffu) << (8 * (s & 3));
+            if constexpr (DUAL) w2[s >> 2] |= ((uint32_t)((res ^ g) - g) & 0xffu) << (8 * (s & 3));
         }
         const int64_t off = t * mod_stride + r * kp + kg * 16;
         *reinterpret_cast<uint4 *>(out1 + off) = make_uint4(w1[0], w1[1], w1[2], w1[3]);
-        if (out2) *reinterpret_cast<uint4 *>(out2 + off) = make_uint4(w2[0], w2[1], w2[2], w2[3]);
+        if constexpr (DUAL) *reinterpret_cast<uint4 *>(out2 + off) = make_uint4(w2[0], w2[1], w2[2], w2[3]);
     }
 }
 
@@ -459,6 +472,68 @@ DDI ddv reconstruct(const double acc[4], double Mhi, double Mlo, int e, bool bad
 }
 
 // Reconstruction + alpha/beta update of rows i0..i0+3 (those < m) of column j.
+// Fixed-point CRT sums of RPL consecutive rows i0.. of column j: acc[e][p][l] accumulates
+// u_t * limb l of round(2^128/m_t) over the moduli (part p: real / imaginary).
+template <bool CPLX, int RPL>
+DDI void oz_crt_accumulate(const uint8_t *__restrict__ U, int64_t ldu, int64_t sU, int nmod,
+                           const ModConst *__restrict__ mcs, int64_t j, int64_t i0,
+                           double (&acc)[RPL][CPLX ? 2 : 1][4])
+{
+    static_assert(RPL == 4, "rows per lane");
+    constexpr int P = CPLX ? 2 : 1;
+#pragma unroll
+    for (int e = 0; e < RPL; ++e)
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+#pragma unroll
+            for (int l = 0; l < 4; ++l) acc[e][p][l] = 0.0;
+#pragma unroll 2
+    for (int t = 0; t < nmod; ++t) {
+        const double w0 = mcs[t].w[0], w1 = mcs[t].w[1], w2 = mcs[t].w[2], w3 = mcs[t].w[3];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const uint32_t word =
+                *reinterpret_cast<const uint32_t *>(U + (int64_t)(t * P + p) * sU + j * ldu + i0);
+#pragma unroll
+            for (int e = 0; e < RPL; ++e) {
+                const double u = (double)((word >> (8 * e)) & 0xffu);
+                acc[e][p][0] = __builtin_fma(u, w0, acc[e][p][0]);
+                acc[e][p][1] = __builtin_fma(u, w1, acc[e][p][1]);
+                acc[e][p][2] = __builtin_fma(u, w2, acc[e][p][2]);
+                acc[e][p][3] = __builtin_fma(u, w3, acc[e][p][3]);
+            }
+        }
+    }
+}
+
+// alpha * (A B^T)_ij from the CRT sums of entry (i, j), with the row/column scalings undone;
+// NaN when row i of A or column j of B holds a NaN or an infinity.
+DDI ddv oz_crt_value(const double (&acc)[1][4], ddv alpha, double Mhi, double Mlo, int ex, bool bad)
+{
+    return dd_mul_exact(alpha, reconstruct(acc[0], Mhi, Mlo, ex, bad));
+}
+DDI zddv oz_crt_value(const double (&acc)[2][4], zddv alpha, double Mhi, double Mlo, int ex, bool bad)
+{
+    const zddv cab{reconstruct(acc[0], Mhi, Mlo, ex, bad), reconstruct(acc[1], Mhi, Mlo, ex, bad)};
+    return zdd_mul_exact(alpha, cab);
+}
+
+// C := beta*C + v for one entry; with beta == 0, C is not read (a NaN already in C must
+// not propagate).
+DDI void oz_crt_store(ddv *cp, ddv v, ddv beta, int beta_mode)
+{
+    ddv c0{0.0, 0.0};
+    if (beta_mode != BETA_ZERO) c0 = dd_beta_prepass(*cp, beta, beta_mode);
+    *cp = dd_add_exact(c0, v);
+}
+DDI void oz_crt_store(zddv *cp, zddv v, zddv beta, int beta_mode)
+{
+    zddv c0{{0.0, 0.0}, {0.0, 0.0}};
+    if (beta_mode != BETA_ZERO) c0 = zdd_beta_prepass(*cp, beta, beta_mode);
+    *cp = zdd_add_exact(c0, v);
+}
+
+// Reconstruction of 4 consecutive rows i0.. of column j, stored directly.
 template <bool CPLX>
 DDI void oz_crt_unit(const uint8_t *__restrict__ U, int64_t ldu, int64_t sU, int nmod,
                      const ModConst *__restrict__ mcs, double Mhi, double Mlo,
@@ -469,30 +544,8 @@ DDI void oz_crt_unit(const uint8_t *__restrict__ U, int64_t ldu, int64_t sU, int
                      typename std::conditional<CPLX, zddv, ddv>::type *__restrict__ C,
                      int64_t ldc, int64_t j, int64_t i0)
 {
-    constexpr int P = CPLX ? 2 : 1;
-    double acc[4][P][4];
-#pragma unroll
-    for (int e = 0; e < 4; ++e)
-#pragma unroll
-        for (int p = 0; p < P; ++p)
-#pragma unroll
-            for (int l = 0; l < 4; ++l) acc[e][p][l] = 0.0;
-    for (int t = 0; t < nmod; ++t) {
-        const double w0 = mcs[t].w[0], w1 = mcs[t].w[1], w2 = mcs[t].w[2], w3 = mcs[t].w[3];
-#pragma unroll
-        for (int p = 0; p < P; ++p) {
-            const uint32_t word =
-                *reinterpret_cast<const uint32_t *>(U + (int64_t)(t * P + p) * sU + j * ldu + i0);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const double u = (double)((word >> (8 * e)) & 0xffu);
-                acc[e][p][0] = __builtin_fma(u, w0, acc[e][p][0]);
-                acc[e][p][1] = __builtin_fma(u, w1, acc[e][p][1]);
-                acc[e][p][2] = __builtin_fma(u, w2, acc[e][p][2]);
-                acc[e][p][3] = __builtin_fma(u, w3, acc[e][p][3]);
-            }
-        }
-    }
+    double acc[4][CPLX ? 2 : 1][4];
+    oz_crt_accumulate<CPLX, 4>(U, ldu, sU, nmod, mcs, j, i0, acc);
     bool badj;
     const int tau = row_sigma(rmaxB[j], bits, badj);
 #pragma unroll
@@ -501,20 +554,9 @@ DDI void oz_crt_unit(const uint8_t *__restrict__ U, int64_t ldu, int64_t sU, int
         if (i >= m) break;
         bool badi;
         const int sigma = row_sigma(rmaxA[i], bits, badi);
-        const int ex = -128 - sigma - tau;
-        const bool bad = badi || badj;
-        if constexpr (!CPLX) {
-            const ddv cab = reconstruct(acc[e][0], Mhi, Mlo, ex, bad);
-            ddv *cp = &C[j * ldc + i];
-            const ddv c0 = dd_beta_prepass(*cp, beta, beta_mode);
-            *cp = dd_add_exact(c0, dd_mul_exact(alpha, cab));
-        } else {
-            const zddv cab{reconstruct(acc[e][0], Mhi, Mlo, ex, bad),
-                           reconstruct(acc[e][1], Mhi, Mlo, ex, bad)};
-            zddv *cp = &C[j * ldc + i];
-            const zddv c0 = zdd_beta_prepass(*cp, beta, beta_mode);
-            *cp = zdd_add_exact(c0, zdd_mul_exact(alpha, cab));
-        }
+        oz_crt_store(&C[j * ldc + i],
+                     oz_crt_value(acc[e], alpha, Mhi, Mlo, -128 - sigma - tau, badi || badj),
+                     beta, beta_mode);
     }
 }
 

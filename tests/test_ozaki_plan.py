@@ -94,6 +94,39 @@ def test_rounded_up_reciprocal_gives_exact_quotients():
                 assert math.floor(float(d) * inv) == d // m, (m, d)
 
 
+def _sym_residue(s, m, inv):
+    """The split kernel's residue step in binary64: s is an exact integer (|s| < 2^45), the
+    FMAs around the quotient are exact integer operations, only s * inv rounds."""
+    q = math.floor(float(s) * inv)
+    r = s - q * m
+    half = (m - 1) >> 1
+    return r - m if r > half else r
+
+
+def test_symmetric_residue_is_exact_for_both_signs():
+    """Split residues need no non-negative offset: floor(s * RU(1/m)) is floor(s/m) for
+    s >= 0, and for s < 0 it is one lower exactly when m divides s, where the remainder m is
+    mapped to 0 by the symmetric step.  Checked on random and worst-case integers of either
+    sign over the whole chunk range |s| < 2^45 (c0 < 2^35, |c1| < 2^36, |c2| <= 2^34, chunk
+    weights below 256), for every modulus."""
+    m_all, _ = mwblas.ozaki_moduli(40)
+    rng = random.Random(11)
+    lim = 2 ** 45
+    for m in (int(v) for v in m_all):
+        inv = _inv_up(m)
+        top = (lim - 1) // m
+        cases = [rng.randrange(-lim + 1, lim) for _ in range(2000)]
+        cases += [q * m + d for q in (0, 1, 2, top - 1, top) for d in (-1, 0, 1)]
+        cases += [rng.randrange(0, top) * m + d for d in (-1, 0, 1, m - 1) for _ in range(100)]
+        cases += [-c for c in cases]
+        lo_r, hi_r = (-128, 127) if m == 256 else (-((m - 1) // 2), (m - 1) // 2)
+        for s in cases:
+            if abs(s) >= lim:
+                continue
+            r = _sym_residue(s, m, inv)
+            assert (r - s) % m == 0 and lo_r <= r <= hi_r, (m, s, r)
+
+
 def _chunks(hi, lo, sigma):
     """The split's integer chunking, step by step in binary64 (every step is exact)."""
     import numpy as np
@@ -127,5 +160,5 @@ def test_split_chunks_are_exact_and_bounded():
             X = int(c0) + int(c1) * 2 ** 35 + int(c2) * 2 ** 70
             assert abs(X) <= 2 ** b
             assert abs(Fraction(X) - Fraction(xh_s) - Fraction(xl_s)) <= Fraction(5, 8)
-            # the shifted combination stays below 2^48 for every modulus weight < 256
+            # the chunk combination stays below 2^45 for every modulus weight < 256
             assert abs(c0) + abs(c1) * 255 + abs(c2) * 255 < 2 ** 45
