@@ -10,7 +10,12 @@ residue GEMM) timed with HIP events on the library's stream; the CPU baseline is
 oracle's restatement of the reference OpenMP mgemm_ timed on this host on a bounded row
 sample of the same problem.
 
+Further objects on the same line (each with its own workload): "batched" (2000 small GEMMs
+in one captured launch per path), "complex" (4096^3 with op(A)='C', both paths) and
+"host_boundary" (the drop-in mgemm_ with host operands, PCIe transfers inside the timing).
+
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--size 8192]
+                    [--no-batched] [--no-complex] [--no-host]
 """
 import argparse
 import json
@@ -116,6 +121,128 @@ def batched_bench(dev, reps=5, count=2000, seed=2000):
     return out
 
 
+def gen_uniform(n_rows, n_cols, gen, device, parts=1):
+    """Column-major DD (parts=1) or complex DD (parts=2) entries: each hi = 2u-1, lo =
+    hi * 2^-53 * (u'-1/2) (non-overlapping); tensor shape (cols, rows, 2*parts)."""
+    import torch
+    limbs = []
+    for _ in range(parts):
+        h = 2 * torch.rand((n_cols, n_rows), generator=gen, device=device, dtype=torch.float64) - 1
+        l = h * 2.0 ** -53 * (torch.rand((n_cols, n_rows), generator=gen, device=device,
+                                         dtype=torch.float64) - 0.5)
+        limbs += [h, l]
+    return torch.stack(limbs, dim=-1).contiguous()
+
+
+def random_scalar(rng, parts):
+    """A random DD (parts=1) or complex DD (parts=2) scalar, hi ~ U[-2,2]."""
+    out = []
+    for _ in range(parts):
+        h = float(rng.uniform(-2.0, 2.0))
+        out += [h, h * 2.0 ** -53 * float(rng.uniform(-0.5, 0.5))]
+    return out
+
+
+def complex_bench(dev, size=4096, oz_reps=5, plain_reps=1, seed=4096):
+    """Complex DD GEMM C := alpha*A^H*B + beta*C at size^3 (op(A)='C', op(B)='N'), random
+    complex alpha and beta, inputs in HBM, on both paths.  Effective rate = 8MNK / t."""
+    import numpy as np
+    import torch
+    import mwblas
+    m = n = k = size
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    A = gen_uniform(k, m, gen, dev, parts=2)     # op(A) = A^H: A is k x m
+    B = gen_uniform(k, n, gen, dev, parts=2)
+    C0 = gen_uniform(m, n, gen, dev, parts=2)
+    rng = np.random.default_rng(seed)
+    alpha, beta = random_scalar(rng, 2), random_scalar(rng, 2)
+    stream = torch.cuda.current_stream().cuda_stream
+    flops = 8.0 * m * n * k
+    out = {"workload": f"complex DD GEMM {m}x{n}x{k}, op(A)='C', op(B)='N', random complex "
+                       f"alpha/beta, uniform entries", "dd_flops": flops,
+           "moduli": mwblas.ozaki_config(k, cplx=True)[0]}
+    res = {}
+    for name, path, reps in (("ozaki", mwblas.PATH_OZAKI, oz_reps),
+                             ("plain", mwblas.PATH_PLAIN, plain_reps)):
+        C = C0.clone()
+        mwblas.gemm_device(True, "C", "N", m, n, k, alpha, A, k, B, k, beta, C, m,
+                           path=path, stream=stream)
+        torch.cuda.synchronize()
+        mwblas.timer_reset()
+        mwblas.timer_enable(True)
+        t = 0.0
+        for _ in range(reps):
+            C.copy_(C0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            mwblas.gemm_device(True, "C", "N", m, n, k, alpha, A, k, B, k, beta, C, m,
+                               path=path, stream=stream)
+            torch.cuda.synchronize()
+            t += time.perf_counter() - t0
+        mwblas.timer_enable(False)
+        kt = mwblas.timer_read()
+        t /= reps
+        entry = {"value": round(flops / t / 1e9, 2), "unit": "DD-GFLOP/s",
+                 "ms_per_call": round(t * 1e3, 3),
+                 "kernels_ms_per_call": {kn: round(v[0] / reps, 3) for kn, v in kt.items()}}
+        if name == "plain":
+            p_ms, p_cnt = kt.get("plain_cplx", (0.0, 0))
+            if p_cnt:
+                # 72 FP64 lane-ops per complex DD multiply-add against 39.3e12 lane-ops/s
+                entry["fp64_frac"] = round(72 * m * n * k / (p_ms / p_cnt * 1e-3) / 1e12
+                                           / (FP64_PEAK_TFLOPS / 2), 4)
+        out[name] = entry
+        res[name] = C
+    if "ozaki" in res and "plain" in res:
+        amax = A[..., 0::2].abs().max().item()
+        bmax = B[..., 0::2].abs().max().item()
+        cmax = C0[..., 0::2].abs().max().item()
+        anorm = float(np.hypot(alpha[0], alpha[2]))
+        bnorm = float(np.hypot(beta[0], beta[2]))
+        d = res["ozaki"] - res["plain"]
+        diff = (d[..., 0::2] + d[..., 1::2]).abs().max().item()
+        out["ozaki_vs_plain_normwise"] = diff / (anorm * 2 * k * amax * bmax + bnorm * cmax)
+        out["speedup_vs_plain"] = round(out["ozaki"]["value"] / out["plain"]["value"], 2)
+    return out
+
+
+def host_boundary_bench(size=4096, reps=3, seed=77):
+    """The drop-in mgemm_ with host (pageable) operands: H2D of A and B, the GEMM on the
+    exact-product path, D2H of C, all inside the timed call (PCIe-inclusive).  Real NN at
+    size^3, alpha=1, beta=0.  Reported beside, never instead of, the HBM-resident value."""
+    import numpy as np
+    import mwblas
+    rng = np.random.default_rng(seed)
+
+    def dd(rows, cols):
+        x = np.empty((cols, rows, 2))
+        x[..., 0] = rng.uniform(-1.0, 1.0, (cols, rows))
+        x[..., 1] = x[..., 0] * 2.0 ** -53 * (rng.random((cols, rows)) - 0.5)
+        return x
+
+    m = n = k = size
+    A, B = dd(m, k), dd(k, n)
+    C = np.zeros((n, m, 2))
+    prev = mwblas.set_path(mwblas.PATH_OZAKI)
+    try:
+        mwblas.mgemm("N", "N", m, n, k, [1.0, 0.0], A, m, B, k, [0.0, 0.0], C, m)
+        best = None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            mwblas.mgemm("N", "N", m, n, k, [1.0, 0.0], A, m, B, k, [0.0, 0.0], C, m)
+            t = time.perf_counter() - t0
+            best = t if best is None else min(best, t)
+    finally:
+        mwblas.set_path(prev)
+    moved = (A.nbytes + B.nbytes + C.nbytes)
+    return {"workload": f"mgemm_ NN {m}x{n}x{k} with host operands (pageable), Ozaki path, "
+                        f"alpha=1, beta=0", "value": round(2.0 * m * n * k / best / 1e9, 2),
+            "unit": "DD-GFLOP/s", "ms_per_call": round(best * 1e3, 2),
+            "host_bytes_moved": moved,
+            "effective_GBps": round(moved / best / 1e9, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +253,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-complex", action="store_true")
+    ap.add_argument("--no-host", action="store_true")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="first run one device copy of this many MiB (the known-size kernel "
                          "that scales hardware byte counters)")
@@ -250,6 +379,10 @@ def main():
     }
     if not args.no_batched:
         result["batched"] = batched_bench(dev)
+    if not args.no_complex:
+        result["complex"] = complex_bench(dev)
+    if not args.no_host:
+        result["host_boundary"] = host_boundary_bench()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(A, B, n, k)
     print(json.dumps(result))

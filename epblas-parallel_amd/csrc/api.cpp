@@ -16,6 +16,7 @@
 // leave C = beta*C (mgemm_serial.cpp:353-365).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cctype>
 #include <cstdarg>
@@ -28,6 +29,7 @@
 
 #include "../../include/mwgemm_hip.h"
 #include "mw_internal.hpp"
+#include "host_copy.hpp"
 #include "ozaki.hpp"
 #include "timer.hpp"
 
@@ -86,6 +88,10 @@ struct DevState {
     hipStream_t stream = nullptr;
     void *stage = nullptr;   // host-pointer staging (A | B | C)
     size_t stage_bytes = 0;
+    void *pin[2] = {nullptr, nullptr};   // pinned host chunks for host <-> device transfers
+    size_t pin_bytes = 0;
+    hipEvent_t pin_ev[2] = {nullptr, nullptr};   // last transfer that used each chunk
+    bool pin_used[2] = {false, false};
 };
 std::mutex g_state_mu;
 std::vector<DevState> g_dev;
@@ -122,6 +128,85 @@ int ensure_stage(DevState *st, size_t bytes)
     }
     MW_TRY(hipMalloc(&st->stage, bytes));
     st->stage_bytes = bytes;
+    return 0;
+}
+
+// Pinned staging chunks: pageable operands are copied (in parallel, on the host) into one
+// chunk while the DMA of the other is in flight.
+constexpr size_t PIN_CHUNK = size_t(32) << 20;
+
+int ensure_pinned(DevState *st, size_t bytes)
+{
+    if (st->pin_bytes >= bytes) return 0;
+    for (int i = 0; i < 2; ++i) {
+        if (!st->pin_ev[i]) MW_TRY(hipEventCreateWithFlags(&st->pin_ev[i], hipEventDisableTiming));
+        if (st->pin_used[i]) MW_TRY(hipEventSynchronize(st->pin_ev[i]));
+        st->pin_used[i] = false;
+        if (st->pin[i]) {
+            MW_TRY(hipHostFree(st->pin[i]));
+            st->pin[i] = nullptr;
+        }
+    }
+    st->pin_bytes = 0;
+    for (int i = 0; i < 2; ++i) MW_TRY(hipHostMalloc(&st->pin[i], bytes, hipHostMallocDefault));
+    st->pin_bytes = bytes;
+    return 0;
+}
+
+int wait_slot(DevState *st, int slot)
+{
+    if (st->pin_used[slot]) MW_TRY(hipEventSynchronize(st->pin_ev[slot]));
+    st->pin_used[slot] = false;
+    return 0;
+}
+
+// Host (pageable) -> device: cols columns of seg bytes, host pitch hp, device pitch dp.
+int upload_pinned(DevState *st, hipStream_t s, char *dst, size_t dp, const char *src,
+                  size_t hp, size_t seg, int64_t cols)
+{
+    if (cols <= 0 || seg == 0) return 0;
+    const int64_t per = (int64_t)std::max<size_t>(1, PIN_CHUNK / seg);
+    if (int rc = ensure_pinned(st, (size_t)std::min<int64_t>(per, cols) * seg)) return rc;
+    for (int64_t c0 = 0, i = 0; c0 < cols; c0 += per, ++i) {
+        const int slot = (int)(i & 1);
+        const int64_t nc = std::min(per, cols - c0);
+        if (int rc = wait_slot(st, slot)) return rc;
+        host_copy_columns(st->pin[slot], seg, src + c0 * hp, hp, seg, nc);
+        MW_TRY(hipMemcpy2DAsync(dst + c0 * dp, dp, st->pin[slot], seg, seg, nc,
+                                hipMemcpyHostToDevice, s));
+        MW_TRY(hipEventRecord(st->pin_ev[slot], s));
+        st->pin_used[slot] = true;
+    }
+    return 0;
+}
+
+// Device -> host (pageable): the DMA of chunk i+1 runs while chunk i is copied out.
+int download_pinned(DevState *st, hipStream_t s, char *dst, size_t hp, const char *src,
+                    size_t dp, size_t seg, int64_t cols)
+{
+    if (cols <= 0 || seg == 0) return 0;
+    const int64_t per = (int64_t)std::max<size_t>(1, PIN_CHUNK / seg);
+    if (int rc = ensure_pinned(st, (size_t)std::min<int64_t>(per, cols) * seg)) return rc;
+    const int64_t nchunk = (cols + per - 1) / per;
+    auto issue = [&](int64_t i) -> int {
+        const int slot = (int)(i & 1);
+        const int64_t c0 = i * per, nc = std::min(per, cols - c0);
+        if (int rc = wait_slot(st, slot)) return rc;
+        MW_TRY(hipMemcpy2DAsync(st->pin[slot], seg, src + c0 * dp, dp, seg, nc,
+                                hipMemcpyDeviceToHost, s));
+        MW_TRY(hipEventRecord(st->pin_ev[slot], s));
+        st->pin_used[slot] = true;
+        return 0;
+    };
+    if (int rc = issue(0)) return rc;
+    for (int64_t i = 0; i < nchunk; ++i) {
+        if (i + 1 < nchunk)
+            if (int rc = issue(i + 1)) return rc;
+        const int slot = (int)(i & 1);
+        if (int rc = wait_slot(st, slot)) return rc;
+        const int64_t c0 = i * per, nc = std::min(per, cols - c0);
+        host_copy_columns(dst + c0 * hp, hp, st->pin[slot], seg, seg, nc);
+    }
     return 0;
 }
 
@@ -271,31 +356,35 @@ int staged_call(Args g, int64_t a_rows, int64_t a_cols, int64_t b_rows, int64_t 
         if (int rc = ensure_stage(st, need)) return rc;
     }
     char *base = static_cast<char *>(st->stage);
-    // Only the logical rows x cols of each operand travel; the device copies keep the
-    // caller's leading dimensions, and C's padding rows are never written back.
+    // Only the logical rows x cols of each operand travel, through pinned chunks; the device
+    // copies keep the caller's leading dimensions, and C's padding rows are never written.
     if (!dA) {
         g.A = reinterpret_cast<decltype(g.A)>(base + offA);
-        MW_TRY(hipMemcpy2DAsync(base + offA, g.lda * sizeof(T), hA, g.lda * sizeof(T),
-                                a_rows * sizeof(T), a_cols, hipMemcpyHostToDevice, s));
+        if (int rc = upload_pinned(st, s, base + offA, g.lda * sizeof(T),
+                                   reinterpret_cast<const char *>(hA), g.lda * sizeof(T),
+                                   a_rows * sizeof(T), a_cols)) return rc;
     }
     if (!dB) {
         g.B = reinterpret_cast<decltype(g.B)>(base + offB);
-        MW_TRY(hipMemcpy2DAsync(base + offB, g.ldb * sizeof(T), hB, g.ldb * sizeof(T),
-                                b_rows * sizeof(T), b_cols, hipMemcpyHostToDevice, s));
+        if (int rc = upload_pinned(st, s, base + offB, g.ldb * sizeof(T),
+                                   reinterpret_cast<const char *>(hB), g.ldb * sizeof(T),
+                                   b_rows * sizeof(T), b_cols)) return rc;
     }
     if (!dC) {
         g.C = reinterpret_cast<decltype(g.C)>(base + offC);
         if (!beta_zero)
-            MW_TRY(hipMemcpy2DAsync(base + offC, g.ldc * sizeof(T), hC, g.ldc * sizeof(T),
-                                    g.m * sizeof(T), g.n, hipMemcpyHostToDevice, s));
+            if (int rc = upload_pinned(st, s, base + offC, g.ldc * sizeof(T),
+                                       reinterpret_cast<const char *>(hC), g.ldc * sizeof(T),
+                                       g.m * sizeof(T), g.n)) return rc;
     }
     if (int rc = run(g, s)) {
         (void)hipStreamSynchronize(s);
         return rc;
     }
     if (!dC)
-        MW_TRY(hipMemcpy2DAsync(hC, g.ldc * sizeof(T), base + offC, g.ldc * sizeof(T),
-                                g.m * sizeof(T), g.n, hipMemcpyDeviceToHost, s));
+        if (int rc = download_pinned(st, s, reinterpret_cast<char *>(hC), g.ldc * sizeof(T),
+                                     base + offC, g.ldc * sizeof(T), g.m * sizeof(T), g.n))
+            return rc;
     MW_TRY(hipStreamSynchronize(s));
     return 0;
 }

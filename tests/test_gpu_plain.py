@@ -146,3 +146,37 @@ def test_4096_all_trans_vs_port_and_binary128(gpu):
                                  entries=ent)
         print(f"  err vs binary128 = {r['err']:.3e}", flush=True)
         assert r["err"] <= 1e-30
+
+
+@pytest.mark.parametrize("path", ["plain", "ozaki"])
+def test_host_pipeline_chunks_match_device_path(gpu, path):
+    """Host operands travel through double-buffered pinned chunks (32 MiB) copied by the
+    host worker pool: operands of several chunks, padded leading dimensions, beta != 0 (C
+    uploaded too), and columns larger than a chunk.  The result must equal the
+    device-pointer call bit for bit, with C's padding rows untouched."""
+    import torch
+    p = mwblas.PATH_PLAIN if path == "plain" else mwblas.PATH_OZAKI
+    rng = np.random.default_rng(31)
+    shapes = [(2100, 1100, 1100)]
+    if path == "plain":   # (the exact-product workspace would pad n=2 to 256 columns)
+        shapes.append((2_200_000, 2, 3))
+    for (m, n, k) in shapes:
+        print(f"host pipeline {path} m={m} n={n} k={k}", flush=True)
+        lda, ldc = m + 5, m + 3
+        A = ddgen.real_matrix(rng, m, k, lda)
+        B = ddgen.real_matrix(rng, k, n)
+        C0 = ddgen.real_matrix(rng, m, n, ldc)
+        al, be = ddgen.real_scalar(rng, scale=2), ddgen.real_scalar(rng, scale=2)
+        prev = mwblas.set_path(p)
+        try:
+            Ch = C0.copy()
+            mwblas.mgemm("N", "N", m, n, k, al, A, lda, B, k, be, Ch, ldc)
+        finally:
+            mwblas.set_path(prev)
+        dA, dB, dC = (torch.from_numpy(x).to(gpu) for x in (A, B, C0))
+        mwblas.gemm_device(False, "N", "N", m, n, k, al, dA, lda, dB, k, be, dC, ldc, path=p,
+                           stream=torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        Cd = dC.cpu().numpy()
+        assert np.array_equal(Ch, Cd, equal_nan=True)
+        assert np.array_equal(Ch[:, m:, :], C0[:, m:, :], equal_nan=True)
