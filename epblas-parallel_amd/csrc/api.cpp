@@ -669,10 +669,10 @@ int mwgemm_ozaki_moduli(int nmod, int *m, int *y)
 }
 
 int mwgemm_testing_residue_gemm(const void *dA, const void *dB, void *dU, int64_t mp,
-                                int64_t np, int64_t kp, int mod_index, void *stream)
+                                int64_t np, int64_t kp, int mod_index, int variant, void *stream)
 {
     return ozaki_residue_gemm_test(static_cast<const int8_t *>(dA), static_cast<const int8_t *>(dB),
-                                   static_cast<uint8_t *>(dU), mp, np, kp, mod_index,
+                                   static_cast<uint8_t *>(dU), mp, np, kp, mod_index, variant,
                                    static_cast<hipStream_t>(stream));
 }
 

@@ -108,7 +108,10 @@ void k_batch_plain(const BatchProb *__restrict__ probs, const int4 *__restrict__
 // ---------------------------------------------------------------------------------
 // Ozaki phases over all problems.
 // ---------------------------------------------------------------------------------
-// Row maxima: unit = (row r, 64-deep k chunk); rows fastest so lanes touch adjacent rows.
+// Row maxima.  Column-contiguous input: unit = (row r, 64-deep k chunk), rows fastest so
+// lanes read adjacent rows.  Row-contiguous input (KC): every problem's unit count is a
+// multiple of 64 and a wave takes the 64 consecutive k of one (row, chunk), reduced across
+// the wave before a single atomic.
 template <bool A_SIDE, bool KC>
 __global__ __launch_bounds__(256)
 void k_batch_rowmax(const BatchProb *__restrict__ probs, UnitSpace us)
@@ -119,17 +122,35 @@ void k_batch_rowmax(const BatchProb *__restrict__ probs, UnitSpace us)
     const BatchProb &p = probs[pi];
     const int64_t R = A_SIDE ? p.m : p.n;
     const int64_t local = u - us.start[pi];
-    const int64_t r = local % R, kc = local / R;
     const ddv *X = A_SIDE ? p.A : p.B;
     const int64_t ld = A_SIDE ? p.lda : p.ldb;
     unsigned long long *rmax = A_SIDE ? p.rA : p.rB;
-    double mx = 0.0;
-    const int64_t kend = (kc * 64 + 64 < p.k) ? kc * 64 + 64 : p.k;
-    for (int64_t kk = kc * 64; kk < kend; ++kk) mx = nan_max(mx, mag(KC ? X[r * ld + kk] : X[kk * ld + r]));
-    if (mx != 0.0 || mx != mx) atomicMax(&rmax[r], (unsigned long long)__double_as_longlong(mx));
+    if constexpr (KC) {
+        const int64_t nch = (p.k + 63) / 64, w = local / 64;
+        const int lane = (int)(local % 64);
+        const int64_t r = w / nch, kk = (w % nch) * 64 + lane;
+        const double mx = kk < p.k ? mag(X[r * ld + kk]) : 0.0;
+        unsigned long long bits = (unsigned long long)__double_as_longlong(mx);
+        // NaN-aware maximum on the bit patterns of non-negative values.
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned long long o = __shfl_xor(bits, off, 64);
+            bits = o > bits ? o : bits;
+        }
+        if (lane == 0 && bits != 0ull) atomicMax(&rmax[r], bits);
+    } else {
+        const int64_t r = local % R, kc = local / R;
+        double mx = 0.0;
+        const int64_t kend = (kc * 64 + 64 < p.k) ? kc * 64 + 64 : p.k;
+        for (int64_t kk = kc * 64; kk < kend; ++kk) mx = nan_max(mx, mag(X[kk * ld + r]));
+        if (mx != 0.0 || mx != mx) atomicMax(&rmax[r], (unsigned long long)__double_as_longlong(mx));
+    }
 }
 
-// Residue split: unit = (row r, 16-wide k group), ordered for coalescing.
+// Residue split: unit = (row r, 16-wide k group).  Row-contiguous input (KC): k groups
+// fastest.  Column-contiguous input: blocks of 8 rows x 8 k groups per wave (unit counts
+// padded to whole blocks), so the residue chunks a wave writes for one row form a 128-byte
+// line and its loads read 128-byte segments.
 template <bool A_SIDE, bool KC>
 __global__ __launch_bounds__(256)
 void k_batch_split(const BatchProb *__restrict__ probs, UnitSpace us, int bits,
@@ -143,8 +164,15 @@ void k_batch_split(const BatchProb *__restrict__ probs, UnitSpace us, int bits,
     const int64_t G = p.Kc / 16;
     const int64_t local = u - us.start[pi];
     int64_t r, kg;
-    if (KC) { kg = local % G; r = local / G; }
-    else    { r = local % R; kg = local / R; }
+    if (KC) {
+        kg = local % G;
+        r = local / G;
+    } else {
+        const int64_t rb8 = (R + 7) / 8, blk = local / 64, w = local % 64;
+        r = (blk % rb8) * 8 + w / 8;
+        kg = (blk / rb8) * 8 + w % 8;
+        if (r >= R || kg >= G) return;
+    }
     const int64_t rows = A_SIDE ? p.Mp : p.Np;
     oz_split_unit<ddv, KC, false>(A_SIDE ? p.A : p.B, A_SIDE ? p.lda : p.ldb, p.k,
                            A_SIDE ? p.rA : p.rB, bits, 0, mcs, nmod,
@@ -380,6 +408,7 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
     std::vector<int64_t> ra(count + 1, 0), rb(count + 1, 0), sa(count + 1, 0), sb(count + 1, 0),
         cr(count + 1, 0);
     const bool oz = path == MWGEMM_PATH_OZAKI;
+    const bool a_kc = P->ta != 0, b_kc = P->tb == 0;   // as in batch_enqueue
     if (oz_variant < 0) oz_variant = tune_lookup(TK_BATCH, count, vol, kmax) % 16;
     P->oz_tile = (oz_variant == 1) ? 64 : 128;
     const int64_t OT = P->oz_tile;
@@ -420,10 +449,12 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
         offA[p] = ws; ws += rup((int64_t)nmod * q.Mp * q.Kc, 256);
         offB[p] = ws; ws += rup((int64_t)nmod * q.Np * q.Kc, 256);
         offU[p] = ws; ws += rup((int64_t)nmod * q.Np * q.Mp, 256);
-        ra[p + 1] = q.m * ((q.k + 63) / 64);
-        rb[p + 1] = q.n * ((q.k + 63) / 64);
-        sa[p + 1] = q.m * (q.Kc / 16);
-        sb[p + 1] = q.n * (q.Kc / 16);
+        // unit counts of the maxima and split kernels (layouts documented there)
+        const int64_t nch = (q.k + 63) / 64, G = q.Kc / 16;
+        ra[p + 1] = a_kc ? q.m * nch * 64 : q.m * nch;
+        rb[p + 1] = b_kc ? q.n * nch * 64 : q.n * nch;
+        sa[p + 1] = a_kc ? q.m * G : rup(q.m, 8) * rup(G, 8);
+        sb[p + 1] = b_kc ? q.n * G : rup(q.n, 8) * rup(G, 8);
         for (int t = 0; t < nmod; ++t)
             for (int64_t tn = 0; tn < q.Np / OT; ++tn)
                 for (int64_t tm = 0; tm < q.Mp / OT; ++tm) tl.push_back(int4{(int)p, t, (int)tm, (int)tn});

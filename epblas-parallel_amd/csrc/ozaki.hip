@@ -622,19 +622,19 @@ int ozaki_gemm_cplx(const GemmArgsZ &g, hipStream_t s)
 }
 
 int ozaki_residue_gemm_test(const int8_t *dA, const int8_t *dB, uint8_t *dU, int64_t mp,
-                            int64_t np, int64_t kp, int mod_index, hipStream_t s)
+                            int64_t np, int64_t kp, int mod_index, int variant, hipStream_t s)
 {
     std::lock_guard<std::mutex> lk(g_oz_mu);
     if (mp % 256 || np % 256 || kp % 64 || kp <= 0 || kp > OZ_MAX_KEFF || mod_index < 0 ||
-        mod_index >= OZ_MAX_MOD)
-        return record_error(-3, "residue gemm test: bad shape");
+        mod_index >= OZ_MAX_MOD || variant < 0 || variant >= OZ_GEMM_VARIANTS)
+        return record_error(-3, "residue gemm test: bad shape or variant");
     OzPlan plan;
     if (!ozaki_make_plan(1, 8, OZ_MAX_MOD, &plan)) return record_error(-5, "plan");
     OzDevice *d;
     if (int rc = oz_device(&d)) return rc;
     ModConst *mcs;
     if (int rc = oz_table(d, plan, &mcs)) return rc;
-    OZ_TRY(launch_gemm(dA, dB, dU, mp, np, kp, 0, 0, 0, mcs + mod_index, 1, 1, 1, s));
+    OZ_TRY(launch_gemm(dA, dB, dU, mp, np, kp, 0, 0, 0, mcs + mod_index, 1, 1, variant, s));
     return 0;
 }
 

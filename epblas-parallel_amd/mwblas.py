@@ -239,13 +239,14 @@ def ozaki_moduli(nmod):
     return m, y
 
 
-def testing_residue_gemm(dA, dB, dU, mp, np_, kp, mod_index, stream=None):
-    """Run the production INT8 residue GEMM kernel once (test hook)."""
+def testing_residue_gemm(dA, dB, dU, mp, np_, kp, mod_index, variant=1, stream=None):
+    """Run the production INT8 residue GEMM kernel of one tile variant once (test hook):
+    0 = 256x256 DMA ring, 1 = 128x128, 2 = 256x128, 3 = 128x256."""
     f = lib().mwgemm_testing_residue_gemm
     f.restype = ctypes.c_int
     f.argtypes = [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
-                  _vp]
-    rc = f(_ptr(dA), _ptr(dB), _ptr(dU), mp, np_, kp, mod_index,
+                  ctypes.c_int, _vp]
+    rc = f(_ptr(dA), _ptr(dB), _ptr(dU), mp, np_, kp, mod_index, variant,
            None if stream is None else ctypes.c_void_p(stream))
     if rc != 0:
         _check(rc)

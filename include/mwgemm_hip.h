@@ -176,7 +176,8 @@ MWGEMM_API int mwgemm_ozaki_moduli(int nmod, int *m, int *y);
  * mod_index-th modulus and weight of the full 40-modulus table.  mp, np multiples of 256,
  * kp a multiple of 64.  Returns 0 or an error code. */
 MWGEMM_API int mwgemm_testing_residue_gemm(const void *dA, const void *dB, void *dU, int64_t mp,
-                                           int64_t np, int64_t kp, int mod_index, void *stream);
+                                           int64_t np, int64_t kp, int mod_index, int variant,
+                                           void *stream);
 
 /* Kernel timers: while enabled, every kernel the library launches is bracketed by HIP
  * events on its own stream.  mwgemm_timer_read synchronises, then writes up to cap

@@ -22,11 +22,15 @@ def ozaki_path():
     mwblas.set_path(prev)
 
 
-@pytest.mark.parametrize("mod_index", [0, 1, 5, 28, 39])
-def test_residue_gemm_kernel_is_exact(gpu, mod_index):
+@pytest.mark.parametrize("mod_index,variant,kp", [
+    (0, 1, 320), (1, 1, 320), (5, 1, 320), (28, 1, 320), (39, 1, 320),
+    (0, 2, 320), (28, 3, 192),
+    # the DMA-ring kernel: fewer, as many and more 64-deep stages than its ring holds (4)
+    (0, 0, 64), (5, 0, 128), (28, 0, 192), (39, 0, 256), (1, 0, 320), (0, 0, 1088)])
+def test_residue_gemm_kernel_is_exact(gpu, mod_index, variant, kp):
     import torch
-    rng = np.random.default_rng(100 + mod_index)
-    mp, np_, kp = 256, 512, 320
+    rng = np.random.default_rng(100 + mod_index + 7 * variant + kp)
+    mp, np_ = 256, 512
     m_all, y_all = mwblas.ozaki_moduli(40)
     m, y = int(m_all[mod_index]), int(y_all[mod_index])
     lo, hi = (-128, 127) if m == 256 else (-(m - 1) // 2, (m - 1) // 2)
@@ -35,7 +39,8 @@ def test_residue_gemm_kernel_is_exact(gpu, mod_index):
     dA = torch.from_numpy(A.astype(np.int8)).to(gpu)
     dB = torch.from_numpy(B.astype(np.int8)).to(gpu)
     dU = torch.zeros((np_, mp), dtype=torch.uint8, device=gpu)
-    mwblas.testing_residue_gemm(dA, dB, dU, mp, np_, kp, mod_index)
+    print(f"residue gemm variant {variant} mod {mod_index} kp {kp}", flush=True)
+    mwblas.testing_residue_gemm(dA, dB, dU, mp, np_, kp, mod_index, variant)
     torch.cuda.synchronize()
     expect = (((A @ B.T) % m) * y) % m            # (mp, np)
     got = dU.cpu().numpy().astype(np.int64).T
