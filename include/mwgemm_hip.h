@@ -7,7 +7,8 @@
  * links this library in front of (or instead of) epblas-parallel::mwblas and its mgemm_ /
  * wgemm_ calls run on the GPU.  Matrices are column-major; every argument is passed by
  * reference (Fortran ABI); no hidden character-length arguments are declared.  Pointers may
- * be host memory (staged through the device) or device memory (used in place).
+ * be host memory (staged through the device via double-buffered pinned chunks, copied by
+ * a host thread pool) or device memory (used in place).
  *
  * The second group has no counterpart in the reference: device-pointer entry points with
  * an explicit HIP stream, a batched entry point, the numeric-path switch, error reporting
