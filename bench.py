@@ -11,11 +11,12 @@ oracle's restatement of the reference OpenMP mgemm_ timed on this host on a boun
 sample of the same problem.
 
 Further objects on the same line (each with its own workload): "batched" (2000 small GEMMs
-in one captured launch per path), "complex" (4096^3 with op(A)='C', both paths) and
-"host_boundary" (the drop-in mgemm_ with host operands, PCIe transfers inside the timing).
+in one captured launch per path), "complex" (4096^3 with op(A)='C', both paths), "gemmtr"
+(the triangular-output GEMM, 4096 x 4096 upper triangle, k = 4096) and "host_boundary" (the
+drop-in mgemm_ with host operands, PCIe transfers inside the timing).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--size 8192]
-                    [--no-batched] [--no-complex] [--no-host]
+                    [--no-batched] [--no-complex] [--no-gemmtr] [--no-host]
 """
 import argparse
 import json
@@ -207,6 +208,47 @@ def complex_bench(dev, size=4096, oz_reps=5, plain_reps=1, seed=4096):
     return out
 
 
+def gemmtr_bench(dev, n=4096, k=4096, reps=3, seed=41):
+    """Triangular-output GEMM (mgemmtr_ semantics, plain path, bit-exact): upper triangle
+    of an n x n C, NN, random alpha/beta, inputs in HBM.  Effective rate = n(n+1)k / t
+    (2 flops per multiply-add over the n(n+1)/2 triangle entries)."""
+    import numpy as np
+    import torch
+    import mwblas
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    A = gen_uniform(n, k, gen, dev)
+    B = gen_uniform(k, n, gen, dev)
+    C0 = gen_uniform(n, n, gen, dev)
+    rng = np.random.default_rng(seed)
+    alpha, beta = random_scalar(rng, 1), random_scalar(rng, 1)
+    stream = torch.cuda.current_stream().cuda_stream
+    C = C0.clone()
+    mwblas.gemmtr_device(False, "U", "N", "N", n, k, alpha, A, n, B, k, beta, C, n, stream=stream)
+    torch.cuda.synchronize()
+    mwblas.timer_reset()
+    mwblas.timer_enable(True)
+    t = 0.0
+    for _ in range(reps):
+        C.copy_(C0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        mwblas.gemmtr_device(False, "U", "N", "N", n, k, alpha, A, n, B, k, beta, C, n,
+                             stream=stream)
+        torch.cuda.synchronize()
+        t += time.perf_counter() - t0
+    mwblas.timer_enable(False)
+    kt = mwblas.timer_read()
+    t /= reps
+    macs = n * (n + 1) / 2 * k
+    return {"workload": f"mgemmtr 'U','N','N' n={n} k={k}, random alpha/beta, plain path",
+            "value": round(2 * macs / t / 1e9, 2), "unit": "DD-GFLOP/s",
+            "ms_per_call": round(t * 1e3, 3),
+            # 18 FP64 lane-ops per DD multiply-add against 39.3e12 lane-ops/s
+            "fp64_frac": round(18 * macs / t / 1e12 / (FP64_PEAK_TFLOPS / 2), 4),
+            "kernels_ms_per_call": {kn: round(v[0] / reps, 3) for kn, v in kt.items()}}
+
+
 def host_boundary_bench(size=4096, reps=3, seed=77):
     """The drop-in mgemm_ with host (pageable) operands: H2D of A and B, the GEMM on the
     exact-product path, D2H of C, all inside the timed call (PCIe-inclusive).  Real NN at
@@ -255,6 +297,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-complex", action="store_true")
     ap.add_argument("--no-host", action="store_true")
+    ap.add_argument("--no-gemmtr", action="store_true")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="first run one device copy of this many MiB (the known-size kernel "
                          "that scales hardware byte counters)")
@@ -381,6 +424,8 @@ def main():
         result["batched"] = batched_bench(dev)
     if not args.no_complex:
         result["complex"] = complex_bench(dev)
+    if not args.no_gemmtr:
+        result["gemmtr"] = gemmtr_bench(dev)
     if not args.no_host:
         result["host_boundary"] = host_boundary_bench()
     if not args.no_cpu:

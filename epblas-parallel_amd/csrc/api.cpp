@@ -302,6 +302,63 @@ int gemm_real_device(GemmArgsR g, int path, hipStream_t s)
     return 0;
 }
 
+// Column block of the triangular-output GEMM's diagonal triangles: MGEMMTR_NB /
+// WGEMMTR_NB as the reference reads them (mgemmtr_serial.cpp:104-112,
+// wgemmtr_serial.cpp), default 8.  The block decides which elements take the scalar path,
+// so it is part of the bit-level contract.
+int64_t gemmtr_nb(bool cplx)
+{
+    auto from_env = [](const char *name) -> int64_t {
+        const char *e = getenv(name);
+        const long v = e ? atol(e) : 0;
+        return v > 0 ? (int64_t)v : 8;
+    };
+    static const int64_t nb_real = from_env("MGEMMTR_NB"), nb_cplx = from_env("WGEMMTR_NB");
+    return cplx ? nb_cplx : nb_real;
+}
+
+int uplo_upper(char uplo)
+{
+    const char u = (char)toupper((unsigned char)uplo);
+    return u == 'U' ? 1 : u == 'L' ? 0 : -1;
+}
+
+// Triangular-output GEMM on device memory (g.m == g.n); the plain path always (its result
+// is bit-for-bit the reference's).
+int gemmtr_real_device(GemmArgsR g, char uplo, hipStream_t s)
+{
+    const int upper = uplo_upper(uplo);
+    if (upper < 0) return record_error(-2, "mgemmtr: invalid uplo letter");
+    if (g.opa < 0 || g.opb < 0) return record_error(-2, "mgemmtr: invalid trans letter");
+    if (g.n <= 0) return 0;
+    const int64_t arows = g.opa == OP_N ? g.n : g.k;
+    const int64_t brows = g.opb == OP_N ? g.k : g.n;
+    const bool need_ab = g.k > 0 && !dd_is0(g.alpha);
+    if (g.ldc < g.n || (need_ab && (g.lda < (arows > 1 ? arows : 1) || g.ldb < (brows > 1 ? brows : 1))))
+        return record_error(-3, "mgemmtr: leading dimension too small (lda=%lld ldb=%lld ldc=%lld)",
+                            (long long)g.lda, (long long)g.ldb, (long long)g.ldc);
+    g.beta_mode = dd_is0(g.beta) ? BETA_ZERO : dd_is1(g.beta) ? BETA_ONE : BETA_SCALE;
+    MW_TRY(plain_gemmtr_real(g, upper == 1, gemmtr_nb(false), s));
+    return 0;
+}
+
+int gemmtr_cplx_device(GemmArgsZ g, char uplo, hipStream_t s)
+{
+    const int upper = uplo_upper(uplo);
+    if (upper < 0) return record_error(-2, "wgemmtr: invalid uplo letter");
+    if (g.opa < 0 || g.opb < 0) return record_error(-2, "wgemmtr: invalid trans letter");
+    if (g.n <= 0) return 0;
+    const int64_t arows = g.opa == OP_N ? g.n : g.k;
+    const int64_t brows = g.opb == OP_N ? g.k : g.n;
+    const bool need_ab = g.k > 0 && !zdd_is0(g.alpha);
+    if (g.ldc < g.n || (need_ab && (g.lda < (arows > 1 ? arows : 1) || g.ldb < (brows > 1 ? brows : 1))))
+        return record_error(-3, "wgemmtr: leading dimension too small (lda=%lld ldb=%lld ldc=%lld)",
+                            (long long)g.lda, (long long)g.ldb, (long long)g.ldc);
+    g.beta_mode = zdd_is0(g.beta) ? BETA_ZERO : zdd_is1(g.beta) ? BETA_ONE : BETA_SCALE;
+    MW_TRY(plain_gemmtr_cplx(g, upper == 1, gemmtr_nb(true), s));
+    return 0;
+}
+
 int gemm_cplx_device(GemmArgsZ g, int path, hipStream_t s)
 {
     if (g.m <= 0 || g.n <= 0) return 0;
@@ -441,6 +498,90 @@ int wgemm_host(char transa, char transb, int64_t m, int64_t n, int64_t k,
                              [path](GemmArgsZ gg, hipStream_t s) { return gemm_cplx_device(gg, path, s); });
 }
 
+GemmArgsR real_args(char transa, char transb, int64_t m, int64_t n, int64_t k,
+                    const mw_float64x2 *alpha, const mw_float64x2 *a, int64_t lda,
+                    const mw_float64x2 *b, int64_t ldb, const mw_float64x2 *beta,
+                    mw_float64x2 *c, int64_t ldc)
+{
+    GemmArgsR g{};
+    g.m = m; g.n = n; g.k = k;
+    g.opa = op_real(transa); g.opb = op_real(transb);
+    g.alpha = ddv{alpha->limbs[0], alpha->limbs[1]};
+    g.beta = ddv{beta->limbs[0], beta->limbs[1]};
+    g.A = reinterpret_cast<const ddv *>(a); g.lda = lda;
+    g.B = reinterpret_cast<const ddv *>(b); g.ldb = ldb;
+    g.C = reinterpret_cast<ddv *>(c); g.ldc = ldc;
+    return g;
+}
+
+GemmArgsZ cplx_args(char transa, char transb, int64_t m, int64_t n, int64_t k,
+                    const mw_complex64x2 *alpha, const mw_complex64x2 *a, int64_t lda,
+                    const mw_complex64x2 *b, int64_t ldb, const mw_complex64x2 *beta,
+                    mw_complex64x2 *c, int64_t ldc)
+{
+    GemmArgsZ g{};
+    g.m = m; g.n = n; g.k = k;
+    g.opa = op_cplx(transa); g.opb = op_cplx(transb);
+    g.alpha = zddv{{alpha->re.limbs[0], alpha->re.limbs[1]}, {alpha->im.limbs[0], alpha->im.limbs[1]}};
+    g.beta = zddv{{beta->re.limbs[0], beta->re.limbs[1]}, {beta->im.limbs[0], beta->im.limbs[1]}};
+    g.A = reinterpret_cast<const zddv *>(a); g.lda = lda;
+    g.B = reinterpret_cast<const zddv *>(b); g.ldb = ldb;
+    g.C = reinterpret_cast<zddv *>(c); g.ldc = ldc;
+    return g;
+}
+
+// Triangular-output GEMM through host (or device) pointers.  C always travels both ways:
+// the complementary triangle must come back as it was.
+int mgemmtr_host(char uplo, char transa, char transb, int64_t n, int64_t k,
+                 const mw_float64x2 *alpha, const mw_float64x2 *a, int64_t lda,
+                 const mw_float64x2 *b, int64_t ldb, const mw_float64x2 *beta, mw_float64x2 *c,
+                 int64_t ldc)
+{
+    if (n <= 0) return 0;
+    GemmArgsR g = real_args(transa, transb, n, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+    if (uplo_upper(uplo) < 0) return record_error(-2, "mgemmtr: invalid uplo letter");
+    if (g.opa < 0 || g.opb < 0) return record_error(-2, "mgemmtr: invalid trans letter");
+    const bool need_ab = k > 0 && !dd_is0(g.alpha);
+    const int64_t ar = need_ab ? (g.opa == OP_N ? n : k) : 0, ac = g.opa == OP_N ? k : n;
+    const int64_t br = need_ab ? (g.opb == OP_N ? k : n) : 0, bc = g.opb == OP_N ? n : k;
+    if (ldc < n || (need_ab && (lda < ar || ldb < br)))
+        return record_error(-3, "mgemmtr: leading dimension too small (lda=%lld ldb=%lld ldc=%lld)",
+                            (long long)lda, (long long)ldb, (long long)ldc);
+    return staged_call<ddv>(g, ar, ac, br, bc, false,
+                            [uplo](GemmArgsR gg, hipStream_t s) { return gemmtr_real_device(gg, uplo, s); });
+}
+
+int wgemmtr_host(char uplo, char transa, char transb, int64_t n, int64_t k,
+                 const mw_complex64x2 *alpha, const mw_complex64x2 *a, int64_t lda,
+                 const mw_complex64x2 *b, int64_t ldb, const mw_complex64x2 *beta,
+                 mw_complex64x2 *c, int64_t ldc)
+{
+    if (n <= 0) return 0;
+    GemmArgsZ g = cplx_args(transa, transb, n, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+    if (uplo_upper(uplo) < 0) return record_error(-2, "wgemmtr: invalid uplo letter");
+    if (g.opa < 0 || g.opb < 0) return record_error(-2, "wgemmtr: invalid trans letter");
+    const bool need_ab = k > 0 && !zdd_is0(g.alpha);
+    const int64_t ar = need_ab ? (g.opa == OP_N ? n : k) : 0, ac = g.opa == OP_N ? k : n;
+    const int64_t br = need_ab ? (g.opb == OP_N ? k : n) : 0, bc = g.opb == OP_N ? n : k;
+    if (ldc < n || (need_ab && (lda < ar || ldb < br)))
+        return record_error(-3, "wgemmtr: leading dimension too small (lda=%lld ldb=%lld ldc=%lld)",
+                            (long long)lda, (long long)ldb, (long long)ldc);
+    return staged_call<zddv>(g, ar, ac, br, bc, false,
+                             [uplo](GemmArgsZ gg, hipStream_t s) { return gemmtr_cplx_device(gg, uplo, s); });
+}
+
+hipStream_t resolve_stream(void *stream, int *rc)
+{
+    *rc = 0;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (s) return s;
+    int dev = 0;
+    if ((*rc = current_device(&dev))) return nullptr;
+    DevState *st = dev_state(dev);
+    if ((*rc = ensure_stream(st))) return nullptr;
+    return st->stream;
+}
+
 void finish(int rc)
 {
     if (rc != 0 && abort_on_error()) {
@@ -462,7 +603,7 @@ double g_timer_ms[KC_COUNT];
 int64_t g_timer_cnt[KC_COUNT];
 const char *kClassNames[KC_COUNT] = {
     "plain_real", "plain_cplx", "beta_only", "oz_exponent", "oz_split", "oz_gemm", "oz_crt",
-    "batch_plain", "batch_oz_split", "batch_oz_gemm", "batch_oz_crt"};
+    "batch_plain", "batch_oz_split", "batch_oz_gemm", "batch_oz_crt", "gemmtr_diag"};
 
 void drain_slots()
 {
@@ -557,6 +698,78 @@ void wgemm_serial(char transa, char transb, ptrdiff_t m, ptrdiff_t n, ptrdiff_t 
                   mw_complex64x2 *c, ptrdiff_t ldc)
 {
     finish(wgemm_host(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc));
+}
+
+void mgemmtr_(const char *uplo, const char *transa, const char *transb, const int *n,
+              const int *k, const mw_float64x2 *alpha, const mw_float64x2 *a, const int *lda,
+              const mw_float64x2 *b, const int *ldb, const mw_float64x2 *beta, mw_float64x2 *c,
+              const int *ldc)
+{
+    finish(mgemmtr_host(*uplo, *transa, *transb, *n, *k, alpha, a, *lda, b, *ldb, beta, c, *ldc));
+}
+
+void mgemmtr_64_(const char *uplo, const char *transa, const char *transb, const int64_t *n,
+                 const int64_t *k, const mw_float64x2 *alpha, const mw_float64x2 *a,
+                 const int64_t *lda, const mw_float64x2 *b, const int64_t *ldb,
+                 const mw_float64x2 *beta, mw_float64x2 *c, const int64_t *ldc)
+{
+    finish(mgemmtr_host(*uplo, *transa, *transb, *n, *k, alpha, a, *lda, b, *ldb, beta, c, *ldc));
+}
+
+void wgemmtr_(const char *uplo, const char *transa, const char *transb, const int *n,
+              const int *k, const mw_complex64x2 *alpha, const mw_complex64x2 *a,
+              const int *lda, const mw_complex64x2 *b, const int *ldb,
+              const mw_complex64x2 *beta, mw_complex64x2 *c, const int *ldc)
+{
+    finish(wgemmtr_host(*uplo, *transa, *transb, *n, *k, alpha, a, *lda, b, *ldb, beta, c, *ldc));
+}
+
+void wgemmtr_64_(const char *uplo, const char *transa, const char *transb, const int64_t *n,
+                 const int64_t *k, const mw_complex64x2 *alpha, const mw_complex64x2 *a,
+                 const int64_t *lda, const mw_complex64x2 *b, const int64_t *ldb,
+                 const mw_complex64x2 *beta, mw_complex64x2 *c, const int64_t *ldc)
+{
+    finish(wgemmtr_host(*uplo, *transa, *transb, *n, *k, alpha, a, *lda, b, *ldb, beta, c, *ldc));
+}
+
+void mgemmtr_serial(char uplo, char transa, char transb, ptrdiff_t n, ptrdiff_t k,
+                    const mw_float64x2 *alpha, const mw_float64x2 *a, ptrdiff_t lda,
+                    const mw_float64x2 *b, ptrdiff_t ldb, const mw_float64x2 *beta,
+                    mw_float64x2 *c, ptrdiff_t ldc)
+{
+    finish(mgemmtr_host(uplo, transa, transb, n, k, alpha, a, lda, b, ldb, beta, c, ldc));
+}
+
+void wgemmtr_serial(char uplo, char transa, char transb, ptrdiff_t n, ptrdiff_t k,
+                    const mw_complex64x2 *alpha, const mw_complex64x2 *a, ptrdiff_t lda,
+                    const mw_complex64x2 *b, ptrdiff_t ldb, const mw_complex64x2 *beta,
+                    mw_complex64x2 *c, ptrdiff_t ldc)
+{
+    finish(wgemmtr_host(uplo, transa, transb, n, k, alpha, a, lda, b, ldb, beta, c, ldc));
+}
+
+int mwgemm_mgemmtr_device(char uplo, char transa, char transb, int64_t n, int64_t k,
+                          const mw_float64x2 *alpha, const mw_float64x2 *dA, int64_t lda,
+                          const mw_float64x2 *dB, int64_t ldb, const mw_float64x2 *beta,
+                          mw_float64x2 *dC, int64_t ldc, void *stream)
+{
+    int rc = 0;
+    hipStream_t s = resolve_stream(stream, &rc);
+    if (rc) return rc;
+    return gemmtr_real_device(real_args(transa, transb, n, n, k, alpha, dA, lda, dB, ldb, beta,
+                                        dC, ldc), uplo, s);
+}
+
+int mwgemm_wgemmtr_device(char uplo, char transa, char transb, int64_t n, int64_t k,
+                          const mw_complex64x2 *alpha, const mw_complex64x2 *dA, int64_t lda,
+                          const mw_complex64x2 *dB, int64_t ldb, const mw_complex64x2 *beta,
+                          mw_complex64x2 *dC, int64_t ldc, void *stream)
+{
+    int rc = 0;
+    hipStream_t s = resolve_stream(stream, &rc);
+    if (rc) return rc;
+    return gemmtr_cplx_device(cplx_args(transa, transb, n, n, k, alpha, dA, lda, dB, ldb, beta,
+                                        dC, ldc), uplo, s);
 }
 
 int mwgemm_set_path(int path)

@@ -52,5 +52,8 @@ hipError_t plain_gemm_real(const GemmArgsR &g, hipStream_t s);
 hipError_t plain_gemm_cplx(const GemmArgsZ &g, hipStream_t s);
 hipError_t plain_beta_only_real(const GemmArgsR &g, hipStream_t s);
 hipError_t plain_beta_only_cplx(const GemmArgsZ &g, hipStream_t s);
+// Triangular-output GEMM on the UPLO triangle of the n x n C (g.m == g.n), column blocks of nb.
+hipError_t plain_gemmtr_real(const GemmArgsR &g, bool upper, int64_t nb, hipStream_t s);
+hipError_t plain_gemmtr_cplx(const GemmArgsZ &g, bool upper, int64_t nb, hipStream_t s);
 
 }  // namespace mw

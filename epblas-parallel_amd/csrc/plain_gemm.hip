@@ -23,29 +23,31 @@ namespace mw {
 // ----------------------------------------------------------------------------------
 // Real kernel.
 // ----------------------------------------------------------------------------------
-template <int TM, int TN, int BK, bool TA, bool TB>
+template <int TM, int TN, int BK, bool TA, bool TB, int TRI = TRI_NONE>
 __global__ __launch_bounds__(256)
 void k_mgemm_plain(int64_t m, int64_t n, int64_t k, ddv alpha, ddv beta, int beta_mode,
                    const ddv *__restrict__ A, int64_t lda, const ddv *__restrict__ B,
-                   int64_t ldb, ddv *__restrict__ C, int64_t ldc, int tiles_m)
+                   int64_t ldb, ddv *__restrict__ C, int64_t ldc, int tiles_m, int64_t tri_nb)
 {
     constexpr int BM = 16 * TM, BN = 16 * TN;
     __shared__ __attribute__((aligned(16))) ddv As[BK][BM];
     __shared__ __attribute__((aligned(16))) ddv Bs[BK][BN];
     const int64_t i0 = (int64_t)(blockIdx.x % tiles_m) * BM;
     const int64_t j0 = (int64_t)(blockIdx.x / tiles_m) * BN;
-    mgemm_plain_tile<TM, TN, BK, TA, TB>(m, n, k, alpha, beta, beta_mode, A, lda, B, ldb, C, ldc,
-                                         i0, j0, As, Bs);
+    if constexpr (TRI != TRI_NONE)
+        if (!tri_tile_live<TRI>(i0, j0, BM, BN, n, tri_nb)) return;
+    mgemm_plain_tile<TM, TN, BK, TA, TB, TRI>(m, n, k, alpha, beta, beta_mode, A, lda, B, ldb, C,
+                                              ldc, i0, j0, As, Bs, tri_nb);
 }
 
 // ----------------------------------------------------------------------------------
 // Complex kernel (same structure; op letters N/T/C with conjugation in the staging).
 // ----------------------------------------------------------------------------------
-template <int TM, int TN, int BK, int OPA, int OPB>
+template <int TM, int TN, int BK, int OPA, int OPB, int TRI = TRI_NONE>
 __global__ __launch_bounds__(256)
 void k_wgemm_plain(int64_t m, int64_t n, int64_t k, zddv alpha, zddv beta, int beta_mode,
                    const zddv *__restrict__ A, int64_t lda, const zddv *__restrict__ B,
-                   int64_t ldb, zddv *__restrict__ C, int64_t ldc, int tiles_m)
+                   int64_t ldb, zddv *__restrict__ C, int64_t ldc, int tiles_m, int64_t tri_nb)
 {
     constexpr int BM = 16 * TM, BN = 16 * TN;
     static_assert(KCHUNK % BK == 0, "BK must divide the flush period");
@@ -56,6 +58,8 @@ void k_wgemm_plain(int64_t m, int64_t n, int64_t k, zddv alpha, zddv beta, int b
     const int tx = tid & 15, ty = tid >> 4;
     const int64_t i0 = (int64_t)(blockIdx.x % tiles_m) * BM;
     const int64_t j0 = (int64_t)(blockIdx.x / tiles_m) * BN;
+    if constexpr (TRI != TRI_NONE)
+        if (!tri_tile_live<TRI>(i0, j0, BM, BN, n, tri_nb)) return;
 
     SlabA<zddv, BM, BK, OPA != OP_N, OPA == OP_C> sa;
     SlabB<zddv, BN, BK, OPB != OP_N, OPB == OP_C> sb;
@@ -65,6 +69,14 @@ void k_wgemm_plain(int64_t m, int64_t n, int64_t k, zddv alpha, zddv beta, int b
     for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[i][j] = zddv{{0.0, 0.0}, {0.0, 0.0}};
+    uint32_t keep = 0;   // written elements of the lane's register tile (see mgemm_plain_tile)
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const int64_t gi = i0 + tx + 16 * i, gj = j0 + ty + 16 * j;
+            if (gi < m && gj < n && tri_rect<TRI>(gi, gj, tri_nb)) keep |= 1u << (i * TN + j);
+        }
 
     const int64_t ntiles = (k + BK - 1) / BK;
     sa.load(A, lda, m, k, i0, 0, tid);
@@ -101,7 +113,7 @@ void k_wgemm_plain(int64_t m, int64_t n, int64_t k, zddv alpha, zddv beta, int b
 #pragma unroll
                 for (int j = 0; j < TN; ++j) {
                     const int64_t gj = j0 + ty + 16 * j;
-                    if (gi < m && gj < n) {
+                    if ((keep >> (i * TN + j)) & 1u) {
                         zddv *cp = &C[gj * ldc + gi];
                         zddv c = *cp;
                         if (first) c = zdd_beta_prepass(c, beta, beta_mode);
@@ -146,23 +158,136 @@ void k_wgemm_beta_only(int64_t m, int64_t n, zddv beta, int beta_mode, zddv *__r
 }
 
 // ----------------------------------------------------------------------------------
+// Triangular-output GEMM (gemmtr): the nb x nb diagonal triangles, one lane per element,
+// with the reference's scalar faithful sequence (mgemmtr_serial.cpp:48-95 / complex
+// wgemmtr_serial.cpp:48-95): the beta scaling, then for op(A) = 'N'
+//     per l ascending: t = alpha * b_lj; skipped when t == 0; c += t * a_il
+// and otherwise  s = sum_l a_li * b_lj (ascending, faithful), c += alpha * s.
+// Lane (j, r): column j, r-th row of its block's triangle part of that column.
+// ----------------------------------------------------------------------------------
+DDI bool tri_diag_row(int64_t j, int64_t r, int64_t n, int64_t nb, bool upper, int64_t &i)
+{
+    const int64_t jc = (j / nb) * nb;
+    if (upper) {
+        i = jc + r;
+        return i <= j;
+    }
+    i = j + r;
+    return i < jc + nb && i < n;
+}
+
+// The chain over l is sequential, so operands are loaded DIAG_PF steps ahead of their use:
+// each lane then waits once per DIAG_PF steps on memory instead of once per step.
+constexpr int DIAG_PF = 8;
+
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256)
+void k_mgemmtr_diag(int64_t n, int64_t k, int64_t nb, int upper, ddv alpha, ddv beta,
+                    int beta_mode, const ddv *__restrict__ A, int64_t lda,
+                    const ddv *__restrict__ B, int64_t ldb, ddv *__restrict__ C, int64_t ldc)
+{
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t j = t / nb, r = t % nb;
+    int64_t i;
+    if (j >= n || !tri_diag_row(j, r, n, nb, upper != 0, i)) return;
+    ddv c = dd_beta_prepass(C[j * ldc + i], beta, beta_mode);
+    ddv acc{0.0, 0.0};
+    for (int64_t l0 = 0; l0 < k; l0 += DIAG_PF) {
+        ddv av[DIAG_PF], bv[DIAG_PF];
+#pragma unroll
+        for (int q = 0; q < DIAG_PF; ++q) {
+            const int64_t l = l0 + q < k ? l0 + q : k - 1;
+            av[q] = TA ? A[i * lda + l] : A[l * lda + i];
+            bv[q] = TB ? B[l * ldb + j] : B[j * ldb + l];
+        }
+#pragma unroll
+        for (int q = 0; q < DIAG_PF; ++q) {
+            if (l0 + q >= k) break;
+            if (!TA) {
+                const ddv tl = dd_mul_exact(alpha, bv[q]);
+                if (tl.hi == 0.0 && tl.lo == 0.0) continue;
+                c = dd_add_exact(c, dd_mul_exact(tl, av[q]));
+            } else {
+                acc = dd_add_exact(acc, dd_mul_exact(av[q], bv[q]));
+            }
+        }
+    }
+    if (TA) c = dd_add_exact(c, dd_mul_exact(alpha, acc));
+    C[j * ldc + i] = c;
+}
+
+template <int OPA, int OPB>
+__global__ __launch_bounds__(256)
+void k_wgemmtr_diag(int64_t n, int64_t k, int64_t nb, int upper, zddv alpha, zddv beta,
+                    int beta_mode, const zddv *__restrict__ A, int64_t lda,
+                    const zddv *__restrict__ B, int64_t ldb, zddv *__restrict__ C, int64_t ldc)
+{
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t j = t / nb, r = t % nb;
+    int64_t i;
+    if (j >= n || !tri_diag_row(j, r, n, nb, upper != 0, i)) return;
+    zddv c = zdd_beta_prepass(C[j * ldc + i], beta, beta_mode);
+    zddv acc{{0.0, 0.0}, {0.0, 0.0}};
+    for (int64_t l0 = 0; l0 < k; l0 += DIAG_PF) {
+        zddv av[DIAG_PF], bv[DIAG_PF];
+#pragma unroll
+        for (int q = 0; q < DIAG_PF; ++q) {
+            const int64_t l = l0 + q < k ? l0 + q : k - 1;
+            const zddv a = OPA == OP_N ? A[l * lda + i] : A[i * lda + l];
+            const zddv b = OPB == OP_N ? B[j * ldb + l] : B[l * ldb + j];
+            av[q] = OPA == OP_C ? zdd_conj(a) : a;
+            bv[q] = OPB == OP_C ? zdd_conj(b) : b;
+        }
+#pragma unroll
+        for (int q = 0; q < DIAG_PF; ++q) {
+            if (l0 + q >= k) break;
+            if (OPA == OP_N) {
+                const zddv tl = zdd_mul_exact(alpha, bv[q]);
+                if (tl.re.hi == 0.0 && tl.re.lo == 0.0 && tl.im.hi == 0.0 && tl.im.lo == 0.0)
+                    continue;
+                c = zdd_add_exact(c, zdd_mul_exact(tl, av[q]));
+            } else {
+                acc = zdd_add_exact(acc, zdd_mul_exact(av[q], bv[q]));
+            }
+        }
+    }
+    if (OPA != OP_N) c = zdd_add_exact(c, zdd_mul_exact(alpha, acc));
+    C[j * ldc + i] = c;
+}
+
+// alpha == 0 or k == 0: the triangle's beta scaling alone (mgemmtr_parallel.cpp:43-52).
+template <typename V>
+__global__ __launch_bounds__(256)
+void k_gemmtr_beta_only(int64_t n, int upper, V beta, int beta_mode, V *__restrict__ C,
+                        int64_t ldc)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n * n) return;
+    const int64_t i = e % n, j = e / n;
+    if (upper ? i > j : i < j) return;
+    V *cp = &C[j * ldc + i];
+    if constexpr (std::is_same<V, ddv>::value) *cp = dd_beta_prepass(*cp, beta, beta_mode);
+    else *cp = zdd_beta_prepass(*cp, beta, beta_mode);
+}
+
+// ----------------------------------------------------------------------------------
 // Launchers.
 // ----------------------------------------------------------------------------------
 namespace {
 constexpr int RTM = 4, RTN = 4, RBK = 16;   // real tile: 64 x 64, 16-deep slabs
 constexpr int CTM = 2, CTN = 2, CBK = 16;   // complex tile: 32 x 32
 
-template <int TM, int TN, bool TA, bool TB>
-hipError_t launch_real_tile(const GemmArgsR &g, hipStream_t s)
+template <int TM, int TN, bool TA, bool TB, int TRI = TRI_NONE>
+hipError_t launch_real_tile(const GemmArgsR &g, hipStream_t s, int64_t tri_nb = 0)
 {
     constexpr int BM = 16 * TM, BN = 16 * TN;
     int tiles_m = (int)((g.m + BM - 1) / BM);
     int64_t tiles = (int64_t)tiles_m * ((g.n + BN - 1) / BN);
     if (tiles > INT32_MAX) return hipErrorInvalidValue;
     ScopedTimer tm(KC_PLAIN_REAL, s);
-    hipLaunchKernelGGL((k_mgemm_plain<TM, TN, RBK, TA, TB>), dim3((unsigned)tiles), dim3(256),
-                       0, s, g.m, g.n, g.k, g.alpha, g.beta, g.beta_mode, g.A, g.lda, g.B,
-                       g.ldb, g.C, g.ldc, tiles_m);
+    hipLaunchKernelGGL((k_mgemm_plain<TM, TN, RBK, TA, TB, TRI>), dim3((unsigned)tiles),
+                       dim3(256), 0, s, g.m, g.n, g.k, g.alpha, g.beta, g.beta_mode, g.A, g.lda,
+                       g.B, g.ldb, g.C, g.ldc, tiles_m, tri_nb);
     return hipGetLastError();
 }
 
@@ -179,17 +304,17 @@ hipError_t launch_real(const GemmArgsR &g, hipStream_t s)
     return launch_real_tile<RTM, RTN, TA, TB>(g, s);
 }
 
-template <int OA, int OB>
-hipError_t launch_cplx(const GemmArgsZ &g, hipStream_t s)
+template <int OA, int OB, int TRI = TRI_NONE>
+hipError_t launch_cplx(const GemmArgsZ &g, hipStream_t s, int64_t tri_nb = 0)
 {
     constexpr int BM = 16 * CTM, BN = 16 * CTN;
     int tiles_m = (int)((g.m + BM - 1) / BM);
     int64_t tiles = (int64_t)tiles_m * ((g.n + BN - 1) / BN);
     if (tiles > INT32_MAX) return hipErrorInvalidValue;
     ScopedTimer tm(KC_PLAIN_CPLX, s);
-    hipLaunchKernelGGL((k_wgemm_plain<CTM, CTN, CBK, OA, OB>), dim3((unsigned)tiles), dim3(256),
-                       0, s, g.m, g.n, g.k, g.alpha, g.beta, g.beta_mode, g.A, g.lda, g.B,
-                       g.ldb, g.C, g.ldc, tiles_m);
+    hipLaunchKernelGGL((k_wgemm_plain<CTM, CTN, CBK, OA, OB, TRI>), dim3((unsigned)tiles),
+                       dim3(256), 0, s, g.m, g.n, g.k, g.alpha, g.beta, g.beta_mode, g.A, g.lda,
+                       g.B, g.ldb, g.C, g.ldc, tiles_m, tri_nb);
     return hipGetLastError();
 }
 
@@ -200,6 +325,45 @@ hipError_t launch_cplx_b(const GemmArgsZ &g, hipStream_t s)
     case OP_N: return launch_cplx<OA, OP_N>(g, s);
     case OP_T: return launch_cplx<OA, OP_T>(g, s);
     default:   return launch_cplx<OA, OP_C>(g, s);
+    }
+}
+
+// gemmtr: the masked rectangle kernel, then the diagonal triangles.
+template <bool TA, bool TB>
+hipError_t launch_gemmtr_real(const GemmArgsR &g, bool upper, int64_t nb, hipStream_t s)
+{
+    hipError_t e = upper ? launch_real_tile<RTM, RTN, TA, TB, TRI_UPPER>(g, s, nb)
+                         : launch_real_tile<RTM, RTN, TA, TB, TRI_LOWER>(g, s, nb);
+    if (e != hipSuccess) return e;
+    ScopedTimer tm(KC_GEMMTR_DIAG, s);
+    const int64_t lanes = g.n * nb;
+    hipLaunchKernelGGL((k_mgemmtr_diag<TA, TB>), dim3((unsigned)((lanes + 255) / 256)),
+                       dim3(256), 0, s, g.n, g.k, nb, upper ? 1 : 0, g.alpha, g.beta,
+                       g.beta_mode, g.A, g.lda, g.B, g.ldb, g.C, g.ldc);
+    return hipGetLastError();
+}
+
+template <int OA, int OB>
+hipError_t launch_gemmtr_cplx(const GemmArgsZ &g, bool upper, int64_t nb, hipStream_t s)
+{
+    hipError_t e = upper ? launch_cplx<OA, OB, TRI_UPPER>(g, s, nb)
+                         : launch_cplx<OA, OB, TRI_LOWER>(g, s, nb);
+    if (e != hipSuccess) return e;
+    ScopedTimer tm(KC_GEMMTR_DIAG, s);
+    const int64_t lanes = g.n * nb;
+    hipLaunchKernelGGL((k_wgemmtr_diag<OA, OB>), dim3((unsigned)((lanes + 255) / 256)),
+                       dim3(256), 0, s, g.n, g.k, nb, upper ? 1 : 0, g.alpha, g.beta,
+                       g.beta_mode, g.A, g.lda, g.B, g.ldb, g.C, g.ldc);
+    return hipGetLastError();
+}
+
+template <int OA>
+hipError_t launch_gemmtr_cplx_b(const GemmArgsZ &g, bool upper, int64_t nb, hipStream_t s)
+{
+    switch (g.opb) {
+    case OP_N: return launch_gemmtr_cplx<OA, OP_N>(g, upper, nb, s);
+    case OP_T: return launch_gemmtr_cplx<OA, OP_T>(g, upper, nb, s);
+    default:   return launch_gemmtr_cplx<OA, OP_C>(g, upper, nb, s);
     }
 }
 }  // namespace
@@ -231,6 +395,46 @@ hipError_t plain_gemm_real(const GemmArgsR &g, hipStream_t s)
     if (ta && !tb)  return launch_real<true, false>(g, s);
     if (!ta && tb)  return launch_real<false, true>(g, s);
     return launch_real<true, true>(g, s);
+}
+
+// g.m == g.n == n.  alpha == 0 or k == 0 scale the triangle only; beta == 1 then does
+// nothing (mgemmtr_parallel.cpp:43-55).
+hipError_t plain_gemmtr_real(const GemmArgsR &g, bool upper, int64_t nb, hipStream_t s)
+{
+    if (g.n <= 0) return hipSuccess;
+    if (g.k == 0 || (g.alpha.hi == 0.0 && g.alpha.lo == 0.0)) {
+        if (g.beta_mode == BETA_ONE) return hipSuccess;
+        ScopedTimer tm(KC_BETA_ONLY, s);
+        const int64_t cnt = g.n * g.n;
+        hipLaunchKernelGGL((k_gemmtr_beta_only<ddv>), dim3((unsigned)((cnt + 255) / 256)),
+                           dim3(256), 0, s, g.n, upper ? 1 : 0, g.beta, g.beta_mode, g.C, g.ldc);
+        return hipGetLastError();
+    }
+    const bool ta = g.opa != OP_N, tb = g.opb != OP_N;
+    if (!ta && !tb) return launch_gemmtr_real<false, false>(g, upper, nb, s);
+    if (ta && !tb)  return launch_gemmtr_real<true, false>(g, upper, nb, s);
+    if (!ta && tb)  return launch_gemmtr_real<false, true>(g, upper, nb, s);
+    return launch_gemmtr_real<true, true>(g, upper, nb, s);
+}
+
+hipError_t plain_gemmtr_cplx(const GemmArgsZ &g, bool upper, int64_t nb, hipStream_t s)
+{
+    if (g.n <= 0) return hipSuccess;
+    const bool a0 = g.alpha.re.hi == 0.0 && g.alpha.re.lo == 0.0 && g.alpha.im.hi == 0.0 &&
+                    g.alpha.im.lo == 0.0;
+    if (g.k == 0 || a0) {
+        if (g.beta_mode == BETA_ONE) return hipSuccess;
+        ScopedTimer tm(KC_BETA_ONLY, s);
+        const int64_t cnt = g.n * g.n;
+        hipLaunchKernelGGL((k_gemmtr_beta_only<zddv>), dim3((unsigned)((cnt + 255) / 256)),
+                           dim3(256), 0, s, g.n, upper ? 1 : 0, g.beta, g.beta_mode, g.C, g.ldc);
+        return hipGetLastError();
+    }
+    switch (g.opa) {
+    case OP_N: return launch_gemmtr_cplx_b<OP_N>(g, upper, nb, s);
+    case OP_T: return launch_gemmtr_cplx_b<OP_T>(g, upper, nb, s);
+    default:   return launch_gemmtr_cplx_b<OP_C>(g, upper, nb, s);
+    }
 }
 
 hipError_t plain_gemm_cplx(const GemmArgsZ &g, hipStream_t s)

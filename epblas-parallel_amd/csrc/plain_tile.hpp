@@ -98,14 +98,43 @@ struct SlabB {
     }
 };
 
+// Triangular-output GEMM (gemmtr) masks.  The n x n C is walked in column blocks of nb;
+// for column j in block [jc, jc + nb) the UPLO triangle splits into the block's diagonal
+// triangle (rows jc..j for 'U', j..jc+nb-1 for 'L', updated by the scalar kernel) and the
+// rectangle beyond it (rows i < jc for 'U', i >= jc + nb for 'L'), which takes the GEMM's
+// own per-element sequence (mgemmtr_serial.cpp:117-166).  TRI_NONE is the plain GEMM.
+enum { TRI_NONE = 0, TRI_UPPER = 1, TRI_LOWER = 2 };
+
+template <int TRI> DDI bool tri_rect(int64_t i, int64_t j, int64_t nb)
+{
+    if constexpr (TRI == TRI_UPPER) return i < (j / nb) * nb;
+    else if constexpr (TRI == TRI_LOWER) return i >= (j / nb) * nb + nb;
+    else return true;
+}
+
+// Whether the BM x BN tile at (i0, j0) of an n x n C holds a rectangle element.
+template <int TRI> DDI bool tri_tile_live(int64_t i0, int64_t j0, int BM, int BN, int64_t n, int64_t nb)
+{
+    if constexpr (TRI == TRI_UPPER) {
+        const int64_t jl = (j0 + BN < n ? j0 + BN : n) - 1;
+        return i0 < (jl / nb) * nb;
+    } else if constexpr (TRI == TRI_LOWER) {
+        const int64_t il = (i0 + BM < n ? i0 + BM : n) - 1;
+        return il >= (j0 / nb) * nb + nb;
+    } else {
+        return true;
+    }
+}
+
 // One BM x BN tile of a real plain GEMM (the whole K loop with the reference's
 // 128-deep chunk flushes); As/Bs are the workgroup's LDS slabs.  Ends with a barrier, so the
-// caller may reuse the LDS for another tile right away.
-template <int TM, int TN, int BK, bool TA, bool TB>
+// caller may reuse the LDS for another tile right away.  With TRI, only the gemmtr rectangle
+// elements of the tile are read and written.
+template <int TM, int TN, int BK, bool TA, bool TB, int TRI = TRI_NONE>
 DDI void mgemm_plain_tile(int64_t m, int64_t n, int64_t k, ddv alpha, ddv beta, int beta_mode,
                           const ddv *__restrict__ A, int64_t lda, const ddv *__restrict__ B,
                           int64_t ldb, ddv *__restrict__ C, int64_t ldc, int64_t i0, int64_t j0,
-                          ddv (&As)[BK][16 * TM], ddv (&Bs)[BK][16 * TN])
+                          ddv (&As)[BK][16 * TM], ddv (&Bs)[BK][16 * TN], int64_t tri_nb = 0)
 {
     constexpr int BM = 16 * TM, BN = 16 * TN;
     static_assert(KCHUNK % BK == 0, "BK must divide the flush period");
@@ -119,6 +148,18 @@ DDI void mgemm_plain_tile(int64_t m, int64_t n, int64_t k, ddv alpha, ddv beta, 
     for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[i][j] = ddv{0.0, 0.0};
+
+    // Which of the lane's TM x TN elements are written: in range and, for gemmtr, in the
+    // rectangle.  Fixed for the tile, so it is one bit mask (one register) for the K loop.
+    static_assert(TM * TN <= 32, "store mask width");
+    uint32_t keep = 0;
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const int64_t gi = i0 + tx + 16 * i, gj = j0 + ty + 16 * j;
+            if (gi < m && gj < n && tri_rect<TRI>(gi, gj, tri_nb)) keep |= 1u << (i * TN + j);
+        }
 
     const int64_t ntiles = (k + BK - 1) / BK;
     sa.load(A, lda, m, k, i0, 0, tid);
@@ -171,7 +212,7 @@ DDI void mgemm_plain_tile(int64_t m, int64_t n, int64_t k, ddv alpha, ddv beta, 
 #pragma unroll
                 for (int j = 0; j < TN; ++j) {
                     const int64_t gj = j0 + ty + 16 * j;
-                    if (gi < m && gj < n) {
+                    if ((keep >> (i * TN + j)) & 1u) {
                         ddv *cp = &C[gj * ldc + gi];
                         ddv c = *cp;
                         if (first) c = dd_beta_prepass(c, beta, beta_mode);

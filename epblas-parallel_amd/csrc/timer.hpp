@@ -16,6 +16,7 @@ enum KernelClass {
     KC_BATCH_OZ_SPLIT,
     KC_BATCH_OZ_GEMM,
     KC_BATCH_OZ_CRT,
+    KC_GEMMTR_DIAG,
     KC_COUNT
 };
 

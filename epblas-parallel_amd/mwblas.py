@@ -52,6 +52,12 @@ def lib():
         f.restype = None
         f.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ip, ip, ip, _vp, _vp, ip, _vp, ip, _vp,
                       _vp, ip]
+    for name, ip in (("mgemmtr_", _i32p), ("mgemmtr_64_", _i64p), ("wgemmtr_", _i32p),
+                     ("wgemmtr_64_", _i64p)):
+        f = getattr(L, name)
+        f.restype = None
+        f.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ip, ip, _vp, _vp, ip,
+                      _vp, ip, _vp, _vp, ip]
     for name in ("mgemm_serial", "wgemm_serial"):
         f = getattr(L, name)
         f.restype = None
@@ -170,6 +176,71 @@ def wgemm_serial(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
     lib().wgemm_serial(transa.encode(), transb.encode(), m, n, k, _ptr(al), _ptr(a), lda,
                        _ptr(b), ldb, _ptr(be), _ptr(c), ldc)
     _check()
+
+
+def _gemmtr(name, width, ip, uplo, transa, transb, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    L = lib()
+    ct = ctypes.c_int32 if ip is _i32p else ctypes.c_int64
+    al, be = _scalar(alpha, width), _scalar(beta, width)
+    args = [ct(v) for v in (n, k, lda, ldb, ldc)]
+    clear_error()
+    getattr(L, name)(uplo.encode(), transa.encode(), transb.encode(), ctypes.byref(args[0]),
+                     ctypes.byref(args[1]), _ptr(al), _ptr(a), ctypes.byref(args[2]), _ptr(b),
+                     ctypes.byref(args[3]), _ptr(be), _ptr(c), ctypes.byref(args[4]))
+    _check()
+
+
+def mgemmtr(uplo, transa, transb, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    """C := alpha*op(A)*op(B) + beta*C on the UPLO triangle of the n x n C (drop-in
+    ``mgemmtr_``)."""
+    _gemmtr("mgemmtr_", 2, _i32p, uplo, transa, transb, n, k, alpha, a, lda, b, ldb, beta, c, ldc)
+
+
+def mgemmtr_64(uplo, transa, transb, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    _gemmtr("mgemmtr_64_", 2, _i64p, uplo, transa, transb, n, k, alpha, a, lda, b, ldb, beta, c,
+            ldc)
+
+
+def wgemmtr(uplo, transa, transb, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    """Complex triangular-output GEMM (drop-in ``wgemmtr_``)."""
+    _gemmtr("wgemmtr_", 4, _i32p, uplo, transa, transb, n, k, alpha, a, lda, b, ldb, beta, c, ldc)
+
+
+def wgemmtr_64(uplo, transa, transb, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    _gemmtr("wgemmtr_64_", 4, _i64p, uplo, transa, transb, n, k, alpha, a, lda, b, ldb, beta, c,
+            ldc)
+
+
+def gemmtr_serial(cplx, uplo, transa, transb, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    """The by-value cores ``mgemmtr_serial`` / ``wgemmtr_serial``."""
+    w = 4 if cplx else 2
+    al, be = _scalar(alpha, w), _scalar(beta, w)
+    f = lib().wgemmtr_serial if cplx else lib().mgemmtr_serial
+    f.restype = None
+    f.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_char, ctypes.c_int64, ctypes.c_int64,
+                  _vp, _vp, ctypes.c_int64, _vp, ctypes.c_int64, _vp, _vp, ctypes.c_int64]
+    clear_error()
+    f(uplo.encode(), transa.encode(), transb.encode(), n, k, _ptr(al), _ptr(a), lda, _ptr(b),
+      ldb, _ptr(be), _ptr(c), ldc)
+    _check()
+
+
+def gemmtr_device(cplx, uplo, transa, transb, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc,
+                  stream=None):
+    """Device-pointer triangular-output GEMM (asynchronous on ``stream``)."""
+    L = lib()
+    w = 4 if cplx else 2
+    al, be = _scalar(alpha, w), _scalar(beta, w)
+    f = L.mwgemm_wgemmtr_device if cplx else L.mwgemm_mgemmtr_device
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_char, ctypes.c_int64, ctypes.c_int64,
+                  _vp, _vp, ctypes.c_int64, _vp, ctypes.c_int64, _vp, _vp, ctypes.c_int64, _vp]
+    clear_error()
+    rc = f(uplo.encode(), transa.encode(), transb.encode(), n, k, _ptr(al), _ptr(dA), lda,
+           _ptr(dB), ldb, _ptr(be), _ptr(dC), ldc,
+           None if stream is None else ctypes.c_void_p(stream))
+    if rc != 0:
+        _check(rc)
 
 
 def gemm_device(cplx, transa, transb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc,

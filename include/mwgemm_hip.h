@@ -80,6 +80,50 @@ MWGEMM_API void wgemm_serial(char transa, char transb, ptrdiff_t m, ptrdiff_t n,
                   const mw_complex64x2 *b, ptrdiff_t ldb, const mw_complex64x2 *beta,
                   mw_complex64x2 *c, ptrdiff_t ldc);
 
+/* Triangular-output GEMM: C := alpha*op(A)*op(B) + beta*C on the UPLO triangle ('U' or
+ * 'L') of the n x n C only; op(A) is n x k, op(B) is k x n.  Replaces mgemmtr_ =
+ * EPBLAS_FACADE_GEMMTR(mgemmtr, TR), src/epblas-parallel/multifloats/mgemmtr_parallel.cpp:82
+ * (macro common/epblas_facade.h:242-248), whose numerics live in mgemmtr_serial.cpp:117-200.
+ * Bit-for-bit the reference's result: columns are walked in blocks of nb (MGEMMTR_NB,
+ * default 8), the nb x nb diagonal triangles take the scalar faithful sequence, the rest
+ * the GEMM's per-element sequence.  Uplo letters other than U/L are rejected. */
+MWGEMM_API void mgemmtr_(const char *uplo, const char *transa, const char *transb, const int *n,
+              const int *k, const mw_float64x2 *alpha, const mw_float64x2 *a, const int *lda,
+              const mw_float64x2 *b, const int *ldb, const mw_float64x2 *beta, mw_float64x2 *c,
+              const int *ldc);
+
+/* ILP64 twin of mgemmtr_ (epblas_facade.h:63-65). */
+MWGEMM_API void mgemmtr_64_(const char *uplo, const char *transa, const char *transb,
+                 const int64_t *n, const int64_t *k, const mw_float64x2 *alpha,
+                 const mw_float64x2 *a, const int64_t *lda, const mw_float64x2 *b,
+                 const int64_t *ldb, const mw_float64x2 *beta, mw_float64x2 *c,
+                 const int64_t *ldc);
+
+/* Complex twin (op letters N/T/C, WGEMMTR_NB).  Replaces wgemmtr_ =
+ * EPBLAS_FACADE_GEMMTR(wgemmtr, TC), wgemmtr_parallel.cpp:85; numerics wgemmtr_serial.cpp. */
+MWGEMM_API void wgemmtr_(const char *uplo, const char *transa, const char *transb, const int *n,
+              const int *k, const mw_complex64x2 *alpha, const mw_complex64x2 *a,
+              const int *lda, const mw_complex64x2 *b, const int *ldb,
+              const mw_complex64x2 *beta, mw_complex64x2 *c, const int *ldc);
+
+/* ILP64 twin of wgemmtr_. */
+MWGEMM_API void wgemmtr_64_(const char *uplo, const char *transa, const char *transb,
+                 const int64_t *n, const int64_t *k, const mw_complex64x2 *alpha,
+                 const mw_complex64x2 *a, const int64_t *lda, const mw_complex64x2 *b,
+                 const int64_t *ldb, const mw_complex64x2 *beta, mw_complex64x2 *c,
+                 const int64_t *ldc);
+
+/* By-value cores.  Replace mgemmtr_serial (mgemmtr_kernel.h:56-63) and wgemmtr_serial
+ * (wgemmtr_kernel.h:52-59). */
+MWGEMM_API void mgemmtr_serial(char uplo, char transa, char transb, ptrdiff_t n, ptrdiff_t k,
+                    const mw_float64x2 *alpha, const mw_float64x2 *a, ptrdiff_t lda,
+                    const mw_float64x2 *b, ptrdiff_t ldb, const mw_float64x2 *beta,
+                    mw_float64x2 *c, ptrdiff_t ldc);
+MWGEMM_API void wgemmtr_serial(char uplo, char transa, char transb, ptrdiff_t n, ptrdiff_t k,
+                    const mw_complex64x2 *alpha, const mw_complex64x2 *a, ptrdiff_t lda,
+                    const mw_complex64x2 *b, ptrdiff_t ldb, const mw_complex64x2 *beta,
+                    mw_complex64x2 *c, ptrdiff_t ldc);
+
 /* ---------------------------------------------------------------------------------
  * Library extensions (no counterpart in the reference).
  * ------------------------------------------------------------------------------- */
@@ -112,6 +156,16 @@ MWGEMM_API int mwgemm_wgemm_device(char transa, char transb, int64_t m, int64_t 
                         const mw_complex64x2 *alpha, const mw_complex64x2 *dA, int64_t lda,
                         const mw_complex64x2 *dB, int64_t ldb, const mw_complex64x2 *beta,
                         mw_complex64x2 *dC, int64_t ldc, int path, void *stream);
+
+/* Device-pointer triangular-output GEMMs (plain path, asynchronous, status code). */
+MWGEMM_API int mwgemm_mgemmtr_device(char uplo, char transa, char transb, int64_t n, int64_t k,
+                          const mw_float64x2 *alpha, const mw_float64x2 *dA, int64_t lda,
+                          const mw_float64x2 *dB, int64_t ldb, const mw_float64x2 *beta,
+                          mw_float64x2 *dC, int64_t ldc, void *stream);
+MWGEMM_API int mwgemm_wgemmtr_device(char uplo, char transa, char transb, int64_t n, int64_t k,
+                          const mw_complex64x2 *alpha, const mw_complex64x2 *dA, int64_t lda,
+                          const mw_complex64x2 *dB, int64_t ldb, const mw_complex64x2 *beta,
+                          mw_complex64x2 *dC, int64_t ldc, void *stream);
 
 /* Batched GEMMs: count independent real DD problems C_p := alpha_p op(A_p) op(B_p) +
  * beta_p C_p (one op pair for the whole batch; per-problem sizes, leading dimensions,

@@ -481,3 +481,187 @@ int port_max_threads(void)
 #endif
 }
 int port_has_avx2(void) { return have_avx2_fma(); }
+
+/* ---------------------------------------------------------------------------------
+ * Triangular-output GEMM: C := alpha*op(A)*op(B) + beta*C on the UPLO triangle of the
+ * n x n C only (mgemmtr_ / wgemmtr_).
+ *
+ * Reference: src/epblas-parallel/multifloats/mgemmtr_serial.cpp:117-200 (block cores,
+ * serial entry) driven by mgemmtr_parallel.cpp:27-77; complex wgemmtr_serial.cpp.  Columns
+ * are walked in blocks of nb (8 unless MGEMMTR_NB / WGEMMTR_NB say otherwise,
+ * mgemmtr_serial.cpp:104-112).  Per block [jc, jc+jb):
+ *   - the triangle slice of each column is beta-scaled (0 -> exact zero, 1 -> kept,
+ *     else C*beta with the faithful product)                    :123-130
+ *   - the jb x jb diagonal triangle is updated with the scalar faithful operators,
+ *     op(A) = 'N': per l, t = alpha*b_lj (skipped when t == 0), c_ij += t*a_il;
+ *     otherwise:   s = sum_l a_li*b_lj (faithful, ascending l), c_ij += alpha*s   :48-95
+ *   - the off-diagonal rectangle (rows above the block for 'U', below it for 'L') goes
+ *     through the serial GEMM with beta = 1                     :136-166
+ * The blocks touch disjoint columns; the reference's OpenMP split over blocks gives the
+ * same bits as the serial sweep, and so does this one.
+ * ------------------------------------------------------------------------------- */
+static void rtr_diag(ptrdiff_t jc, ptrdiff_t jb, ptrdiff_t k, dd alpha, const dd *a,
+                     ptrdiff_t lda, const dd *b, ptrdiff_t ldb, dd *c, ptrdiff_t ldc, int upper,
+                     char ta, char tb)
+{
+    for (ptrdiff_t j = jc; j < jc + jb; ++j) {
+        const ptrdiff_t is = upper ? jc : j, ie = upper ? j + 1 : jc + jb;
+        dd *cj = c + (size_t)j * ldc;
+        if (ta == 'N') {
+            for (ptrdiff_t l = 0; l < k; ++l) {
+                const dd bl = (tb == 'N') ? b[(size_t)j * ldb + l] : b[(size_t)l * ldb + j];
+                const dd t = dd_mul_exact(alpha, bl);
+                if (dd_is0(t)) continue;
+                const dd *al = a + (size_t)l * lda;
+                for (ptrdiff_t i = is; i < ie; ++i) cj[i] = dd_add_exact(cj[i], dd_mul_exact(t, al[i]));
+            }
+        } else {
+            for (ptrdiff_t i = is; i < ie; ++i) {
+                dd s = { 0.0, 0.0 };
+                for (ptrdiff_t l = 0; l < k; ++l) {
+                    const dd bl = (tb == 'N') ? b[(size_t)j * ldb + l] : b[(size_t)l * ldb + j];
+                    s = dd_add_exact(s, dd_mul_exact(a[(size_t)i * lda + l], bl));
+                }
+                cj[i] = dd_add_exact(cj[i], dd_mul_exact(alpha, s));
+            }
+        }
+    }
+}
+
+static void rtr_beta_cols(ptrdiff_t j0, ptrdiff_t j1, ptrdiff_t n, int upper, dd beta, dd *c,
+                          ptrdiff_t ldc)
+{
+    for (ptrdiff_t j = j0; j < j1; ++j) {
+        const ptrdiff_t is = upper ? 0 : j, ie = upper ? j + 1 : n;
+        dd *cj = c + (size_t)j * ldc;
+        for (ptrdiff_t i = is; i < ie; ++i) {
+            if (dd_is0(beta)) { cj[i].hi = 0.0; cj[i].lo = 0.0; }
+            else if (!dd_is1(beta)) cj[i] = dd_mul_exact(cj[i], beta);
+        }
+    }
+}
+
+/* nb <= 0 selects the reference default of 8.  Returns 0, or -1 when a GEMM pack buffer
+ * could not be allocated. */
+int port_mgemmtr(char uplo, char transa, char transb, ptrdiff_t n, ptrdiff_t k,
+                 const dd *alpha_, const dd *a, ptrdiff_t lda, const dd *b, ptrdiff_t ldb,
+                 const dd *beta_, dd *c, ptrdiff_t ldc, ptrdiff_t nb, int nthreads)
+{
+    const dd alpha = *alpha_, beta = *beta_, one = { 1.0, 0.0 };
+    const int upper = toupper((unsigned char)uplo) == 'U';
+    const char ta = trans_real(transa), tb = trans_real(transb);
+    if (n <= 0) return 0;
+    if (dd_is0(alpha) || k == 0) {
+        if (dd_is1(beta)) return 0;
+        rtr_beta_cols(0, n, n, upper, beta, c, ldc);
+        return 0;
+    }
+    if (nb <= 0) nb = 8;
+    int failed = 0;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1) reduction(|: failed)
+#endif
+    for (ptrdiff_t jc = 0; jc < n; jc += nb) {
+        const ptrdiff_t jb = (n - jc < nb) ? n - jc : nb;
+        rtr_beta_cols(jc, jc + jb, n, upper, beta, c, ldc);
+        rtr_diag(jc, jb, k, alpha, a, lda, b, ldb, c, ldc, upper, ta, tb);
+        const dd *bblk = (tb == 'N') ? b + (size_t)jc * ldb : b + jc;
+        if (upper) {
+            if (jc > 0 && port_mgemm(ta, tb, jc, jb, k, &alpha, a, lda, bblk, ldb, &one,
+                                     c + (size_t)jc * ldc, ldc, 1) != 0)
+                failed = 1;
+        } else {
+            const ptrdiff_t r0 = jc + jb, trailing = n - r0;
+            const dd *ablk = (ta == 'N') ? a + r0 : a + (size_t)r0 * lda;
+            if (trailing > 0 && port_mgemm(ta, tb, trailing, jb, k, &alpha, ablk, lda, bblk, ldb,
+                                           &one, c + (size_t)jc * ldc + r0, ldc, 1) != 0)
+                failed = 1;
+        }
+    }
+    return failed ? -1 : 0;
+}
+
+static inline zdd zop(zdd x, int conj) { return conj ? zdd_conj(x) : x; }
+
+static void ztr_diag(ptrdiff_t jc, ptrdiff_t jb, ptrdiff_t k, zdd alpha, const zdd *a,
+                     ptrdiff_t lda, const zdd *b, ptrdiff_t ldb, zdd *c, ptrdiff_t ldc, int upper,
+                     char ta, char tb)
+{
+    const int trans_a = ta != 'N', trans_b = tb != 'N', conj_a = ta == 'C', conj_b = tb == 'C';
+    for (ptrdiff_t j = jc; j < jc + jb; ++j) {
+        const ptrdiff_t is = upper ? jc : j, ie = upper ? j + 1 : jc + jb;
+        zdd *cj = c + (size_t)j * ldc;
+        if (!trans_a) {
+            for (ptrdiff_t l = 0; l < k; ++l) {
+                const zdd bl = trans_b ? zop(b[(size_t)l * ldb + j], conj_b) : b[(size_t)j * ldb + l];
+                const zdd t = zdd_mul_exact(alpha, bl);
+                if (zdd_is0(t)) continue;
+                const zdd *al = a + (size_t)l * lda;
+                for (ptrdiff_t i = is; i < ie; ++i) cj[i] = zdd_add_exact(cj[i], zdd_mul_exact(t, al[i]));
+            }
+        } else {
+            for (ptrdiff_t i = is; i < ie; ++i) {
+                zdd s = { { 0.0, 0.0 }, { 0.0, 0.0 } };
+                for (ptrdiff_t l = 0; l < k; ++l) {
+                    const zdd al = zop(a[(size_t)i * lda + l], conj_a);
+                    const zdd bl = trans_b ? zop(b[(size_t)l * ldb + j], conj_b) : b[(size_t)j * ldb + l];
+                    s = zdd_add_exact(s, zdd_mul_exact(al, bl));
+                }
+                cj[i] = zdd_add_exact(cj[i], zdd_mul_exact(alpha, s));
+            }
+        }
+    }
+}
+
+static void ztr_beta_cols(ptrdiff_t j0, ptrdiff_t j1, ptrdiff_t n, int upper, zdd beta, zdd *c,
+                          ptrdiff_t ldc)
+{
+    for (ptrdiff_t j = j0; j < j1; ++j) {
+        const ptrdiff_t is = upper ? 0 : j, ie = upper ? j + 1 : n;
+        zdd *cj = c + (size_t)j * ldc;
+        for (ptrdiff_t i = is; i < ie; ++i) {
+            if (zdd_is0(beta)) memset(&cj[i], 0, sizeof(zdd));
+            else if (!zdd_is1(beta)) cj[i] = zdd_mul_exact(cj[i], beta);
+        }
+    }
+}
+
+int port_wgemmtr(char uplo, char transa, char transb, ptrdiff_t n, ptrdiff_t k,
+                 const zdd *alpha_, const zdd *a, ptrdiff_t lda, const zdd *b, ptrdiff_t ldb,
+                 const zdd *beta_, zdd *c, ptrdiff_t ldc, ptrdiff_t nb, int nthreads)
+{
+    const zdd alpha = *alpha_, beta = *beta_, one = { { 1.0, 0.0 }, { 0.0, 0.0 } };
+    const int upper = toupper((unsigned char)uplo) == 'U';
+    const char ta = trans_cplx(transa), tb = trans_cplx(transb);
+    if (n <= 0) return 0;
+    if (zdd_is0(alpha) || k == 0) {
+        if (zdd_is1(beta)) return 0;
+        ztr_beta_cols(0, n, n, upper, beta, c, ldc);
+        return 0;
+    }
+    if (nb <= 0) nb = 8;
+    int failed = 0;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1) reduction(|: failed)
+#endif
+    for (ptrdiff_t jc = 0; jc < n; jc += nb) {
+        const ptrdiff_t jb = (n - jc < nb) ? n - jc : nb;
+        ztr_beta_cols(jc, jc + jb, n, upper, beta, c, ldc);
+        ztr_diag(jc, jb, k, alpha, a, lda, b, ldb, c, ldc, upper, ta, tb);
+        const zdd *bblk = (tb == 'N') ? b + (size_t)jc * ldb : b + jc;
+        if (upper) {
+            if (jc > 0 && port_wgemm(ta, tb, jc, jb, k, &alpha, a, lda, bblk, ldb, &one,
+                                     c + (size_t)jc * ldc, ldc, 1) != 0)
+                failed = 1;
+        } else {
+            const ptrdiff_t r0 = jc + jb, trailing = n - r0;
+            const zdd *ablk = (ta == 'N') ? a + r0 : a + (size_t)r0 * lda;
+            if (trailing > 0 && port_wgemm(ta, tb, trailing, jb, k, &alpha, ablk, lda, bblk, ldb,
+                                           &one, c + (size_t)jc * ldc + r0, ldc, 1) != 0)
+                failed = 1;
+        }
+    }
+    return failed ? -1 : 0;
+}

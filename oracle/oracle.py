@@ -44,6 +44,11 @@ def lib():
             f.restype = ctypes.c_int
             f.argtypes = [c, c, _c_i64, _c_i64, _c_i64, _vp, _vp, _c_i64, _vp, _c_i64, _vp,
                           _vp, _c_i64, ctypes.c_int]
+        for name in ("port_mgemmtr", "port_wgemmtr"):
+            f = getattr(_LIB, name)
+            f.restype = ctypes.c_int
+            f.argtypes = [c, c, c, _c_i64, _c_i64, _vp, _vp, _c_i64, _vp, _c_i64, _vp, _vp,
+                          _c_i64, _c_i64, ctypes.c_int]
         for name in ("q128_mgemm_entries", "q128_wgemm_entries"):
             f = getattr(_LIB, name)
             f.restype = None
@@ -111,6 +116,22 @@ def port_wgemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nthreads=0)
                           _p(beta), _p(C), ldc, nthreads)
     if rc != 0:
         raise MemoryError("port_wgemm: pack buffer allocation failed")
+
+
+def port_mgemmtr(uplo, ta, tb, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nb=8, nthreads=0):
+    """CPU restatement of the reference mgemmtr_ (UPLO triangle of C, in place)."""
+    rc = lib().port_mgemmtr(_ch(uplo), _ch(ta), _ch(tb), n, k, _p(alpha), _p(A), lda, _p(B),
+                            ldb, _p(beta), _p(C), ldc, nb, nthreads)
+    if rc != 0:
+        raise MemoryError("port_mgemmtr: pack buffer allocation failed")
+
+
+def port_wgemmtr(uplo, ta, tb, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nb=8, nthreads=0):
+    """CPU restatement of the reference wgemmtr_ (UPLO triangle of C, in place)."""
+    rc = lib().port_wgemmtr(_ch(uplo), _ch(ta), _ch(tb), n, k, _p(alpha), _p(A), lda, _p(B),
+                            ldb, _p(beta), _p(C), ldc, nb, nthreads)
+    if rc != 0:
+        raise MemoryError("port_wgemmtr: pack buffer allocation failed")
 
 
 def _entries(ent):

@@ -24,7 +24,9 @@ def header_symbols():
 
 def test_header_declares_the_dropin_symbols():
     syms = header_symbols()
-    for s in ("mgemm_", "mgemm_64_", "wgemm_", "wgemm_64_", "mgemm_serial", "wgemm_serial"):
+    for s in ("mgemm_", "mgemm_64_", "wgemm_", "wgemm_64_", "mgemm_serial", "wgemm_serial",
+              "mgemmtr_", "mgemmtr_64_", "wgemmtr_", "wgemmtr_64_", "mgemmtr_serial",
+              "wgemmtr_serial"):
         assert s in syms
 
 
@@ -56,6 +58,15 @@ def test_invalid_arguments_fail_loudly():
         mwblas.mgemm("N", "N", 4, 4, 4, [1, 0], A, 3, A, 4, [0, 0], C, 4)
     with pytest.raises(mwblas.MwgemmError, match="trans"):
         mwblas.wgemm("N", "Q", 4, 4, 4, [1, 0, 0, 0], A, 4, A, 4, [0, 0, 0, 0], C, 4)
+    # triangular-output GEMM: uplo must be U or L (the reference treats any other letter
+    # as L); trans letters and leading dimensions as for the GEMM
+    with pytest.raises(mwblas.MwgemmError, match="uplo"):
+        mwblas.mgemmtr("X", "N", "N", 4, 4, [1, 0], A, 4, A, 4, [0, 0], C, 4)
+    with pytest.raises(mwblas.MwgemmError, match="trans"):
+        mwblas.mgemmtr("U", "N", "Z", 4, 4, [1, 0], A, 4, A, 4, [0, 0], C, 4)
+    with pytest.raises(mwblas.MwgemmError, match="leading dimension"):
+        mwblas.wgemmtr("L", "N", "N", 4, 4, [1, 0, 0, 0], A, 4, A, 4, [0, 0, 0, 0], C, 3)
+    mwblas.mgemmtr("U", "N", "N", 0, 4, [1, 0], A, 4, A, 4, [0, 0], C, 4)   # n == 0: no-op
     assert np.all(C == 0.0)
 
 
