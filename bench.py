@@ -12,11 +12,12 @@ sample of the same problem.
 
 Further objects on the same line (each with its own workload): "batched" (2000 small GEMMs
 in one captured launch per path), "complex" (4096^3 with op(A)='C', both paths), "gemmtr"
-(the triangular-output GEMM, 4096 x 4096 upper triangle, k = 4096) and "host_boundary" (the
-drop-in mgemm_ with host operands, PCIe transfers inside the timing).
+(the triangular-output GEMM, 4096 x 4096 upper triangle, k = 4096), "gemv" (16384 x 16384
+matrix-vector products, HBM-bound) and "host_boundary" (the drop-in mgemm_ with host
+operands, PCIe transfers inside the timing).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--size 8192]
-                    [--no-batched] [--no-complex] [--no-gemmtr] [--no-host]
+                    [--no-batched] [--no-complex] [--no-gemmtr] [--no-gemv] [--no-host]
 """
 import argparse
 import json
@@ -249,6 +250,44 @@ def gemmtr_bench(dev, n=4096, k=4096, reps=3, seed=41):
             "kernels_ms_per_call": {kn: round(v[0] / reps, 3) for kn, v in kt.items()}}
 
 
+def gemv_bench(dev, n=16384, reps=5, seed=43):
+    """Matrix-vector products (mgemv_ semantics, bit-exact, 16 row bands for op 'N'): an
+    n x n real DD A in HBM, ops 'N' and 'T'.  HBM-bound: A (16 B per entry) is read once, so
+    the rate is reported as A-bytes / t against the 8 TB/s HBM peak."""
+    import numpy as np
+    import torch
+    import mwblas
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    A = gen_uniform(n, n, gen, dev)
+    x = gen_uniform(n, 1, gen, dev)
+    y0 = gen_uniform(n, 1, gen, dev)
+    rng = np.random.default_rng(seed)
+    alpha, beta = random_scalar(rng, 1), random_scalar(rng, 1)
+    stream = torch.cuda.current_stream().cuda_stream
+    out = {"workload": f"mgemv {n}x{n}, random alpha/beta, unit strides", "bytes_A": 16 * n * n}
+    for trans in ("N", "T"):
+        y = y0.clone()
+        mwblas.gemv_device(False, trans, n, n, alpha, A, n, x, 1, beta, y, 1, nbands=16,
+                           stream=stream)
+        torch.cuda.synchronize()
+        t = 0.0
+        for _ in range(reps):
+            y.copy_(y0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            mwblas.gemv_device(False, trans, n, n, alpha, A, n, x, 1, beta, y, 1, nbands=16,
+                               stream=stream)
+            torch.cuda.synchronize()
+            t += time.perf_counter() - t0
+        t /= reps
+        out[trans] = {"ms_per_call": round(t * 1e3, 4),
+                      "GBps": round(16 * n * n / t / 1e9, 1),
+                      "hbm_frac": round(16 * n * n / t / 1e9 / 8000.0, 4),
+                      "dd_gflops": round(2.0 * n * n / t / 1e9, 2)}
+    return out
+
+
 def host_boundary_bench(size=4096, reps=3, seed=77):
     """The drop-in mgemm_ with host (pageable) operands: H2D of A and B, the GEMM on the
     exact-product path, D2H of C, all inside the timed call (PCIe-inclusive).  Real NN at
@@ -298,6 +337,7 @@ def main():
     ap.add_argument("--no-complex", action="store_true")
     ap.add_argument("--no-host", action="store_true")
     ap.add_argument("--no-gemmtr", action="store_true")
+    ap.add_argument("--no-gemv", action="store_true")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="first run one device copy of this many MiB (the known-size kernel "
                          "that scales hardware byte counters)")
@@ -426,6 +466,8 @@ def main():
         result["complex"] = complex_bench(dev)
     if not args.no_gemmtr:
         result["gemmtr"] = gemmtr_bench(dev)
+    if not args.no_gemv:
+        result["gemv"] = gemv_bench(dev)
     if not args.no_host:
         result["host_boundary"] = host_boundary_bench()
     if not args.no_cpu:

@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libmwgemm_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["api.cpp", "host_copy.cpp", "plain_gemm.hip", "ozaki.hip", "batched.hip",
+SOURCES = ["api.cpp", "host_copy.cpp", "plain_gemm.hip", "gemv.hip", "ozaki.hip", "batched.hip",
            "autotune.cpp"]
 
 CFLAGS = [

@@ -25,6 +25,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/mwgemm_hip.h"
@@ -570,6 +571,131 @@ int wgemmtr_host(char uplo, char transa, char transb, int64_t n, int64_t k,
                              [uplo](GemmArgsZ gg, hipStream_t s) { return gemmtr_cplx_device(gg, uplo, s); });
 }
 
+// ----------------------------------------------------------------------------------
+// Matrix-vector products.
+// ----------------------------------------------------------------------------------
+// Row bands of the real op-'N' product: the reference splits the rows over its OpenMP team
+// when m >= 64 (mgemv.cpp:95-100, at most 256 threads), and the rows after the last whole
+// 4-block of each band take the faithful operations, so the team size is part of the bit
+// pattern.  MWGEMV_THREADS names it; else OMP_NUM_THREADS (its first entry); else the
+// host's hardware thread count, which is what the reference's OpenMP default would use.
+int64_t gemv_bands(int64_t m)
+{
+    static const int64_t team = [] {
+        for (const char *name : {"MWGEMV_THREADS", "OMP_NUM_THREADS"}) {
+            if (const char *e = getenv(name)) {
+                const long v = atol(e);
+                if (v > 0) return (int64_t)v;
+            }
+        }
+        const unsigned hw = std::thread::hardware_concurrency();
+        return (int64_t)(hw ? hw : 1);
+    }();
+    if (m < 64) return 1;
+    return team < 256 ? team : 256;
+}
+
+int gemv_check(const char *name, int op, int64_t m, int64_t n, int64_t lda, int64_t incx,
+               int64_t incy)
+{
+    if (op < 0) return record_error(-2, "%s: invalid trans letter", name);
+    if (m < 0 || n < 0) return record_error(-3, "%s: negative dimension", name);
+    if (lda < (m > 1 ? m : 1))
+        return record_error(-3, "%s: leading dimension too small (lda=%lld)", name, (long long)lda);
+    if (incx == 0 || incy == 0) return record_error(-3, "%s: zero vector increment", name);
+    return 0;
+}
+
+int64_t span_of(int64_t len, int64_t inc) { return len > 0 ? 1 + (len - 1) * (inc < 0 ? -inc : inc) : 0; }
+
+// Host-pointer staging for the matrix-vector products: A's logical rows and the x / y spans
+// travel through the pinned chunks; y's span comes back whole (entries between strided
+// ones included, unchanged).
+template <typename T, typename Run>
+int staged_gemv(const T *A, int64_t lda, int64_t m, int64_t n, const T *x, int64_t xspan, T *y,
+                int64_t yspan, Run run)
+{
+    std::lock_guard<std::mutex> call_lk(g_call_mu);
+    int dev = 0;
+    if (int rc = current_device(&dev)) return rc;
+    DevState *st = dev_state(dev);
+    if (int rc = ensure_stream(st)) return rc;
+    hipStream_t s = st->stream;
+    const bool dA = is_device_ptr(A), dx = is_device_ptr(x), dy = is_device_ptr(y);
+    auto align = [](size_t v) { return (v + 255) & ~size_t(255); };
+    const int64_t a_elems = extent(m, n, lda);
+    size_t need = 0;
+    const size_t offA = need; if (!dA) need += align(a_elems * sizeof(T));
+    const size_t offx = need; if (!dx) need += align(xspan * sizeof(T));
+    const size_t offy = need; if (!dy) need += align(yspan * sizeof(T));
+    if (need)
+        if (int rc = ensure_stage(st, need)) return rc;
+    char *base = static_cast<char *>(st->stage);
+    const T *gA = A, *gx = x;
+    T *gy = y;
+    if (!dA) {
+        gA = reinterpret_cast<const T *>(base + offA);
+        if (int rc = upload_pinned(st, s, base + offA, lda * sizeof(T),
+                                   reinterpret_cast<const char *>(A), lda * sizeof(T),
+                                   m * sizeof(T), n)) return rc;
+    }
+    if (!dx) {
+        gx = reinterpret_cast<const T *>(base + offx);
+        if (int rc = upload_pinned(st, s, base + offx, xspan * sizeof(T),
+                                   reinterpret_cast<const char *>(x), xspan * sizeof(T),
+                                   xspan * sizeof(T), 1)) return rc;
+    }
+    if (!dy) {
+        gy = reinterpret_cast<T *>(base + offy);
+        if (int rc = upload_pinned(st, s, base + offy, yspan * sizeof(T),
+                                   reinterpret_cast<const char *>(y), yspan * sizeof(T),
+                                   yspan * sizeof(T), 1)) return rc;
+    }
+    MW_TRY(run(gA, gx, gy, s));
+    if (!dy)
+        if (int rc = download_pinned(st, s, reinterpret_cast<char *>(y), yspan * sizeof(T),
+                                     base + offy, yspan * sizeof(T), yspan * sizeof(T), 1))
+            return rc;
+    MW_TRY(hipStreamSynchronize(s));
+    return 0;
+}
+
+int mgemv_host(char trans, int64_t m, int64_t n, const mw_float64x2 *alpha,
+               const mw_float64x2 *a, int64_t lda, const mw_float64x2 *x, int64_t incx,
+               const mw_float64x2 *beta, mw_float64x2 *y, int64_t incy)
+{
+    const int op = op_real(trans);
+    if (int rc = gemv_check("mgemv", op, m, n, lda, incx, incy)) return rc;
+    if (m == 0 || n == 0) return 0;
+    const bool notrans = op == OP_N;
+    const ddv al{alpha->limbs[0], alpha->limbs[1]}, be{beta->limbs[0], beta->limbs[1]};
+    const int64_t nb = notrans ? gemv_bands(m) : 1;
+    return staged_gemv<ddv>(reinterpret_cast<const ddv *>(a), lda, m, n,
+                            reinterpret_cast<const ddv *>(x), span_of(notrans ? n : m, incx),
+                            reinterpret_cast<ddv *>(y), span_of(notrans ? m : n, incy),
+                            [&](const ddv *gA, const ddv *gx, ddv *gy, hipStream_t s) {
+                                return gemv_real(op, m, n, al, be, gA, lda, gx, incx, gy, incy, nb, s);
+                            });
+}
+
+int wgemv_host(char trans, int64_t m, int64_t n, const mw_complex64x2 *alpha,
+               const mw_complex64x2 *a, int64_t lda, const mw_complex64x2 *x, int64_t incx,
+               const mw_complex64x2 *beta, mw_complex64x2 *y, int64_t incy)
+{
+    const int op = op_cplx(trans);
+    if (int rc = gemv_check("wgemv", op, m, n, lda, incx, incy)) return rc;
+    if (m == 0 || n == 0) return 0;
+    const bool notrans = op == OP_N;
+    const zddv al{{alpha->re.limbs[0], alpha->re.limbs[1]}, {alpha->im.limbs[0], alpha->im.limbs[1]}};
+    const zddv be{{beta->re.limbs[0], beta->re.limbs[1]}, {beta->im.limbs[0], beta->im.limbs[1]}};
+    return staged_gemv<zddv>(reinterpret_cast<const zddv *>(a), lda, m, n,
+                             reinterpret_cast<const zddv *>(x), span_of(notrans ? n : m, incx),
+                             reinterpret_cast<zddv *>(y), span_of(notrans ? m : n, incy),
+                             [&](const zddv *gA, const zddv *gx, zddv *gy, hipStream_t s) {
+                                 return gemv_cplx(op, m, n, al, be, gA, lda, gx, incx, gy, incy, s);
+                             });
+}
+
 hipStream_t resolve_stream(void *stream, int *rc)
 {
     *rc = 0;
@@ -603,7 +729,7 @@ double g_timer_ms[KC_COUNT];
 int64_t g_timer_cnt[KC_COUNT];
 const char *kClassNames[KC_COUNT] = {
     "plain_real", "plain_cplx", "beta_only", "oz_exponent", "oz_split", "oz_gemm", "oz_crt",
-    "batch_plain", "batch_oz_split", "batch_oz_gemm", "batch_oz_crt", "gemmtr_diag"};
+    "batch_plain", "batch_oz_split", "batch_oz_gemm", "batch_oz_crt", "gemmtr_diag", "gemv"};
 
 void drain_slots()
 {
@@ -770,6 +896,71 @@ int mwgemm_wgemmtr_device(char uplo, char transa, char transb, int64_t n, int64_
     if (rc) return rc;
     return gemmtr_cplx_device(cplx_args(transa, transb, n, n, k, alpha, dA, lda, dB, ldb, beta,
                                         dC, ldc), uplo, s);
+}
+
+void mgemv_(const char *trans, const int *m, const int *n, const mw_float64x2 *alpha,
+            const mw_float64x2 *a, const int *lda, const mw_float64x2 *x, const int *incx,
+            const mw_float64x2 *beta, mw_float64x2 *y, const int *incy)
+{
+    finish(mgemv_host(*trans, *m, *n, alpha, a, *lda, x, *incx, beta, y, *incy));
+}
+
+void mgemv_64_(const char *trans, const int64_t *m, const int64_t *n, const mw_float64x2 *alpha,
+               const mw_float64x2 *a, const int64_t *lda, const mw_float64x2 *x,
+               const int64_t *incx, const mw_float64x2 *beta, mw_float64x2 *y,
+               const int64_t *incy)
+{
+    finish(mgemv_host(*trans, *m, *n, alpha, a, *lda, x, *incx, beta, y, *incy));
+}
+
+void wgemv_(const char *trans, const int *m, const int *n, const mw_complex64x2 *alpha,
+            const mw_complex64x2 *a, const int *lda, const mw_complex64x2 *x, const int *incx,
+            const mw_complex64x2 *beta, mw_complex64x2 *y, const int *incy)
+{
+    finish(wgemv_host(*trans, *m, *n, alpha, a, *lda, x, *incx, beta, y, *incy));
+}
+
+void wgemv_64_(const char *trans, const int64_t *m, const int64_t *n,
+               const mw_complex64x2 *alpha, const mw_complex64x2 *a, const int64_t *lda,
+               const mw_complex64x2 *x, const int64_t *incx, const mw_complex64x2 *beta,
+               mw_complex64x2 *y, const int64_t *incy)
+{
+    finish(wgemv_host(*trans, *m, *n, alpha, a, *lda, x, *incx, beta, y, *incy));
+}
+
+int mwgemm_mgemv_device(char trans, int64_t m, int64_t n, const mw_float64x2 *alpha,
+                        const mw_float64x2 *dA, int64_t lda, const mw_float64x2 *dx,
+                        int64_t incx, const mw_float64x2 *beta, mw_float64x2 *dy, int64_t incy,
+                        int64_t nbands, void *stream)
+{
+    const int op = op_real(trans);
+    if (int rc = gemv_check("mgemv", op, m, n, lda, incx, incy)) return rc;
+    int rc = 0;
+    hipStream_t s = resolve_stream(stream, &rc);
+    if (rc) return rc;
+    if (nbands <= 0) nbands = op == OP_N ? gemv_bands(m) : 1;
+    MW_TRY(gemv_real(op, m, n, ddv{alpha->limbs[0], alpha->limbs[1]},
+                     ddv{beta->limbs[0], beta->limbs[1]}, reinterpret_cast<const ddv *>(dA), lda,
+                     reinterpret_cast<const ddv *>(dx), incx, reinterpret_cast<ddv *>(dy), incy,
+                     nbands, s));
+    return 0;
+}
+
+int mwgemm_wgemv_device(char trans, int64_t m, int64_t n, const mw_complex64x2 *alpha,
+                        const mw_complex64x2 *dA, int64_t lda, const mw_complex64x2 *dx,
+                        int64_t incx, const mw_complex64x2 *beta, mw_complex64x2 *dy,
+                        int64_t incy, void *stream)
+{
+    const int op = op_cplx(trans);
+    if (int rc = gemv_check("wgemv", op, m, n, lda, incx, incy)) return rc;
+    int rc = 0;
+    hipStream_t s = resolve_stream(stream, &rc);
+    if (rc) return rc;
+    const zddv al{{alpha->re.limbs[0], alpha->re.limbs[1]}, {alpha->im.limbs[0], alpha->im.limbs[1]}};
+    const zddv be{{beta->re.limbs[0], beta->re.limbs[1]}, {beta->im.limbs[0], beta->im.limbs[1]}};
+    MW_TRY(gemv_cplx(op, m, n, al, be, reinterpret_cast<const zddv *>(dA), lda,
+                     reinterpret_cast<const zddv *>(dx), incx, reinterpret_cast<zddv *>(dy), incy, s));
+    return 0;
 }
 
 int mwgemm_set_path(int path)

@@ -52,6 +52,14 @@ hipError_t plain_gemm_real(const GemmArgsR &g, hipStream_t s);
 hipError_t plain_gemm_cplx(const GemmArgsZ &g, hipStream_t s);
 hipError_t plain_beta_only_real(const GemmArgsR &g, hipStream_t s);
 hipError_t plain_beta_only_cplx(const GemmArgsZ &g, hipStream_t s);
+// Matrix-vector products (gemv.hip), op in OP_N/OP_T/OP_C (real 'C' = 'T'); nbands: the
+// reference's OpenMP row split for real op 'N'.
+hipError_t gemv_real(int op, int64_t m, int64_t n, ddv alpha, ddv beta, const ddv *A, int64_t lda,
+                     const ddv *x, int64_t incx, ddv *y, int64_t incy, int64_t nbands,
+                     hipStream_t s);
+hipError_t gemv_cplx(int op, int64_t m, int64_t n, zddv alpha, zddv beta, const zddv *A,
+                     int64_t lda, const zddv *x, int64_t incx, zddv *y, int64_t incy,
+                     hipStream_t s);
 // Triangular-output GEMM on the UPLO triangle of the n x n C (g.m == g.n), column blocks of nb.
 hipError_t plain_gemmtr_real(const GemmArgsR &g, bool upper, int64_t nb, hipStream_t s);
 hipError_t plain_gemmtr_cplx(const GemmArgsZ &g, bool upper, int64_t nb, hipStream_t s);

@@ -165,6 +165,13 @@ void k_wgemm_beta_only(int64_t m, int64_t n, zddv beta, int beta_mode, zddv *__r
 // and otherwise  s = sum_l a_li * b_lj (ascending, faithful), c += alpha * s.
 // Lane (j, r): column j, r-th row of its block's triangle part of that column.
 // ----------------------------------------------------------------------------------
+DDI ddv mul_exact_any(const ddv &a, const ddv &b) { return dd_mul_exact(a, b); }
+DDI zddv mul_exact_any(const zddv &a, const zddv &b) { return zdd_mul_exact(a, b); }
+DDI ddv add_exact_any(const ddv &a, const ddv &b) { return dd_add_exact(a, b); }
+DDI zddv add_exact_any(const zddv &a, const zddv &b) { return zdd_add_exact(a, b); }
+DDI bool is_zero_any(const ddv &v) { return v.hi == 0.0 && v.lo == 0.0; }
+DDI bool is_zero_any(const zddv &v) { return is_zero_any(v.re) && is_zero_any(v.im); }
+
 DDI bool tri_diag_row(int64_t j, int64_t r, int64_t n, int64_t nb, bool upper, int64_t &i)
 {
     const int64_t jc = (j / nb) * nb;
@@ -177,8 +184,22 @@ DDI bool tri_diag_row(int64_t j, int64_t r, int64_t n, int64_t nb, bool upper, i
 }
 
 // The chain over l is sequential, so operands are loaded DIAG_PF steps ahead of their use:
-// each lane then waits once per DIAG_PF steps on memory instead of once per step.
+// each lane then waits once per DIAG_PF steps on memory instead of once per step.  A batch
+// is branch-free (a zero t_l keeps c by a select, which also keeps a NaN or Inf of a
+// skipped column out of c), so the compiler issues all of its loads first.
 constexpr int DIAG_PF = 8;
+
+template <bool TA, typename V>
+DDI void tr_step(V &c, V &acc, const V &alpha, const V &a, const V &b)
+{
+    if constexpr (!TA) {
+        const V tl = mul_exact_any(alpha, b);
+        const V upd = add_exact_any(c, mul_exact_any(tl, a));
+        c = is_zero_any(tl) ? c : upd;
+    } else {
+        acc = add_exact_any(acc, mul_exact_any(a, b));
+    }
+}
 
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256)
@@ -192,26 +213,21 @@ void k_mgemmtr_diag(int64_t n, int64_t k, int64_t nb, int upper, ddv alpha, ddv 
     if (j >= n || !tri_diag_row(j, r, n, nb, upper != 0, i)) return;
     ddv c = dd_beta_prepass(C[j * ldc + i], beta, beta_mode);
     ddv acc{0.0, 0.0};
-    for (int64_t l0 = 0; l0 < k; l0 += DIAG_PF) {
+    auto a_at = [&](int64_t l) { return TA ? A[i * lda + l] : A[l * lda + i]; };
+    auto b_at = [&](int64_t l) { return TB ? B[l * ldb + j] : B[j * ldb + l]; };
+    int64_t l0 = 0;
+    for (; l0 + DIAG_PF <= k; l0 += DIAG_PF) {
         ddv av[DIAG_PF], bv[DIAG_PF];
 #pragma unroll
         for (int q = 0; q < DIAG_PF; ++q) {
-            const int64_t l = l0 + q < k ? l0 + q : k - 1;
-            av[q] = TA ? A[i * lda + l] : A[l * lda + i];
-            bv[q] = TB ? B[l * ldb + j] : B[j * ldb + l];
+            av[q] = a_at(l0 + q);
+            bv[q] = b_at(l0 + q);
         }
+        __builtin_amdgcn_sched_barrier(0);   // all loads of the batch before its steps
 #pragma unroll
-        for (int q = 0; q < DIAG_PF; ++q) {
-            if (l0 + q >= k) break;
-            if (!TA) {
-                const ddv tl = dd_mul_exact(alpha, bv[q]);
-                if (tl.hi == 0.0 && tl.lo == 0.0) continue;
-                c = dd_add_exact(c, dd_mul_exact(tl, av[q]));
-            } else {
-                acc = dd_add_exact(acc, dd_mul_exact(av[q], bv[q]));
-            }
-        }
+        for (int q = 0; q < DIAG_PF; ++q) tr_step<TA>(c, acc, alpha, av[q], bv[q]);
     }
+    for (; l0 < k; ++l0) tr_step<TA>(c, acc, alpha, a_at(l0), b_at(l0));
     if (TA) c = dd_add_exact(c, dd_mul_exact(alpha, acc));
     C[j * ldc + i] = c;
 }
@@ -228,29 +244,27 @@ void k_wgemmtr_diag(int64_t n, int64_t k, int64_t nb, int upper, zddv alpha, zdd
     if (j >= n || !tri_diag_row(j, r, n, nb, upper != 0, i)) return;
     zddv c = zdd_beta_prepass(C[j * ldc + i], beta, beta_mode);
     zddv acc{{0.0, 0.0}, {0.0, 0.0}};
-    for (int64_t l0 = 0; l0 < k; l0 += DIAG_PF) {
+    auto a_at = [&](int64_t l) {
+        const zddv a = OPA == OP_N ? A[l * lda + i] : A[i * lda + l];
+        return OPA == OP_C ? zdd_conj(a) : a;
+    };
+    auto b_at = [&](int64_t l) {
+        const zddv b = OPB == OP_N ? B[j * ldb + l] : B[l * ldb + j];
+        return OPB == OP_C ? zdd_conj(b) : b;
+    };
+    int64_t l0 = 0;
+    for (; l0 + DIAG_PF <= k; l0 += DIAG_PF) {
         zddv av[DIAG_PF], bv[DIAG_PF];
 #pragma unroll
         for (int q = 0; q < DIAG_PF; ++q) {
-            const int64_t l = l0 + q < k ? l0 + q : k - 1;
-            const zddv a = OPA == OP_N ? A[l * lda + i] : A[i * lda + l];
-            const zddv b = OPB == OP_N ? B[j * ldb + l] : B[l * ldb + j];
-            av[q] = OPA == OP_C ? zdd_conj(a) : a;
-            bv[q] = OPB == OP_C ? zdd_conj(b) : b;
+            av[q] = a_at(l0 + q);
+            bv[q] = b_at(l0 + q);
         }
+        __builtin_amdgcn_sched_barrier(0);   // all loads of the batch before its steps
 #pragma unroll
-        for (int q = 0; q < DIAG_PF; ++q) {
-            if (l0 + q >= k) break;
-            if (OPA == OP_N) {
-                const zddv tl = zdd_mul_exact(alpha, bv[q]);
-                if (tl.re.hi == 0.0 && tl.re.lo == 0.0 && tl.im.hi == 0.0 && tl.im.lo == 0.0)
-                    continue;
-                c = zdd_add_exact(c, zdd_mul_exact(tl, av[q]));
-            } else {
-                acc = zdd_add_exact(acc, zdd_mul_exact(av[q], bv[q]));
-            }
-        }
+        for (int q = 0; q < DIAG_PF; ++q) tr_step<OPA != OP_N>(c, acc, alpha, av[q], bv[q]);
     }
+    for (; l0 < k; ++l0) tr_step<OPA != OP_N>(c, acc, alpha, a_at(l0), b_at(l0));
     if (OPA != OP_N) c = zdd_add_exact(c, zdd_mul_exact(alpha, acc));
     C[j * ldc + i] = c;
 }

@@ -17,6 +17,7 @@ enum KernelClass {
     KC_BATCH_OZ_GEMM,
     KC_BATCH_OZ_CRT,
     KC_GEMMTR_DIAG,
+    KC_GEMV,
     KC_COUNT
 };
 

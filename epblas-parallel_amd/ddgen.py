@@ -106,6 +106,13 @@ def rand_int_log(rng, lo, hi):
     return int(min(hi, max(lo, int(x))))
 
 
+def rand_incx(rng):
+    """Vector stride as the reference fuzz draws it: 1 (50%), 2 (20%), 3 (15%), -1 (10%),
+    -2 (5%) (fuzz_util.fypp:405-420)."""
+    r = rng.random()
+    return 1 if r < 0.5 else 2 if r < 0.7 else 3 if r < 0.85 else -1 if r < 0.95 else -2
+
+
 def rand_pad(rng):
     """Leading-dimension padding 0-4 / 5-32 / 33-128 (fuzz_util.fypp:390-401)."""
     r = rng.random()

@@ -243,6 +243,63 @@ def gemmtr_device(cplx, uplo, transa, transb, n, k, alpha, dA, lda, dB, ldb, bet
         _check(rc)
 
 
+def _gemv(name, width, ct, trans, m, n, alpha, a, lda, x, incx, beta, y, incy):
+    L = lib()
+    al, be = _scalar(alpha, width), _scalar(beta, width)
+    args = [ct(v) for v in (m, n, lda, incx, incy)]
+    f = getattr(L, name)
+    f.restype = None
+    ip = ctypes.POINTER(ct)
+    f.argtypes = [ctypes.c_char_p, ip, ip, _vp, _vp, ip, _vp, ip, _vp, _vp, ip]
+    clear_error()
+    f(trans.encode(), ctypes.byref(args[0]), ctypes.byref(args[1]), _ptr(al), _ptr(a),
+      ctypes.byref(args[2]), _ptr(x), ctypes.byref(args[3]), _ptr(be), _ptr(y),
+      ctypes.byref(args[4]))
+    _check()
+
+
+def mgemv(trans, m, n, alpha, a, lda, x, incx, beta, y, incy):
+    """y := alpha*op(A)*x + beta*y (real DD) through the drop-in ``mgemv_``.  Vectors are
+    (span, 2) arrays holding every |inc|-th entry."""
+    _gemv("mgemv_", 2, ctypes.c_int32, trans, m, n, alpha, a, lda, x, incx, beta, y, incy)
+
+
+def mgemv_64(trans, m, n, alpha, a, lda, x, incx, beta, y, incy):
+    _gemv("mgemv_64_", 2, ctypes.c_int64, trans, m, n, alpha, a, lda, x, incx, beta, y, incy)
+
+
+def wgemv(trans, m, n, alpha, a, lda, x, incx, beta, y, incy):
+    """Complex DD matrix-vector product through the drop-in ``wgemv_``."""
+    _gemv("wgemv_", 4, ctypes.c_int32, trans, m, n, alpha, a, lda, x, incx, beta, y, incy)
+
+
+def wgemv_64(trans, m, n, alpha, a, lda, x, incx, beta, y, incy):
+    _gemv("wgemv_64_", 4, ctypes.c_int64, trans, m, n, alpha, a, lda, x, incx, beta, y, incy)
+
+
+def gemv_device(cplx, trans, m, n, alpha, dA, lda, dx, incx, beta, dy, incy, nbands=0,
+                stream=None):
+    """Device-pointer matrix-vector product (asynchronous on ``stream``)."""
+    L = lib()
+    w = 4 if cplx else 2
+    al, be = _scalar(alpha, w), _scalar(beta, w)
+    i64 = ctypes.c_int64
+    if cplx:
+        f = L.mwgemm_wgemv_device
+        f.argtypes = [ctypes.c_char, i64, i64, _vp, _vp, i64, _vp, i64, _vp, _vp, i64, _vp]
+        extra = ()
+    else:
+        f = L.mwgemm_mgemv_device
+        f.argtypes = [ctypes.c_char, i64, i64, _vp, _vp, i64, _vp, i64, _vp, _vp, i64, i64, _vp]
+        extra = (nbands,)
+    f.restype = ctypes.c_int
+    clear_error()
+    rc = f(trans.encode(), m, n, _ptr(al), _ptr(dA), lda, _ptr(dx), incx, _ptr(be), _ptr(dy),
+           incy, *extra, None if stream is None else ctypes.c_void_p(stream))
+    if rc != 0:
+        _check(rc)
+
+
 def gemm_device(cplx, transa, transb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc,
                 path=PATH_PLAIN, stream=None):
     """Device-pointer GEMM (tensors on the GPU), asynchronous on ``stream`` (an int handle

@@ -124,6 +124,33 @@ MWGEMM_API void wgemmtr_serial(char uplo, char transa, char transb, ptrdiff_t n,
                     const mw_complex64x2 *b, ptrdiff_t ldb, const mw_complex64x2 *beta,
                     mw_complex64x2 *c, ptrdiff_t ldc);
 
+/* Matrix-vector product: y := alpha*op(A)*x + beta*y, A m x n, op 'N', 'T' ('C' = 'T'),
+ * strided x and y (negative strides from the far end).  Replaces mgemv_ =
+ * EPBLAS_FACADE_GEMV(mgemv, TR), src/epblas-parallel/multifloats/mgemv.cpp:277-279
+ * (macro common/epblas_facade.h:132-138; numerics mgemv.cpp:60-275).  Bit-for-bit the
+ * reference's result; for op 'N' that includes its OpenMP row bands (the rows after the last
+ * whole 4-block of each band use faithful operations), reproduced for the team size
+ * MWGEMV_THREADS, else OMP_NUM_THREADS, else the hardware thread count (m < 64: one band).
+ * m == 0 or n == 0 leaves y untouched.  Zero increments and short lda are rejected. */
+MWGEMM_API void mgemv_(const char *trans, const int *m, const int *n, const mw_float64x2 *alpha,
+            const mw_float64x2 *a, const int *lda, const mw_float64x2 *x, const int *incx,
+            const mw_float64x2 *beta, mw_float64x2 *y, const int *incy);
+/* ILP64 twin of mgemv_. */
+MWGEMM_API void mgemv_64_(const char *trans, const int64_t *m, const int64_t *n,
+               const mw_float64x2 *alpha, const mw_float64x2 *a, const int64_t *lda,
+               const mw_float64x2 *x, const int64_t *incx, const mw_float64x2 *beta,
+               mw_float64x2 *y, const int64_t *incy);
+/* Complex twin (op N/T/C).  Replaces wgemv_ = EPBLAS_FACADE_GEMV(wgemv, TC), wgemv.cpp; its
+ * op-'N' row split is 4-aligned, so only the final m % 4 rows take the faithful tail. */
+MWGEMM_API void wgemv_(const char *trans, const int *m, const int *n, const mw_complex64x2 *alpha,
+            const mw_complex64x2 *a, const int *lda, const mw_complex64x2 *x, const int *incx,
+            const mw_complex64x2 *beta, mw_complex64x2 *y, const int *incy);
+/* ILP64 twin of wgemv_. */
+MWGEMM_API void wgemv_64_(const char *trans, const int64_t *m, const int64_t *n,
+               const mw_complex64x2 *alpha, const mw_complex64x2 *a, const int64_t *lda,
+               const mw_complex64x2 *x, const int64_t *incx, const mw_complex64x2 *beta,
+               mw_complex64x2 *y, const int64_t *incy);
+
 /* ---------------------------------------------------------------------------------
  * Library extensions (no counterpart in the reference).
  * ------------------------------------------------------------------------------- */
@@ -166,6 +193,17 @@ MWGEMM_API int mwgemm_wgemmtr_device(char uplo, char transa, char transb, int64_
                           const mw_complex64x2 *alpha, const mw_complex64x2 *dA, int64_t lda,
                           const mw_complex64x2 *dB, int64_t ldb, const mw_complex64x2 *beta,
                           mw_complex64x2 *dC, int64_t ldc, void *stream);
+
+/* Device-pointer matrix-vector products (asynchronous, status code).  nbands: the row bands
+ * of the real op-'N' product (<= 0: the drop-in's rule above). */
+MWGEMM_API int mwgemm_mgemv_device(char trans, int64_t m, int64_t n, const mw_float64x2 *alpha,
+                        const mw_float64x2 *dA, int64_t lda, const mw_float64x2 *dx,
+                        int64_t incx, const mw_float64x2 *beta, mw_float64x2 *dy, int64_t incy,
+                        int64_t nbands, void *stream);
+MWGEMM_API int mwgemm_wgemv_device(char trans, int64_t m, int64_t n, const mw_complex64x2 *alpha,
+                        const mw_complex64x2 *dA, int64_t lda, const mw_complex64x2 *dx,
+                        int64_t incx, const mw_complex64x2 *beta, mw_complex64x2 *dy,
+                        int64_t incy, void *stream);
 
 /* Batched GEMMs: count independent real DD problems C_p := alpha_p op(A_p) op(B_p) +
  * beta_p C_p (one op pair for the whole batch; per-problem sizes, leading dimensions,

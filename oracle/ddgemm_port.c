@@ -665,3 +665,130 @@ int port_wgemmtr(char uplo, char transa, char transb, ptrdiff_t n, ptrdiff_t k,
     }
     return failed ? -1 : 0;
 }
+
+/* ---------------------------------------------------------------------------------
+ * Matrix-vector products mgemv_ / wgemv_: y := alpha*op(A)*x + beta*y, reference AVX2
+ * numerics (src/epblas-parallel/multifloats/mgemv.cpp:225-275, wgemv.cpp).
+ *   m == 0 or n == 0: return before anything (mgemv.cpp:250); then y := beta*y
+ *   (0 -> exact zero, 1 -> kept, else faithful beta*y; mf_kernels.h:63-74); alpha == 0:
+ *   return.  Strided vectors are gathered/scattered, negative strides from the far end
+ *   (mf_kernels.h:84-93); the arithmetic is that of the unit-stride cores:
+ *   op 'N' (mgemv.cpp:60-105): rows are split into nbands bands [lo, hi)
+ *     (blas_part_bound, the OpenMP team of the reference); within a band rows in whole
+ *     4-blocks from lo take y := fast_add(y, fast_mul(t_j, a_ij)), the last (hi-lo) % 4
+ *     rows take y := faithful_add(y, faithful_mul(t_j, a_ij)); t_j = faithful(alpha*x_j),
+ *     columns with x_j == 0 skipped, j ascending.  Complex (wgemv.cpp:55-150): the band
+ *     bounds are floored to multiples of 4 and the last band runs to m, so only the final
+ *     m % 4 rows take the faithful tail.
+ *   op 'T'/'C' (mgemv.cpp:150-205): per column four accumulators s_L over rows i = L mod 4
+ *     below m - m%4 (fast ops, s := fast_add(s, fast_mul(a_ij, x_i))), reduced as
+ *     fast_add(fast_add(s0, s2), fast_add(s1, s3)); the last m % 4 rows faithfully
+ *     (s := s + a_ij*x_i); y_j := faithful_add(y_j, faithful_mul(alpha, s)).  'C' conjugates
+ *     A in the complex case.
+ * ------------------------------------------------------------------------------- */
+static inline ptrdiff_t part_bound(ptrdiff_t total, ptrdiff_t idx, ptrdiff_t nparts)
+{
+    const ptrdiff_t base = total / nparts, rem = total % nparts;
+    return idx * base + (idx < rem ? idx : rem);
+}
+
+static inline ptrdiff_t vec_base(ptrdiff_t len, ptrdiff_t inc)
+{
+    return inc < 0 ? -(len - 1) * inc : 0;
+}
+
+int port_mgemv(char trans, ptrdiff_t m, ptrdiff_t n, const dd *alpha_, const dd *a,
+               ptrdiff_t lda, const dd *x, ptrdiff_t incx, const dd *beta_, dd *y,
+               ptrdiff_t incy, ptrdiff_t nbands)
+{
+    const dd alpha = *alpha_, beta = *beta_;
+    const int notrans = trans_real(trans) == 'N';
+    if (m == 0 || n == 0) return 0;
+    const ptrdiff_t leny = notrans ? m : n, lenx = notrans ? n : m;
+    dd *xs = (dd *)malloc(sizeof(dd) * (size_t)lenx), *ys = (dd *)malloc(sizeof(dd) * (size_t)leny);
+    if (!xs || !ys) { free(xs); free(ys); return -1; }
+    const ptrdiff_t bx = vec_base(lenx, incx), by = vec_base(leny, incy);
+    for (ptrdiff_t i = 0; i < lenx; ++i) xs[i] = x[bx + i * incx];
+    for (ptrdiff_t i = 0; i < leny; ++i) {
+        dd v = y[by + i * incy];
+        if (dd_is0(beta)) { v.hi = 0.0; v.lo = 0.0; }
+        else if (!dd_is1(beta)) v = dd_mul_exact(beta, v);
+        ys[i] = v;
+    }
+    if (!dd_is0(alpha)) {
+        if (notrans) {
+            if (nbands < 1) nbands = 1;
+            for (ptrdiff_t t = 0; t < nbands; ++t) {
+                const ptrdiff_t lo = part_bound(m, t, nbands), hi = part_bound(m, t + 1, nbands);
+                const ptrdiff_t vend = lo + (hi - lo) / 4 * 4;
+                for (ptrdiff_t j = 0; j < n; ++j) {
+                    if (dd_is0(xs[j])) continue;
+                    const dd tj = dd_mul_exact(alpha, xs[j]);
+                    const dd *aj = a + (size_t)j * lda;
+                    for (ptrdiff_t i = lo; i < vend; ++i) ys[i] = dd_add_fast(ys[i], dd_mul_fast(tj, aj[i]));
+                    for (ptrdiff_t i = vend; i < hi; ++i) ys[i] = dd_add_exact(ys[i], dd_mul_exact(tj, aj[i]));
+                }
+            }
+        } else {
+            const ptrdiff_t m4 = m / 4 * 4;
+            for (ptrdiff_t j = 0; j < n; ++j) {
+                const dd *aj = a + (size_t)j * lda;
+                dd s4[4] = { { 0, 0 }, { 0, 0 }, { 0, 0 }, { 0, 0 } };
+                for (ptrdiff_t i = 0; i < m4; ++i) s4[i & 3] = dd_add_fast(s4[i & 3], dd_mul_fast(aj[i], xs[i]));
+                dd s = dd_add_fast(dd_add_fast(s4[0], s4[2]), dd_add_fast(s4[1], s4[3]));
+                for (ptrdiff_t i = m4; i < m; ++i) s = dd_add_exact(s, dd_mul_exact(aj[i], xs[i]));
+                ys[j] = dd_add_exact(ys[j], dd_mul_exact(alpha, s));
+            }
+        }
+    }
+    for (ptrdiff_t i = 0; i < leny; ++i) y[by + i * incy] = ys[i];
+    free(xs); free(ys);
+    return 0;
+}
+
+int port_wgemv(char trans, ptrdiff_t m, ptrdiff_t n, const zdd *alpha_, const zdd *a,
+               ptrdiff_t lda, const zdd *x, ptrdiff_t incx, const zdd *beta_, zdd *y,
+               ptrdiff_t incy)
+{
+    const zdd alpha = *alpha_, beta = *beta_;
+    const char tr = trans_cplx(trans);
+    const int notrans = tr == 'N', conj_a = tr == 'C';
+    if (m == 0 || n == 0) return 0;
+    const ptrdiff_t leny = notrans ? m : n, lenx = notrans ? n : m;
+    zdd *xs = (zdd *)malloc(sizeof(zdd) * (size_t)lenx), *ys = (zdd *)malloc(sizeof(zdd) * (size_t)leny);
+    if (!xs || !ys) { free(xs); free(ys); return -1; }
+    const ptrdiff_t bx = vec_base(lenx, incx), by = vec_base(leny, incy);
+    for (ptrdiff_t i = 0; i < lenx; ++i) xs[i] = x[bx + i * incx];
+    for (ptrdiff_t i = 0; i < leny; ++i) {
+        zdd v = y[by + i * incy];
+        if (zdd_is0(beta)) memset(&v, 0, sizeof v);
+        else if (!zdd_is1(beta)) v = zdd_mul_exact(beta, v);
+        ys[i] = v;
+    }
+    if (!zdd_is0(alpha)) {
+        const ptrdiff_t m4 = m / 4 * 4;
+        if (notrans) {
+            for (ptrdiff_t j = 0; j < n; ++j) {
+                if (zdd_is0(xs[j])) continue;
+                const zdd tj = zdd_mul_exact(alpha, xs[j]);
+                const zdd *aj = a + (size_t)j * lda;
+                for (ptrdiff_t i = 0; i < m4; ++i) ys[i] = zdd_add_fast(ys[i], zdd_mul_fast(tj, aj[i]));
+                for (ptrdiff_t i = m4; i < m; ++i) ys[i] = zdd_add_exact(ys[i], zdd_mul_exact(tj, aj[i]));
+            }
+        } else {
+            for (ptrdiff_t j = 0; j < n; ++j) {
+                const zdd *aj = a + (size_t)j * lda;
+                zdd s4[4];
+                memset(s4, 0, sizeof s4);
+                for (ptrdiff_t i = 0; i < m4; ++i)
+                    s4[i & 3] = zdd_add_fast(s4[i & 3], zdd_mul_fast(zop(aj[i], conj_a), xs[i]));
+                zdd s = zdd_add_fast(zdd_add_fast(s4[0], s4[2]), zdd_add_fast(s4[1], s4[3]));
+                for (ptrdiff_t i = m4; i < m; ++i) s = zdd_add_exact(s, zdd_mul_exact(zop(aj[i], conj_a), xs[i]));
+                ys[j] = zdd_add_exact(ys[j], zdd_mul_exact(alpha, s));
+            }
+        }
+    }
+    for (ptrdiff_t i = 0; i < leny; ++i) y[by + i * incy] = ys[i];
+    free(xs); free(ys);
+    return 0;
+}

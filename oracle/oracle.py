@@ -49,6 +49,12 @@ def lib():
             f.restype = ctypes.c_int
             f.argtypes = [c, c, c, _c_i64, _c_i64, _vp, _vp, _c_i64, _vp, _c_i64, _vp, _vp,
                           _c_i64, _c_i64, ctypes.c_int]
+        _LIB.port_mgemv.restype = ctypes.c_int
+        _LIB.port_mgemv.argtypes = [c, _c_i64, _c_i64, _vp, _vp, _c_i64, _vp, _c_i64, _vp, _vp,
+                                    _c_i64, _c_i64]
+        _LIB.port_wgemv.restype = ctypes.c_int
+        _LIB.port_wgemv.argtypes = [c, _c_i64, _c_i64, _vp, _vp, _c_i64, _vp, _c_i64, _vp, _vp,
+                                    _c_i64]
         for name in ("q128_mgemm_entries", "q128_wgemm_entries"):
             f = getattr(_LIB, name)
             f.restype = None
@@ -132,6 +138,21 @@ def port_wgemmtr(uplo, ta, tb, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nb=8, 
                             ldb, _p(beta), _p(C), ldc, nb, nthreads)
     if rc != 0:
         raise MemoryError("port_wgemmtr: pack buffer allocation failed")
+
+
+def port_mgemv(trans, m, n, alpha, A, lda, x, incx, beta, y, incy, nbands=1):
+    """CPU restatement of the reference mgemv_ (in place on y).  nbands: the reference's
+    OpenMP team size for op 'N' (row bands; 1 = a serial run)."""
+    if lib().port_mgemv(_ch(trans), m, n, _p(alpha), _p(A), lda, _p(x), incx, _p(beta), _p(y),
+                        incy, nbands) != 0:
+        raise MemoryError("port_mgemv: scratch allocation failed")
+
+
+def port_wgemv(trans, m, n, alpha, A, lda, x, incx, beta, y, incy):
+    """CPU restatement of the reference wgemv_ (in place on y)."""
+    if lib().port_wgemv(_ch(trans), m, n, _p(alpha), _p(A), lda, _p(x), incx, _p(beta), _p(y),
+                        incy) != 0:
+        raise MemoryError("port_wgemv: scratch allocation failed")
 
 
 def _entries(ent):

@@ -26,7 +26,7 @@ def test_header_declares_the_dropin_symbols():
     syms = header_symbols()
     for s in ("mgemm_", "mgemm_64_", "wgemm_", "wgemm_64_", "mgemm_serial", "wgemm_serial",
               "mgemmtr_", "mgemmtr_64_", "wgemmtr_", "wgemmtr_64_", "mgemmtr_serial",
-              "wgemmtr_serial"):
+              "wgemmtr_serial", "mgemv_", "mgemv_64_", "wgemv_", "wgemv_64_"):
         assert s in syms
 
 
@@ -67,6 +67,17 @@ def test_invalid_arguments_fail_loudly():
     with pytest.raises(mwblas.MwgemmError, match="leading dimension"):
         mwblas.wgemmtr("L", "N", "N", 4, 4, [1, 0, 0, 0], A, 4, A, 4, [0, 0, 0, 0], C, 3)
     mwblas.mgemmtr("U", "N", "N", 0, 4, [1, 0], A, 4, A, 4, [0, 0], C, 4)   # n == 0: no-op
+    # matrix-vector products: trans letter, lda, zero increments
+    v = np.zeros((8, 2))
+    with pytest.raises(mwblas.MwgemmError, match="trans"):
+        mwblas.mgemv("Q", 4, 4, [1, 0], A, 4, v, 1, [0, 0], v, 1)
+    with pytest.raises(mwblas.MwgemmError, match="leading dimension"):
+        mwblas.mgemv("N", 4, 4, [1, 0], A, 3, v, 1, [0, 0], v, 1)
+    with pytest.raises(mwblas.MwgemmError, match="increment"):
+        mwblas.wgemv("T", 4, 4, [1, 0, 0, 0], A, 4, v, 0, [0, 0, 0, 0], v, 1)
+    y = np.full((4, 2), 3.0)
+    mwblas.mgemv("N", 0, 4, [1, 0], A, 1, v, 1, [2, 0], y, 1)     # m == 0: y untouched
+    assert np.all(y == 3.0)
     assert np.all(C == 0.0)
 
 
