@@ -13,8 +13,9 @@ sample of the same problem.
 Further objects on the same line (each with its own workload): "batched" (2000 small GEMMs
 in one captured launch per path), "complex" (4096^3 with op(A)='C', both paths), "gemmtr"
 (the triangular-output GEMM, 4096 x 4096 upper triangle, k = 4096), "gemv" (16384 x 16384
-matrix-vector products, HBM-bound) and "host_boundary" (the drop-in mgemm_ with host
-operands, PCIe transfers inside the timing).
+matrix-vector products, HBM-bound), "host_boundary" (the drop-in mgemm_ with host
+operands, PCIe transfers inside the timing) and "serial_trailing" (a blocked SYRK's
+trailing updates through mgemm_serial on host arrays).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--size 8192]
                     [--no-batched] [--no-complex] [--no-gemmtr] [--no-gemv] [--no-host]
@@ -324,6 +325,55 @@ def host_boundary_bench(size=4096, reps=3, seed=77):
             "effective_GBps": round(moved / best / 1e9, 2)}
 
 
+def serial_trailing_bench(n=4096, k=1024, nb=64, seed=78):
+    """The trailing updates of a blocked lower SYRK, C(j0:n, jc:jc+nb) += alpha *
+    A(j0:n, :) A(jc:jc+nb, :)^T, issued the way msyrk_serial.cpp:331-339 issues them: one
+    mgemm_serial call per 64-column block on interior pointers of host arrays (the
+    diagonal-block kernels between them are not part of this library and are not run).
+    Every call stages its operands over PCIe.  Also the cost of one tiny 8x8x8 call (the
+    median of individually timed calls, measured first: right after a few hundred MB of
+    fresh host arrays appear, calls are transiently slower for reasons outside the
+    library)."""
+    import numpy as np
+    import mwblas
+    alpha, one = np.array([1.0, 0.0]), np.array([1.0, 0.0])
+    tiny = np.zeros((8, 8, 2))
+    tiny[..., 0] = 1.0
+    out = np.zeros((8, 8, 2))
+    times = []
+    for i in range(220):
+        t1 = time.perf_counter()
+        mwblas.mgemm_serial("N", "N", 8, 8, 8, alpha, tiny, 8, tiny, 8, one, out, 8)
+        if i >= 20:
+            times.append(time.perf_counter() - t1)
+    tiny_us = float(np.median(times)) * 1e6
+    rng = np.random.default_rng(seed)
+    lda = ldc = n
+    A = np.empty((k, n, 2))
+    A[..., 0] = rng.uniform(-1.0, 1.0, (k, n))
+    A[..., 1] = A[..., 0] * 2.0 ** -53 * (rng.random((k, n)) - 0.5)
+    C = np.zeros((n, n, 2))
+    flat_a, flat_c = A.reshape(-1, 2), C.reshape(-1, 2)
+    blocks = [(jc, min(nb, n - jc)) for jc in range(0, n, nb) if n - jc - min(nb, n - jc) > 0]
+
+    def sequence():
+        for jc, jb in blocks:
+            j0 = jc + jb
+            mwblas.mgemm_serial("N", "T", n - j0, jb, k, alpha, flat_a[j0:], lda, flat_a[jc:],
+                                lda, one, flat_c[jc * ldc + j0:], ldc)
+
+    sequence()
+    t0 = time.perf_counter()
+    sequence()
+    t = time.perf_counter() - t0
+    macs = sum((n - jc - jb) * jb * k for jc, jb in blocks)
+    return {"workload": f"lower-SYRK trailing updates n={n} k={k} nb={nb}: {len(blocks)} "
+                        f"mgemm_serial('N','T') calls on host arrays",
+            "value": round(2.0 * macs / t / 1e9, 2), "unit": "DD-GFLOP/s",
+            "ms_total": round(t * 1e3, 2), "calls": len(blocks),
+            "us_per_tiny_call_8x8x8": round(tiny_us, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -470,6 +520,7 @@ def main():
         result["gemv"] = gemv_bench(dev)
     if not args.no_host:
         result["host_boundary"] = host_boundary_bench()
+        result["serial_trailing"] = serial_trailing_bench()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(A, B, n, k)
     print(json.dumps(result))

@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -89,13 +90,16 @@ struct DevState {
     hipStream_t stream = nullptr;
     void *stage = nullptr;   // host-pointer staging (A | B | C)
     size_t stage_bytes = 0;
-    void *pin[2] = {nullptr, nullptr};   // pinned host chunks for host <-> device transfers
+    char *pin[2] = {nullptr, nullptr};   // pinned host chunks for host <-> device transfers
     size_t pin_bytes = 0;
     hipEvent_t pin_ev[2] = {nullptr, nullptr};   // last transfer that used each chunk
     bool pin_used[2] = {false, false};
+    int pin_slot = 0;                            // chunk the next region is taken from
+    size_t pin_off = 0;                          // first free byte in that chunk
+    hipEvent_t dl_ev[2] = {nullptr, nullptr};    // completion of download chunks, in turn
 };
 std::mutex g_state_mu;
-std::vector<DevState> g_dev;
+std::deque<DevState> g_dev;   // a deque keeps each device's state at a fixed address
 std::mutex g_call_mu;        // host-pointer entry points are serialised
 std::atomic<int> g_path{-1};
 
@@ -118,9 +122,13 @@ int ensure_stream(DevState *st)
     return 0;
 }
 
+// Device staging for host operands.  Grows by at least half each time, so that a caller
+// whose shapes grow call by call (a blocked solver's trailing updates) reallocates rarely.
 int ensure_stage(DevState *st, size_t bytes)
 {
     if (st->stage_bytes >= bytes) return 0;
+    bytes = std::max(bytes, st->stage_bytes + st->stage_bytes / 2);
+    bytes = (bytes + (size_t(2) << 20) - 1) & ~((size_t(2) << 20) - 1);
     if (st->stage) {
         MW_TRY(hipStreamSynchronize(st->stream));
         MW_TRY(hipFree(st->stage));
@@ -132,27 +140,12 @@ int ensure_stage(DevState *st, size_t bytes)
     return 0;
 }
 
-// Pinned staging chunks: pageable operands are copied (in parallel, on the host) into one
-// chunk while the DMA of the other is in flight.
+// Pinned staging: two chunks used as a ring of regions.  Pageable operands are copied (in
+// parallel, on the host) into a region while the DMAs of earlier regions are in flight;
+// small operands share a chunk, so a small call waits once, for its result.
 constexpr size_t PIN_CHUNK = size_t(32) << 20;
 
-int ensure_pinned(DevState *st, size_t bytes)
-{
-    if (st->pin_bytes >= bytes) return 0;
-    for (int i = 0; i < 2; ++i) {
-        if (!st->pin_ev[i]) MW_TRY(hipEventCreateWithFlags(&st->pin_ev[i], hipEventDisableTiming));
-        if (st->pin_used[i]) MW_TRY(hipEventSynchronize(st->pin_ev[i]));
-        st->pin_used[i] = false;
-        if (st->pin[i]) {
-            MW_TRY(hipHostFree(st->pin[i]));
-            st->pin[i] = nullptr;
-        }
-    }
-    st->pin_bytes = 0;
-    for (int i = 0; i < 2; ++i) MW_TRY(hipHostMalloc(&st->pin[i], bytes, hipHostMallocDefault));
-    st->pin_bytes = bytes;
-    return 0;
-}
+size_t pin_align(size_t x) { return (x + 255) & ~size_t(255); }
 
 int wait_slot(DevState *st, int slot)
 {
@@ -161,52 +154,138 @@ int wait_slot(DevState *st, int slot)
     return 0;
 }
 
-// Host (pageable) -> device: cols columns of seg bytes, host pitch hp, device pitch dp.
-int upload_pinned(DevState *st, hipStream_t s, char *dst, size_t dp, const char *src,
-                  size_t hp, size_t seg, int64_t cols)
+// Chunks of at least bytes each (PIN_CHUNK unless one region needs more).
+int ensure_pinned(DevState *st, size_t bytes)
+{
+    bytes = pin_align(bytes);
+    for (int i = 0; i < 2; ++i) {
+        if (!st->pin_ev[i]) MW_TRY(hipEventCreateWithFlags(&st->pin_ev[i], hipEventDisableTiming));
+        if (!st->dl_ev[i]) MW_TRY(hipEventCreateWithFlags(&st->dl_ev[i], hipEventDisableTiming));
+    }
+    if (st->pin_bytes >= bytes) return 0;
+    bytes = std::max(bytes, PIN_CHUNK);
+    for (int i = 0; i < 2; ++i) {
+        if (int rc = wait_slot(st, i)) return rc;
+        if (st->pin[i]) {
+            MW_TRY(hipHostFree(st->pin[i]));
+            st->pin[i] = nullptr;
+        }
+    }
+    st->pin_bytes = 0;
+    for (int i = 0; i < 2; ++i)
+        MW_TRY(hipHostMalloc(reinterpret_cast<void **>(&st->pin[i]), bytes, hipHostMallocDefault));
+    st->pin_bytes = bytes;
+    st->pin_slot = 0;
+    st->pin_off = 0;
+    return 0;
+}
+
+// A region of sz bytes (sz <= pin_bytes): the rest of the current chunk if it has room,
+// else the other chunk once the transfers that last used it have finished.  The caller
+// records pin_ev[pin_slot] after the transfer it issues from the region.
+int pin_take(DevState *st, size_t sz, char **out)
+{
+    sz = pin_align(sz);
+    if (st->pin_off + sz > st->pin_bytes) {
+        st->pin_slot ^= 1;
+        if (int rc = wait_slot(st, st->pin_slot)) return rc;
+        st->pin_off = 0;
+    }
+    *out = st->pin[st->pin_slot] + st->pin_off;
+    st->pin_off += sz;
+    return 0;
+}
+
+int pin_mark(DevState *st, hipStream_t s)
+{
+    MW_TRY(hipEventRecord(st->pin_ev[st->pin_slot], s));
+    st->pin_used[st->pin_slot] = true;
+    return 0;
+}
+
+// Host (pageable) -> device: cols columns of seg bytes at host pitch hp, packed on the
+// device (pitch seg), one contiguous DMA per chunk.
+int upload_pinned(DevState *st, hipStream_t s, char *dst, const char *src, size_t hp, size_t seg,
+                  int64_t cols)
 {
     if (cols <= 0 || seg == 0) return 0;
     const int64_t per = (int64_t)std::max<size_t>(1, PIN_CHUNK / seg);
     if (int rc = ensure_pinned(st, (size_t)std::min<int64_t>(per, cols) * seg)) return rc;
-    for (int64_t c0 = 0, i = 0; c0 < cols; c0 += per, ++i) {
-        const int slot = (int)(i & 1);
+    for (int64_t c0 = 0; c0 < cols; c0 += per) {
         const int64_t nc = std::min(per, cols - c0);
-        if (int rc = wait_slot(st, slot)) return rc;
-        host_copy_columns(st->pin[slot], seg, src + c0 * hp, hp, seg, nc);
-        MW_TRY(hipMemcpy2DAsync(dst + c0 * dp, dp, st->pin[slot], seg, seg, nc,
-                                hipMemcpyHostToDevice, s));
-        MW_TRY(hipEventRecord(st->pin_ev[slot], s));
-        st->pin_used[slot] = true;
+        char *reg = nullptr;
+        if (int rc = pin_take(st, (size_t)nc * seg, &reg)) return rc;
+        host_copy_columns(reg, seg, src + c0 * hp, hp, seg, nc);
+        MW_TRY(hipMemcpyAsync(dst + c0 * seg, reg, (size_t)nc * seg, hipMemcpyHostToDevice, s));
+        if (int rc = pin_mark(st, s)) return rc;
     }
     return 0;
 }
 
-// Device -> host (pageable): the DMA of chunk i+1 runs while chunk i is copied out.
+// One host operand of a staged call: cols columns of seg bytes at host pitch hp, to be
+// packed at byte offset doff of the device staging buffer.
+struct HostOperand {
+    const char *host;
+    size_t hp, seg;
+    int64_t cols;
+    size_t doff;
+};
+
+// Uploads the operands.  When their packed span fits one pinned chunk they are gathered
+// into one region laid out like the device buffer and travel in a single DMA (a small call
+// then costs one transfer each way); otherwise each goes through the chunked pipeline.
+int upload_operands(DevState *st, hipStream_t s, char *dbase, const HostOperand *ops, int nops)
+{
+    size_t lo = SIZE_MAX, hi = 0;
+    for (int i = 0; i < nops; ++i) {
+        lo = std::min(lo, ops[i].doff);
+        hi = std::max(hi, ops[i].doff + ops[i].seg * (size_t)ops[i].cols);
+    }
+    if (hi <= lo) return 0;
+    if (hi - lo <= PIN_CHUNK) {
+        if (int rc = ensure_pinned(st, hi - lo)) return rc;
+        char *reg = nullptr;
+        if (int rc = pin_take(st, hi - lo, &reg)) return rc;
+        for (int i = 0; i < nops; ++i)
+            host_copy_columns(reg + (ops[i].doff - lo), ops[i].seg, ops[i].host, ops[i].hp,
+                              ops[i].seg, ops[i].cols);
+        MW_TRY(hipMemcpyAsync(dbase + lo, reg, hi - lo, hipMemcpyHostToDevice, s));
+        return pin_mark(st, s);
+    }
+    for (int i = 0; i < nops; ++i)
+        if (int rc = upload_pinned(st, s, dbase + ops[i].doff, ops[i].host, ops[i].hp,
+                                   ops[i].seg, ops[i].cols)) return rc;
+    return 0;
+}
+
+// Packed device columns (pitch seg) -> host (pageable, pitch hp): the DMA of chunk i+1 runs
+// while chunk i is copied out.  Chunk i+1's region never overlaps chunk i's: it follows it
+// in the same pinned chunk, or lies in the other one, whose earlier regions were all copied
+// out before.  Returns once the last chunk is on the host; as the final DMA is the last
+// work on the stream, the stream is then idle.
 int download_pinned(DevState *st, hipStream_t s, char *dst, size_t hp, const char *src,
-                    size_t dp, size_t seg, int64_t cols)
+                    size_t seg, int64_t cols)
 {
     if (cols <= 0 || seg == 0) return 0;
     const int64_t per = (int64_t)std::max<size_t>(1, PIN_CHUNK / seg);
     if (int rc = ensure_pinned(st, (size_t)std::min<int64_t>(per, cols) * seg)) return rc;
     const int64_t nchunk = (cols + per - 1) / per;
+    char *reg[2] = {nullptr, nullptr};
     auto issue = [&](int64_t i) -> int {
-        const int slot = (int)(i & 1);
         const int64_t c0 = i * per, nc = std::min(per, cols - c0);
-        if (int rc = wait_slot(st, slot)) return rc;
-        MW_TRY(hipMemcpy2DAsync(st->pin[slot], seg, src + c0 * dp, dp, seg, nc,
-                                hipMemcpyDeviceToHost, s));
-        MW_TRY(hipEventRecord(st->pin_ev[slot], s));
-        st->pin_used[slot] = true;
-        return 0;
+        if (int rc = pin_take(st, (size_t)nc * seg, &reg[i & 1])) return rc;
+        MW_TRY(hipMemcpyAsync(reg[i & 1], src + c0 * seg, (size_t)nc * seg,
+                              hipMemcpyDeviceToHost, s));
+        MW_TRY(hipEventRecord(st->dl_ev[i & 1], s));
+        return pin_mark(st, s);
     };
     if (int rc = issue(0)) return rc;
     for (int64_t i = 0; i < nchunk; ++i) {
         if (i + 1 < nchunk)
             if (int rc = issue(i + 1)) return rc;
-        const int slot = (int)(i & 1);
-        if (int rc = wait_slot(st, slot)) return rc;
+        MW_TRY(hipEventSynchronize(st->dl_ev[i & 1]));
         const int64_t c0 = i * per, nc = std::min(per, cols - c0);
-        host_copy_columns(dst + c0 * hp, hp, st->pin[slot], seg, seg, nc);
+        host_copy_columns(dst + c0 * hp, hp, reg[i & 1], seg, seg, nc);
     }
     return 0;
 }
@@ -245,13 +324,6 @@ bool dd_is0(const ddv &x) { return x.hi == 0.0 && x.lo == 0.0; }
 bool dd_is1(const ddv &x) { return x.hi == 1.0 && x.lo == 0.0; }
 bool zdd_is0(const zddv &x) { return dd_is0(x.re) && dd_is0(x.im); }
 bool zdd_is1(const zddv &x) { return dd_is1(x.re) && dd_is0(x.im); }
-
-// Elements spanned by a column-major rows x cols matrix with leading dimension ld.
-int64_t extent(int64_t rows, int64_t cols, int64_t ld)
-{
-    if (rows <= 0 || cols <= 0) return 0;
-    return (cols - 1) * ld + rows;
-}
 
 bool is_device_ptr(const void *p)
 {
@@ -388,10 +460,10 @@ template <typename T, typename Args, typename Run>
 int staged_call(Args g, int64_t a_rows, int64_t a_cols, int64_t b_rows, int64_t b_cols,
                 bool beta_zero, Run run)
 {
-    const int64_t a_elems = extent(a_rows, a_cols, g.lda);
-    const int64_t b_elems = extent(b_rows, b_cols, g.ldb);
-    const int64_t c_elems = extent(g.m, g.n, g.ldc);
     if (g.m <= 0 || g.n <= 0) return 0;
+    const int64_t a_elems = a_rows > 0 && a_cols > 0 ? a_rows * a_cols : 0;
+    const int64_t b_elems = b_rows > 0 && b_cols > 0 ? b_rows * b_cols : 0;
+    const int64_t c_elems = g.m * g.n;
     std::lock_guard<std::mutex> call_lk(g_call_mu);
     int dev = 0;
     if (int rc = current_device(&dev)) return rc;
@@ -402,47 +474,53 @@ int staged_call(Args g, int64_t a_rows, int64_t a_cols, int64_t b_rows, int64_t 
     const T *hA = reinterpret_cast<const T *>(g.A);
     const T *hB = reinterpret_cast<const T *>(g.B);
     T *hC = reinterpret_cast<T *>(g.C);
+    const int64_t ldc_host = g.ldc;
     const bool dA = a_elems == 0 || is_device_ptr(hA);
     const bool dB = b_elems == 0 || is_device_ptr(hB);
     const bool dC = is_device_ptr(hC);
     size_t need = 0;
-    auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
-    size_t offA = need; if (!dA) need += align(a_elems * sizeof(T));
-    size_t offB = need; if (!dB) need += align(b_elems * sizeof(T));
-    size_t offC = need; if (!dC) need += align(c_elems * sizeof(T));
+    size_t offA = need; if (!dA) need += pin_align(a_elems * sizeof(T));
+    size_t offB = need; if (!dB) need += pin_align(b_elems * sizeof(T));
+    size_t offC = need; if (!dC) need += pin_align(c_elems * sizeof(T));
     if (need) {
         if (int rc = ensure_stage(st, need)) return rc;
     }
     char *base = static_cast<char *>(st->stage);
-    // Only the logical rows x cols of each operand travel, through pinned chunks; the device
-    // copies keep the caller's leading dimensions, and C's padding rows are never written.
+    // Only the logical rows x cols of each operand travel.  The device copies are packed
+    // (leading dimension = rows; the kernels' per-element arithmetic does not depend on it),
+    // so each transfer is contiguous, and C's padding rows on the host are never written.
+    HostOperand ops[3];
+    int nops = 0;
     if (!dA) {
+        ops[nops++] = {reinterpret_cast<const char *>(hA), g.lda * sizeof(T), a_rows * sizeof(T),
+                       a_cols, offA};
         g.A = reinterpret_cast<decltype(g.A)>(base + offA);
-        if (int rc = upload_pinned(st, s, base + offA, g.lda * sizeof(T),
-                                   reinterpret_cast<const char *>(hA), g.lda * sizeof(T),
-                                   a_rows * sizeof(T), a_cols)) return rc;
+        g.lda = a_rows;
     }
     if (!dB) {
+        ops[nops++] = {reinterpret_cast<const char *>(hB), g.ldb * sizeof(T), b_rows * sizeof(T),
+                       b_cols, offB};
         g.B = reinterpret_cast<decltype(g.B)>(base + offB);
-        if (int rc = upload_pinned(st, s, base + offB, g.ldb * sizeof(T),
-                                   reinterpret_cast<const char *>(hB), g.ldb * sizeof(T),
-                                   b_rows * sizeof(T), b_cols)) return rc;
+        g.ldb = b_rows;
     }
     if (!dC) {
-        g.C = reinterpret_cast<decltype(g.C)>(base + offC);
         if (!beta_zero)
-            if (int rc = upload_pinned(st, s, base + offC, g.ldc * sizeof(T),
-                                       reinterpret_cast<const char *>(hC), g.ldc * sizeof(T),
-                                       g.m * sizeof(T), g.n)) return rc;
+            ops[nops++] = {reinterpret_cast<const char *>(hC), ldc_host * sizeof(T),
+                           g.m * sizeof(T), g.n, offC};
+        g.C = reinterpret_cast<decltype(g.C)>(base + offC);
+        g.ldc = g.m;
+    }
+    if (int rc = upload_operands(st, s, base, ops, nops)) {
+        (void)hipStreamSynchronize(s);
+        return rc;
     }
     if (int rc = run(g, s)) {
         (void)hipStreamSynchronize(s);
         return rc;
     }
     if (!dC)
-        if (int rc = download_pinned(st, s, reinterpret_cast<char *>(hC), g.ldc * sizeof(T),
-                                     base + offC, g.ldc * sizeof(T), g.m * sizeof(T), g.n))
-            return rc;
+        return download_pinned(st, s, reinterpret_cast<char *>(hC), ldc_host * sizeof(T),
+                               base + offC, g.m * sizeof(T), g.n);
     MW_TRY(hipStreamSynchronize(s));
     return 0;
 }
@@ -622,40 +700,46 @@ int staged_gemv(const T *A, int64_t lda, int64_t m, int64_t n, const T *x, int64
     if (int rc = ensure_stream(st)) return rc;
     hipStream_t s = st->stream;
     const bool dA = is_device_ptr(A), dx = is_device_ptr(x), dy = is_device_ptr(y);
-    auto align = [](size_t v) { return (v + 255) & ~size_t(255); };
-    const int64_t a_elems = extent(m, n, lda);
     size_t need = 0;
-    const size_t offA = need; if (!dA) need += align(a_elems * sizeof(T));
-    const size_t offx = need; if (!dx) need += align(xspan * sizeof(T));
-    const size_t offy = need; if (!dy) need += align(yspan * sizeof(T));
+    const size_t offA = need; if (!dA) need += pin_align(m * n * sizeof(T));
+    const size_t offx = need; if (!dx) need += pin_align(xspan * sizeof(T));
+    const size_t offy = need; if (!dy) need += pin_align(yspan * sizeof(T));
     if (need)
         if (int rc = ensure_stage(st, need)) return rc;
     char *base = static_cast<char *>(st->stage);
     const T *gA = A, *gx = x;
     T *gy = y;
+    int64_t glda = lda;
+    // A travels packed (leading dimension m on the device); the vectors as their whole
+    // strided spans, so the kernels see the caller's increments.
+    HostOperand ops[3];
+    int nops = 0;
     if (!dA) {
+        ops[nops++] = {reinterpret_cast<const char *>(A), lda * sizeof(T), m * sizeof(T), n, offA};
         gA = reinterpret_cast<const T *>(base + offA);
-        if (int rc = upload_pinned(st, s, base + offA, lda * sizeof(T),
-                                   reinterpret_cast<const char *>(A), lda * sizeof(T),
-                                   m * sizeof(T), n)) return rc;
+        glda = m;
     }
     if (!dx) {
+        ops[nops++] = {reinterpret_cast<const char *>(x), xspan * sizeof(T), xspan * sizeof(T), 1,
+                       offx};
         gx = reinterpret_cast<const T *>(base + offx);
-        if (int rc = upload_pinned(st, s, base + offx, xspan * sizeof(T),
-                                   reinterpret_cast<const char *>(x), xspan * sizeof(T),
-                                   xspan * sizeof(T), 1)) return rc;
     }
     if (!dy) {
+        ops[nops++] = {reinterpret_cast<const char *>(y), yspan * sizeof(T), yspan * sizeof(T), 1,
+                       offy};
         gy = reinterpret_cast<T *>(base + offy);
-        if (int rc = upload_pinned(st, s, base + offy, yspan * sizeof(T),
-                                   reinterpret_cast<const char *>(y), yspan * sizeof(T),
-                                   yspan * sizeof(T), 1)) return rc;
     }
-    MW_TRY(run(gA, gx, gy, s));
+    if (int rc = upload_operands(st, s, base, ops, nops)) {
+        (void)hipStreamSynchronize(s);
+        return rc;
+    }
+    if (int rc = run(gA, glda, gx, gy, s)) {
+        (void)hipStreamSynchronize(s);
+        return rc;
+    }
     if (!dy)
-        if (int rc = download_pinned(st, s, reinterpret_cast<char *>(y), yspan * sizeof(T),
-                                     base + offy, yspan * sizeof(T), yspan * sizeof(T), 1))
-            return rc;
+        return download_pinned(st, s, reinterpret_cast<char *>(y), yspan * sizeof(T),
+                               base + offy, yspan * sizeof(T), 1);
     MW_TRY(hipStreamSynchronize(s));
     return 0;
 }
@@ -673,8 +757,8 @@ int mgemv_host(char trans, int64_t m, int64_t n, const mw_float64x2 *alpha,
     return staged_gemv<ddv>(reinterpret_cast<const ddv *>(a), lda, m, n,
                             reinterpret_cast<const ddv *>(x), span_of(notrans ? n : m, incx),
                             reinterpret_cast<ddv *>(y), span_of(notrans ? m : n, incy),
-                            [&](const ddv *gA, const ddv *gx, ddv *gy, hipStream_t s) {
-                                return gemv_real(op, m, n, al, be, gA, lda, gx, incx, gy, incy, nb, s);
+                            [&](const ddv *gA, int64_t glda, const ddv *gx, ddv *gy, hipStream_t s) {
+                                return gemv_real(op, m, n, al, be, gA, glda, gx, incx, gy, incy, nb, s);
                             });
 }
 
@@ -691,8 +775,9 @@ int wgemv_host(char trans, int64_t m, int64_t n, const mw_complex64x2 *alpha,
     return staged_gemv<zddv>(reinterpret_cast<const zddv *>(a), lda, m, n,
                              reinterpret_cast<const zddv *>(x), span_of(notrans ? n : m, incx),
                              reinterpret_cast<zddv *>(y), span_of(notrans ? m : n, incy),
-                             [&](const zddv *gA, const zddv *gx, zddv *gy, hipStream_t s) {
-                                 return gemv_cplx(op, m, n, al, be, gA, lda, gx, incx, gy, incy, s);
+                             [&](const zddv *gA, int64_t glda, const zddv *gx, zddv *gy,
+                                 hipStream_t s) {
+                                 return gemv_cplx(op, m, n, al, be, gA, glda, gx, incx, gy, incy, s);
                              });
 }
 

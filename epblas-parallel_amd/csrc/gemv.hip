@@ -4,9 +4,11 @@
 // (16 B real, 32 B complex, per multiply-add).
 //
 // op 'N': one lane per output row, the j loop sequential per row as in the reference (its
-//   order decides the bits).  A workgroup (one wave) stages t_j = alpha*x_j (faithful) and
-//   the x_j == 0 skip flags of 256 columns at a time in LDS; A is read column by column
-//   with consecutive lanes on consecutive rows.  Rows in whole 4-blocks of their band take the
+//   order decides the bits).  A workgroup (one wave) stages t_j = alpha*x_j (faithful),
+//   the x_j == 0 skip flags and a per-batch "any skip" flag for 1024 columns at a time in
+//   LDS; A is read column by column with consecutive lanes on consecutive rows.  With m
+//   rows there are only m/64 waves, so op 'N' is bound by the FP64 issue rate of the SIMDs
+//   it occupies rather than by HBM once m is below ~64 rows per SIMD.  Rows in whole 4-blocks of their band take the
 //   fast operations, the band tails the faithful ones (the band layout is the reference's
 //   OpenMP row split for mgemv, and "whole 4-blocks, last m % 4 rows" for wgemv).
 // op 'T'/'C': four lanes per output column, lane L accumulating the rows i = L (mod 4)
@@ -71,43 +73,75 @@ DDI ddv shfl_xor(const ddv &v, int mask)
 }
 DDI zddv shfl_xor(const zddv &v, int mask) { return zddv{shfl_xor(v.re, mask), shfl_xor(v.im, mask)}; }
 
-// One step of a row's chain: y := y + t*a (fast operations, or faithful for tail rows);
-// columns with x_j == 0 are skipped by keeping y (a NaN or Inf in such a column must not
-// reach y), as a select so that the batch stays one straight-line block.
-template <bool TAIL, typename V>
+// One step of a row's chain: y := y + t*a (fast operations, or faithful for tail rows).
+// With SKIP, a column with x_j == 0 keeps y (a NaN or Inf in such a column must not reach
+// y), as a select so that the batch stays one straight-line block.
+template <bool TAIL, bool SKIP, typename V>
 DDI V row_step(const V &yi, const V &t, const V &a, int skip)
 {
     const V upd = TAIL ? add_exact(yi, mul_exact(t, a)) : add_fast(yi, mul_fast(t, a));
-    return skip ? yi : upd;
+    return (SKIP && skip) ? yi : upd;
 }
 
-// The column chunk [j0, j0 + cnt) of one row: full batches of pf_n loads, each batch's
-// loads issued before the previous batch's steps (no branches inside a batch), then the
-// remainder one by one.
+// The steps of one batch of pf_n columns whose A entries are in registers; the selects
+// are paid only in a batch that holds a zero x_j (the flag is the same for the whole wave).
 template <bool TAIL, typename V>
-DDI V row_chunk(V yi, const V *__restrict__ ac, int64_t lda, int cnt, const V *ts,
-                const int *skip)
+DDI V row_batch(V yi, const V *av, const V *ts, const int *skip, int any_skip)
 {
     constexpr int PF = pf_n<V>();
-    int q0 = 0;
-    if (cnt >= PF) {
-        V cur[PF], nxt[PF];
+    if (__builtin_amdgcn_readfirstlane(any_skip)) {
 #pragma unroll
-        for (int q = 0; q < PF; ++q) cur[q] = ac[(int64_t)q * lda];
-        for (; q0 + 2 * PF <= cnt; q0 += PF) {
+        for (int q = 0; q < PF; ++q) yi = row_step<TAIL, true>(yi, ts[q], av[q], skip[q]);
+    } else {
 #pragma unroll
-            for (int q = 0; q < PF; ++q) nxt[q] = ac[(int64_t)(q0 + PF + q) * lda];
-            __builtin_amdgcn_sched_barrier(0);   // next batch in flight before these steps
-#pragma unroll
-            for (int q = 0; q < PF; ++q) yi = row_step<TAIL>(yi, ts[q0 + q], cur[q], skip[q0 + q]);
-#pragma unroll
-            for (int q = 0; q < PF; ++q) cur[q] = nxt[q];
-        }
-#pragma unroll
-        for (int q = 0; q < PF; ++q) yi = row_step<TAIL>(yi, ts[q0 + q], cur[q], skip[q0 + q]);
-        q0 += PF;
+        for (int q = 0; q < PF; ++q) yi = row_step<TAIL, false>(yi, ts[q], av[q], 0);
     }
-    for (; q0 < cnt; ++q0) yi = row_step<TAIL>(yi, ts[q0], ac[(int64_t)q0 * lda], skip[q0]);
+    return yi;
+}
+
+// Loads the A entries of batch b of the chunk: the column base is the same for the whole
+// wave, the row offset i is the lane's.
+template <typename V>
+DDI void row_load(V *av, const V *__restrict__ acol, int64_t lda, int b, int64_t i)
+{
+    constexpr int PF = pf_n<V>();
+    const V *base = acol + (int64_t)b * PF * lda;
+#pragma unroll
+    for (int q = 0; q < PF; ++q) av[q] = (base + (int64_t)q * lda)[i];
+}
+
+// The column chunk [j0, j0 + cnt) of row i: full batches of pf_n loads, each batch's loads
+// issued before the previous batch's steps (two register sets used in turn, no branches
+// inside a batch), then the remainder one by one.
+template <bool TAIL, typename V>
+DDI V row_chunk(V yi, const V *__restrict__ acol, int64_t lda, int64_t i, int cnt,
+                const V *ts, const int *skip, const int *bskip)
+{
+    constexpr int PF = pf_n<V>();
+    const int nb = cnt / PF;
+    if (nb > 0) {
+        V ba[PF], bb[PF];
+        row_load(ba, acol, lda, 0, i);
+        int b = 0;
+        for (; b + 2 < nb; b += 2) {
+            row_load(bb, acol, lda, b + 1, i);
+            __builtin_amdgcn_sched_barrier(0);   // next batch in flight before these steps
+            yi = row_batch<TAIL>(yi, ba, ts + b * PF, skip + b * PF, bskip[b]);
+            row_load(ba, acol, lda, b + 2, i);
+            __builtin_amdgcn_sched_barrier(0);
+            yi = row_batch<TAIL>(yi, bb, ts + (b + 1) * PF, skip + (b + 1) * PF, bskip[b + 1]);
+        }
+        if (b + 1 < nb) {
+            row_load(bb, acol, lda, b + 1, i);
+            __builtin_amdgcn_sched_barrier(0);
+            yi = row_batch<TAIL>(yi, ba, ts + b * PF, skip + b * PF, bskip[b]);
+            yi = row_batch<TAIL>(yi, bb, ts + (b + 1) * PF, skip + (b + 1) * PF, bskip[b + 1]);
+        } else {
+            yi = row_batch<TAIL>(yi, ba, ts + b * PF, skip + b * PF, bskip[b]);
+        }
+    }
+    for (int q = nb * PF; q < cnt; ++q)
+        yi = row_step<TAIL, true>(yi, ts[q], (acol + (int64_t)q * lda)[i], skip[q]);
     return yi;
 }
 
@@ -118,8 +152,10 @@ void k_gemv_n(int64_t m, int64_t n, V alpha, V beta, int beta_mode, int alpha0,
               const V *__restrict__ A, int64_t lda, const V *__restrict__ x, int64_t incx,
               V *__restrict__ y, int64_t incy, int64_t nbands)
 {
+    constexpr int PF = pf_n<V>();
     __shared__ V ts[GEMV_CHUNK];
     __shared__ int skip[GEMV_CHUNK];
+    __shared__ int bskip[GEMV_CHUNK / PF];   // per batch: does it hold a zero x_j
     const int64_t i = (int64_t)blockIdx.x * GEMV_NB + threadIdx.x;
     const bool live = i < m;
     const int64_t by = vec_base(m, incy), bx = vec_base(n, incx);
@@ -139,10 +175,16 @@ void k_gemv_n(int64_t m, int64_t n, V alpha, V beta, int beta_mode, int alpha0,
             }
             __syncthreads();
             const int cnt = (int)(n - j0 < GEMV_CHUNK ? n - j0 : GEMV_CHUNK);
+            for (int b = threadIdx.x; b < GEMV_CHUNK / PF; b += GEMV_NB) {
+                int f = 0;
+                for (int q = b * PF; q < b * PF + PF && q < cnt; ++q) f |= skip[q];
+                bskip[b] = f;
+            }
+            __syncthreads();
             if (!live) continue;
-            const V *ac = A + j0 * lda + i;
-            yi = tail ? row_chunk<true>(yi, ac, lda, cnt, ts, skip)
-                      : row_chunk<false>(yi, ac, lda, cnt, ts, skip);
+            const V *acol = A + j0 * lda;
+            yi = tail ? row_chunk<true>(yi, acol, lda, i, cnt, ts, skip, bskip)
+                      : row_chunk<false>(yi, acol, lda, i, cnt, ts, skip, bskip);
         }
     }
     if (live) y[by + i * incy] = yi;
