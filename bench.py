@@ -10,12 +10,13 @@ residue GEMM) timed with HIP events on the library's stream; the CPU baseline is
 oracle's restatement of the reference OpenMP mgemm_ timed on this host on a bounded row
 sample of the same problem.
 
-Further objects on the same line (each with its own workload): "batched" (2000 small GEMMs
-in one captured launch per path), "complex" (4096^3 with op(A)='C', both paths), "gemmtr"
-(the triangular-output GEMM, 4096 x 4096 upper triangle, k = 4096), "gemv" (16384 x 16384
-matrix-vector products, HBM-bound), "host_boundary" (the drop-in mgemm_ with host
-operands, PCIe transfers inside the timing) and "serial_trailing" (a blocked SYRK's
-trailing updates through mgemm_serial on host arrays).
+Further objects on the same line (each with its own workload): "library_int8_gemm" (the
+vendor INT8 GEMM at one residue GEMM's shape, a reference point for the roofline),
+"batched" (2000 small GEMMs in one captured launch per path), "complex" (4096^3 with
+op(A)='C', both paths), "gemmtr" (the triangular-output GEMM, 4096 x 4096 upper triangle,
+k = 4096), "gemv" (16384 x 16384 matrix-vector products, HBM-bound), "host_boundary" (the
+drop-in mgemm_ with host operands, PCIe transfers inside the timing) and "serial_trailing"
+(a blocked SYRK's trailing updates through mgemm_serial on host arrays).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--size 8192]
                     [--no-batched] [--no-complex] [--no-gemmtr] [--no-gemv] [--no-host]
@@ -325,6 +326,33 @@ def host_boundary_bench(size=4096, reps=3, seed=77):
             "effective_GBps": round(moved / best / 1e9, 2)}
 
 
+def library_int8_bench(dev, size, reps=20):
+    """A plain INT8 x INT8 -> INT32 GEMM of one residue GEMM's shape (size^3) through the
+    vendor library (torch._int_mm, hipBLASLt on ROCm), as a reference point for what this
+    GPU sustains on INT8 MFMA at this shape: int8 ops per second and the fraction of the
+    5.0 POP/s peak.  Nothing of it is used by the library."""
+    import torch
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(size)
+    a = torch.randint(-127, 128, (size, size), dtype=torch.int8, device=dev, generator=gen)
+    b = torch.randint(-127, 128, (size, size), dtype=torch.int8, device=dev,
+                      generator=gen).t().contiguous().t()
+    for _ in range(3):
+        torch._int_mm(a, b)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        torch._int_mm(a, b)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    tops = 2.0 * size ** 3 / (ms * 1e-3) / 1e12
+    return {"workload": f"torch._int_mm int8 {size}^3 (B column-major), int32 output",
+            "ms": round(ms, 3), "TOP/s": round(tops, 1),
+            "frac": round(tops / INT8_PEAK_TOPS, 4)}
+
+
 def serial_trailing_bench(n=4096, k=1024, nb=64, seed=78):
     """The trailing updates of a blocked lower SYRK, C(j0:n, jc:jc+nb) += alpha *
     A(j0:n, :) A(jc:jc+nb, :)^T, issued the way msyrk_serial.cpp:331-339 issues them: one
@@ -510,6 +538,10 @@ def main():
                      "frac": round(achieved / INT8_PEAK_TOPS, 4) if achieved else None,
                      "traffic": traffic, "kernel": "k_oz_gemm_ring (INT8 residue GEMM, int8 ops)"},
     }
+    try:
+        result["library_int8_gemm"] = library_int8_bench(dev, n)
+    except (RuntimeError, AttributeError) as e:   # reference point only
+        result["library_int8_gemm"] = {"error": str(e)[:200]}
     if not args.no_batched:
         result["batched"] = batched_bench(dev)
     if not args.no_complex:

@@ -60,6 +60,15 @@ DDI int find_problem(const int64_t *__restrict__ start, int count, int64_t u)
     return lo;
 }
 
+// The problem of the calling wave.  Every problem's unit count is padded to a multiple of
+// 64, so a wave's 64 units all belong to one problem: the search runs once per wave, on a
+// wave-uniform index (scalar loads), instead of once per lane.
+DDI int wave_problem(const UnitSpace &us)
+{
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    return find_problem(us.start, us.count, (int64_t)blockIdx.x * 256 + w * 64);
+}
+
 __global__ __launch_bounds__(256)
 void k_batch_fill(unsigned long long *p, int64_t n, int *counters, int ncounters)
 {
@@ -71,8 +80,9 @@ void k_batch_fill(unsigned long long *p, int64_t n, int *counters, int ncounters
 // ---------------------------------------------------------------------------------
 // Plain persistent kernel.
 // ---------------------------------------------------------------------------------
+// Two workgroups per CU (at most 256 VGPRs): the tile routine alone sits just below that.
 template <bool TA, bool TB>
-__global__ __launch_bounds__(256)
+__global__ __launch_bounds__(256, 2)
 void k_batch_plain(const BatchProb *__restrict__ probs, const int4 *__restrict__ tiles,
                    int ntiles, int *__restrict__ counter)
 {
@@ -82,7 +92,9 @@ void k_batch_plain(const BatchProb *__restrict__ probs, const int4 *__restrict__
     for (;;) {
         if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
         __syncthreads();
-        const int t = s_tile;
+        // the same for the whole workgroup: the tile record and its problem load into
+        // scalar registers
+        const int t = __builtin_amdgcn_readfirstlane(s_tile);
         __syncthreads();
         if (t >= ntiles) return;          // every workgroup ends here once the queue drains
         const int4 tr = tiles[t];
@@ -118,7 +130,7 @@ void k_batch_rowmax(const BatchProb *__restrict__ probs, UnitSpace us)
 {
     const int64_t u = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (u >= us.start[us.count]) return;
-    const int pi = find_problem(us.start, us.count, u);
+    const int pi = wave_problem(us);
     const BatchProb &p = probs[pi];
     const int64_t R = A_SIDE ? p.m : p.n;
     const int64_t local = u - us.start[pi];
@@ -139,6 +151,7 @@ void k_batch_rowmax(const BatchProb *__restrict__ probs, UnitSpace us)
         }
         if (lane == 0 && bits != 0ull) atomicMax(&rmax[r], bits);
     } else {
+        if (local >= R * ((p.k + 63) / 64)) return;   // padding units
         const int64_t r = local % R, kc = local / R;
         double mx = 0.0;
         const int64_t kend = (kc * 64 + 64 < p.k) ? kc * 64 + 64 : p.k;
@@ -158,13 +171,14 @@ void k_batch_split(const BatchProb *__restrict__ probs, UnitSpace us, int bits,
 {
     const int64_t u = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (u >= us.start[us.count]) return;
-    const int pi = find_problem(us.start, us.count, u);
+    const int pi = wave_problem(us);
     const BatchProb &p = probs[pi];
     const int64_t R = A_SIDE ? p.m : p.n;
     const int64_t G = p.Kc / 16;
     const int64_t local = u - us.start[pi];
     int64_t r, kg;
     if (KC) {
+        if (local >= R * G) return;   // padding units
         kg = local % G;
         r = local / G;
     } else {
@@ -193,7 +207,9 @@ void k_batch_oz_gemm(const BatchProb *__restrict__ probs, const int4 *__restrict
     for (;;) {
         if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
         __syncthreads();
-        const int t = s_tile;
+        // the same for the whole workgroup: the tile record and its problem load into
+        // scalar registers
+        const int t = __builtin_amdgcn_readfirstlane(s_tile);
         __syncthreads();
         if (t >= ntiles) return;
         const int4 tr = tiles[t];      // (problem, modulus, tm, tn)
@@ -213,10 +229,11 @@ void k_batch_crt(const BatchProb *__restrict__ probs, UnitSpace us, int nmod,
 {
     const int64_t u = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (u >= us.start[us.count]) return;
-    const int pi = find_problem(us.start, us.count, u);
+    const int pi = wave_problem(us);
     const BatchProb &p = probs[pi];
     const int64_t groups = (p.m + 3) / 4;
     const int64_t local = u - us.start[pi];
+    if (local >= p.n * groups) return;   // padding units
     const int64_t j = local / groups, i0 = (local % groups) * 4;
     if (!p.active) {
         for (int e = 0; e < 4 && i0 + e < p.m; ++e) {
@@ -441,7 +458,7 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
                     for (int64_t tm = 0; tm < (q.m + 63) / 64; ++tm) tl.push_back(int4{(int)p, (int)tm, (int)tn, (int)(q.active ? q.k : 0)});
             continue;
         }
-        cr[p + 1] = empty ? 0 : q.n * ((q.m + 3) / 4);
+        cr[p + 1] = empty ? 0 : rup(q.n * ((q.m + 3) / 4), 64);
         if (empty || !q.active) continue;
         q.Mp = rup(q.m, OT); q.Np = rup(q.n, OT); q.Kc = rup(q.k, 64);
         offR[p] = rmax_n;
@@ -449,12 +466,13 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
         offA[p] = ws; ws += rup((int64_t)nmod * q.Mp * q.Kc, 256);
         offB[p] = ws; ws += rup((int64_t)nmod * q.Np * q.Kc, 256);
         offU[p] = ws; ws += rup((int64_t)nmod * q.Np * q.Mp, 256);
-        // unit counts of the maxima and split kernels (layouts documented there)
+        // unit counts of the maxima and split kernels (layouts documented there), each
+        // padded to whole waves so that a wave never spans two problems
         const int64_t nch = (q.k + 63) / 64, G = q.Kc / 16;
-        ra[p + 1] = a_kc ? q.m * nch * 64 : q.m * nch;
-        rb[p + 1] = b_kc ? q.n * nch * 64 : q.n * nch;
-        sa[p + 1] = a_kc ? q.m * G : rup(q.m, 8) * rup(G, 8);
-        sb[p + 1] = b_kc ? q.n * G : rup(q.n, 8) * rup(G, 8);
+        ra[p + 1] = a_kc ? q.m * nch * 64 : rup(q.m * nch, 64);
+        rb[p + 1] = b_kc ? q.n * nch * 64 : rup(q.n * nch, 64);
+        sa[p + 1] = a_kc ? rup(q.m * G, 64) : rup(q.m, 8) * rup(G, 8);
+        sb[p + 1] = b_kc ? rup(q.n * G, 64) : rup(q.n, 8) * rup(G, 8);
         for (int t = 0; t < nmod; ++t)
             for (int64_t tn = 0; tn < q.Np / OT; ++tn)
                 for (int64_t tm = 0; tm < q.Mp / OT; ++tm) tl.push_back(int4{(int)p, t, (int)tm, (int)tn});

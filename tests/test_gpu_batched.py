@@ -89,12 +89,15 @@ def test_plain_batched_graph_replays(gpu):
     plan.close()
 
 
-def test_ozaki_batched_vs_binary128(gpu):
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
+def test_ozaki_batched_vs_binary128(gpu, ta, tb):
+    """Every operand layout of the residue split and maxima (row- and column-contiguous A
+    and B), problems of ragged sizes so that unit spaces are padded to whole waves."""
     import torch
-    rng = np.random.default_rng(33)
-    probs = _problems(rng, 40, 1, 300)
+    rng = np.random.default_rng(33 + 2 * (ta == "T") + (tb == "T"))
+    probs = _problems(rng, 40, 1, 300, ta, tb)
     dA, dB, dC = _device(probs, gpu)
-    plan = mwblas.BatchedPlan("N", "N", *_args(probs, dA, dB, dC), path=mwblas.PATH_OZAKI)
+    plan = mwblas.BatchedPlan(ta, tb, *_args(probs, dA, dB, dC), path=mwblas.PATH_OZAKI)
     assert plan.info()["path"] == mwblas.PATH_OZAKI
     stream = torch.cuda.current_stream().cuda_stream
     for rep in range(2):
@@ -107,10 +110,10 @@ def test_ozaki_batched_vs_binary128(gpu):
             C = dC[i].cpu().numpy()
             if p["m"] == 0 or p["n"] == 0:
                 continue
-            r = oracle.q128_gemm_err(False, "N", "N", p["m"], p["n"], p["k"], p["alpha"], p["A"],
+            r = oracle.q128_gemm_err(False, ta, tb, p["m"], p["n"], p["k"], p["alpha"], p["A"],
                                      p["lda"], p["B"], p["ldb"], p["beta"], p["C0"], p["ldc"], C,
                                      p["ldc"])
-            assert r["err"] <= 1e-30, (rep, i, p["m"], p["n"], p["k"], r)
+            assert r["err"] <= 1e-30, (ta, tb, rep, i, p["m"], p["n"], p["k"], r)
             assert np.array_equal(C[:, p["m"]:, :], p["C0"][:, p["m"]:, :])
     plan.close()
 
