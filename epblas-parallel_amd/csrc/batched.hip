@@ -194,11 +194,13 @@ void k_batch_split(const BatchProb *__restrict__ probs, UnitSpace us, int bits,
                            rows * p.Kc, r, kg);
 }
 
-// Residue GEMMs: persistent queue over (problem, modulus, tm, tn) tiles of 64*T x 64*T.
+// Residue GEMMs: persistent queue over (problem, first modulus, tm, tn) tiles of 64*T x 64*T,
+// each running the products of `group` consecutive moduli as one slab pipeline.
 template <int T>
 __global__ __launch_bounds__(256, 2)
 void k_batch_oz_gemm(const BatchProb *__restrict__ probs, const int4 *__restrict__ tiles,
-                     int ntiles, int *__restrict__ counter, const ModConst *__restrict__ mcs)
+                     int ntiles, int *__restrict__ counter, const ModConst *__restrict__ mcs,
+                     int nmod, int group)
 {
     constexpr int BT = 64 * T;
     __shared__ __attribute__((aligned(16))) int8_t sa[2][BT * 80];
@@ -212,13 +214,41 @@ void k_batch_oz_gemm(const BatchProb *__restrict__ probs, const int4 *__restrict
         const int t = __builtin_amdgcn_readfirstlane(s_tile);
         __syncthreads();
         if (t >= ntiles) return;
-        const int4 tr = tiles[t];      // (problem, modulus, tm, tn)
+        const int4 tr = tiles[t];      // (problem, first modulus, tm, tn)
         const BatchProb &p = probs[tr.x];
         const int64_t i0 = (int64_t)tr.z * BT, j0 = (int64_t)tr.w * BT;
-        const int8_t *Ab = p.resA + (int64_t)tr.y * p.Mp * p.Kc + i0 * p.Kc;
-        const int8_t *Bb = p.resB + (int64_t)tr.y * p.Np * p.Kc + j0 * p.Kc;
-        uint8_t *Ub = p.U + (int64_t)tr.y * p.Np * p.Mp;
-        oz_gemm_tile<T, T>(Ab, Bb, Ub, p.Kc, p.Mp, mcs[tr.y], i0, j0, sa, sb);
+        const int64_t sA = p.Mp * p.Kc, sB = p.Np * p.Kc, sU = p.Np * p.Mp;
+        const int nm = nmod - tr.y < group ? nmod - tr.y : group;
+        oz_gemm_tile<T, T>(p.resA + tr.y * sA + i0 * p.Kc, p.resB + tr.y * sB + j0 * p.Kc,
+                           p.U + tr.y * sU, p.Kc, p.Mp, mcs + tr.y, nm, sA, sB, sU, i0, j0,
+                           sa, sb);
+    }
+}
+
+// The same queue through the DMA-fed LDS ring (128 x 128 tiles, four stages of 64-deep
+// slabs in flight): with the short K of small problems a register-staged tile waits on
+// memory at every slab, the ring keeps three slabs in flight across modulus boundaries.
+__global__ __launch_bounds__(256, 2)
+void k_batch_oz_ring(const BatchProb *__restrict__ probs, const int4 *__restrict__ tiles,
+                     int ntiles, int *__restrict__ counter, const ModConst *__restrict__ mcs,
+                     int nmod, int group)
+{
+    __shared__ __attribute__((aligned(1024))) int8_t smem[RING_NST * ring_slot<128>()];
+    __shared__ int s_tile;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(s_tile);
+        __syncthreads();
+        if (t >= ntiles) return;
+        const int4 tr = tiles[t];      // (problem, first modulus, tm, tn)
+        const BatchProb &p = probs[tr.x];
+        const int64_t i0 = (int64_t)tr.z * 128, j0 = (int64_t)tr.w * 128;
+        const int64_t sA = p.Mp * p.Kc, sB = p.Np * p.Kc, sU = p.Np * p.Mp;
+        const int nm = nmod - tr.y < group ? nmod - tr.y : group;
+        oz_gemm_tile_ring<2, 2, 2, 2, true>(p.resA + tr.y * sA + i0 * p.Kc,
+                                            p.resB + tr.y * sB + j0 * p.Kc, p.U + tr.y * sU,
+                                            p.Kc, p.Mp, mcs + tr.y, nm, sA, sB, sU, i0, j0, smem);
     }
 }
 
@@ -280,6 +310,7 @@ struct BatchPlan {
     int64_t rmax_count = 0;
     int grid_persist = 0;
     int oz_tile = 128;                  // residue-GEMM tile edge (128 or 64)
+    int mod_group = 0;                  // moduli per residue tile (all of them unless split)
     OzPlan oz{};
     ModConst *mcs = nullptr;
     hipGraph_t graph = nullptr;
@@ -343,11 +374,11 @@ static int batch_enqueue(BatchPlan *P, hipStream_t s)
     if (P->ntiles) {
         ScopedTimer tm(KC_BATCH_OZ_GEMM, s);
         if (P->oz_tile == 128)
-            hipLaunchKernelGGL((k_batch_oz_gemm<2>), dim3((unsigned)P->grid_persist), dim3(256), 0, s,
-                               P->probs, P->tiles, P->ntiles, P->counters, P->mcs);
+            hipLaunchKernelGGL(k_batch_oz_ring, dim3((unsigned)P->grid_persist), dim3(256), 0, s,
+                               P->probs, P->tiles, P->ntiles, P->counters, P->mcs, P->oz.nmod, P->mod_group);
         else
             hipLaunchKernelGGL((k_batch_oz_gemm<1>), dim3((unsigned)P->grid_persist), dim3(256), 0, s,
-                               P->probs, P->tiles, P->ntiles, P->counters, P->mcs);
+                               P->probs, P->tiles, P->ntiles, P->counters, P->mcs, P->oz.nmod, P->mod_group);
         B_TRY(hipGetLastError());
     }
     if (P->n_crt) {
@@ -473,9 +504,23 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
         rb[p + 1] = b_kc ? q.n * nch * 64 : rup(q.n * nch, 64);
         sa[p + 1] = a_kc ? rup(q.m * G, 64) : rup(q.m, 8) * rup(G, 8);
         sb[p + 1] = b_kc ? rup(q.n * G, 64) : rup(q.n, 8) * rup(G, 8);
-        for (int t = 0; t < nmod; ++t)
-            for (int64_t tn = 0; tn < q.Np / OT; ++tn)
-                for (int64_t tm = 0; tm < q.Mp / OT; ++tm) tl.push_back(int4{(int)p, t, (int)tm, (int)tn});
+        for (int64_t tn = 0; tn < q.Np / OT; ++tn)
+            for (int64_t tm = 0; tm < q.Mp / OT; ++tm) tl.push_back(int4{(int)p, 0, (int)tm, (int)tn});
+    }
+    // Residue tiles run all moduli in one slab pipeline; only when that leaves fewer than
+    // about four tiles per CU are the moduli split into groups (tile .y = first modulus).
+    if (oz && !tl.empty()) {
+        const int64_t want = 4 * (int64_t)cus;
+        int groups = (int)std::min<int64_t>(nmod, (want + (int64_t)tl.size() - 1) / (int64_t)tl.size());
+        groups = std::max(groups, 1);
+        P->mod_group = (nmod + groups - 1) / groups;
+        if (P->mod_group < nmod) {
+            std::vector<int4> split;
+            split.reserve(tl.size() * (size_t)groups);
+            for (const int4 &t : tl)
+                for (int t0 = 0; t0 < nmod; t0 += P->mod_group) split.push_back(int4{t.x, t0, t.z, t.w});
+            tl.swap(split);
+        }
     }
     // Longest jobs first: plain tiles cost ~K, residue tiles ~Kc.
     if (!oz)

@@ -8,9 +8,10 @@
 //   2. For each modulus m_t of a table of pairwise-coprime moduli <= 256, take symmetric
 //      residues A'_t = A' mod m_t, B'_t = B' mod m_t, which fit in int8.
 //   3. C'_t = A'_t B'_t on the INT8 MFMA units with INT32 accumulation -- exact, since
-//      k * 128^2 < 2^31 -- reduced mod m_t and pre-multiplied by the CRT weight y_t.
+//      k * 128^2 < 2^31 -- reduced to u_t = C'_t mod m_t.
 //   4. Reconstruct C' = A'B' exactly (|C'| < M/2, M = prod m_t) from its residues: C'/M =
-//      frac(sum_t u_t / m_t), evaluated in 128-bit fixed point with exact FP64 integer
+//      frac(sum_t u_t y_t / m_t) with the CRT weights y_t = (M/m_t)^-1 mod m_t, evaluated in
+//      128-bit fixed point (limbs of round(2^128 y_t / m_t)) with exact FP64 integer
 //      arithmetic, then C' = M * that as a double-double, scaled back by 2^-(sigma+tau),
 //      and combined as C := beta-prepass(C0) + alpha*C' (faithful DD ops).
 // Error: normwise (relative to K*max|A|*max|B|) at most ~4*2^-b from the rounding in
@@ -166,8 +167,10 @@ bool ozaki_make_plan(int64_t k_eff, int target_bits, int force_nmod, OzPlan *p)
         c.y = (double)mod_inverse(others, m);
         c.p1 = (double)mod_pow2(35, m);
         c.p2 = (double)mod_pow2(70, m);
-        // round(2^128 / m) by long division of the 5-limb number 2^128.
-        uint32_t num[5] = {0, 0, 0, 0, 1};
+        // round(y * 2^128 / m) mod 2^128 by long division of the 5-limb number y * 2^128:
+        // the CRT weight y is folded into the fixed-point limbs, so the residue GEMMs store
+        // plain residues v mod m (frac(((v mod m) y mod m) / m) == frac((v mod m) y / m)).
+        uint32_t num[5] = {0, 0, 0, 0, (uint32_t)c.y};
         uint32_t q[5] = {0, 0, 0, 0, 0};
         uint64_t rem = 0;
         for (int i = 4; i >= 0; --i) {
@@ -282,7 +285,7 @@ void k_oz_gemm(const int8_t *__restrict__ A, const int8_t *__restrict__ B,
     const int64_t i0 = (int64_t)tm * BM;
     const int64_t j0 = (int64_t)tn * BN;
     oz_gemm_tile<TM, TN, BK>(A + batch * sA + i0 * kp, B + batch * sB + j0 * kp, U + batch * sU,
-                             kp, ldu, mcs[batch / per], i0, j0, sa, sb);
+                             kp, ldu, mcs + batch / per, 1, 0, 0, 0, i0, j0, sa, sb);
 }
 
 // 256 x 256 residue tiles through the DMA-fed LDS ring (eight waves, one workgroup per CU).
@@ -297,8 +300,9 @@ void k_oz_gemm_ring(const int8_t *__restrict__ A, const int8_t *__restrict__ B,
     tile_order(blockIdx.x, tiles_m, (int)gridDim.x / tiles_m, tm, tn);
     const int64_t i0 = (int64_t)tm * 256;
     const int64_t j0 = (int64_t)tn * 256;
-    oz_gemm_tile_ring(A + batch * sA + i0 * kp, B + batch * sB + j0 * kp, U + batch * sU, kp, ldu,
-                      mcs[batch / per], i0, j0, smem);
+    oz_gemm_tile_ring<4, 2, 2, 4, false>(A + batch * sA + i0 * kp, B + batch * sB + j0 * kp,
+                                         U + batch * sU, kp, ldu, mcs + batch / per, 1, 0, 0, 0,
+                                         i0, j0, smem);
 }
 
 // Reconstruction kernel: a workgroup owns 1024 consecutive rows of one column.  Each lane

@@ -42,7 +42,7 @@ bool ozaki_supported_cplx(int64_t k);
 int ozaki_gemm_real(const GemmArgsR &g, hipStream_t s);
 int ozaki_gemm_cplx(const GemmArgsZ &g, hipStream_t s);
 
-// Test hook: U[j*ldu + i] = ((sum_k A[i*kp + k] * B[j*kp + k]) mod m * y) mod m for one
+// Test hook: U[j*ldu + i] = (sum_k A[i*kp + k] * B[j*kp + k]) mod m for one
 // modulus of the table, computed by the production INT8 MFMA kernel of the given tile
 // variant (0..3).  mp and np must be multiples of 256, kp a multiple of 64.
 int ozaki_residue_gemm_test(const int8_t *dA, const int8_t *dB, uint8_t *dU, int64_t mp,

@@ -124,15 +124,13 @@ DDI void oz_split_unit(const T *__restrict__ X, int64_t ld, int64_t K,
     }
 }
 
-// u = ((v mod m) * y) mod m for an exact INT32 dot product v (|v| <= 2^30): shifted by
-// m*2^24 to a non-negative integer below 2^33, both quotients are exact with inv_up.
-DDI uint32_t crt_weight_residue(int v, const ModConst &mc)
+// u = v mod m for an exact INT32 dot product v (|v| <= 2^30): shifted by m*2^24 to a
+// non-negative integer below 2^33, the quotient is exact with inv_up.  (The CRT weight
+// y = (M/m)^-1 mod m lives in the reconstruction's fixed-point limbs.)
+DDI uint32_t crt_residue(int v, const ModConst &mc)
 {
-    const double m = mc.m;
     const double d = (double)v + mc.off_acc;
-    const double r = __builtin_fma(-floor(d * mc.inv_up), m, d);
-    const double w = r * mc.y;
-    return (uint32_t)__builtin_fma(-floor(w * mc.inv_up), m, w);
+    return (uint32_t)__builtin_fma(-floor(d * mc.inv_up), mc.m, d);
 }
 
 // Workgroup -> output tile.  Workgroups are dispatched round-robin over the 8 XCDs (each
@@ -153,14 +151,19 @@ DDI void tile_order(int bid, int tiles_m, int tiles_n, int &tm, int &tn)
     tn = r / gm;
 }
 
-// One BM x BN tile of a residue GEMM: U_b = ((A_b B_b^T) mod m * y) mod m, A_b: [Mp][kp] int8,
-// B_b: [Np][kp] int8, U_b column-major [Np][ldu] uint8.  Four wave64 per workgroup in a
-// 2 x 2 arrangement, each wave owns TM x TN MFMA tiles of 32 x 32 (v_mfma_i32_32x32x32_i8);
-// 64-deep K slabs double-buffered in LDS with 80-byte row pitch (conflict-free b128 reads).
+// One BM x BN tile of nmod residue GEMMs: for t < nmod, U_t = (A_t B_t^T) mod m_t, A_t = Ab + t*sA: [Mp][kp] int8, B_t = Bb + t*sB: [Np][kp] int8, U_t = Ub + t*sU
+// column-major [Np][ldu] uint8.  Four wave64 per workgroup in a 2 x 2 arrangement, each wave
+// owns TM x TN MFMA tiles of 32 x 32 (v_mfma_i32_32x32x32_i8); 64-deep K slabs
+// double-buffered in LDS with 80-byte row pitch (conflict-free b128 reads).  The slabs of
+// all nmod products form one sequence, so the pipeline runs straight across modulus
+// boundaries: the next modulus's first slab is in flight while the previous modulus's
+// accumulators are reduced and stored (short K, as in batches of small problems, then
+// costs no pipeline refill per modulus).
 template <int TM, int TN, int BK = 64>
 DDI void oz_gemm_tile(const int8_t *__restrict__ Ab, const int8_t *__restrict__ Bb,
-                      uint8_t *__restrict__ Ub, int64_t kp, int64_t ldu, const ModConst &mc,
-                      int64_t i0, int64_t j0, int8_t (&sa)[2][64 * TM * (BK + 16)],
+                      uint8_t *__restrict__ Ub, int64_t kp, int64_t ldu,
+                      const ModConst *__restrict__ mcs, int nmod, int64_t sA, int64_t sB,
+                      int64_t sU, int64_t i0, int64_t j0, int8_t (&sa)[2][64 * TM * (BK + 16)],
                       int8_t (&sb)[2][64 * TN * (BK + 16)])
 {
     // Row pitch BK+16 bytes: 16 consecutive rows land in 16 distinct 16-byte bank slots
@@ -171,15 +174,15 @@ DDI void oz_gemm_tile(const int8_t *__restrict__ Ab, const int8_t *__restrict__ 
     const int wm = w & 1, wn = w >> 1;
 
     v4i ra[LA], rb[LB];
-#define OZ_GLOAD(K0)                                                                          \
+#define OZ_GLOAD(PA, PB, K0)                                                                  \
     do {                                                                                      \
         _Pragma("unroll") for (int q = 0; q < LA; ++q) {                                      \
             const int c = tid + 256 * q;                                                      \
-            ra[q] = *reinterpret_cast<const v4i *>(Ab + (int64_t)(c / CPR) * kp + (K0) + (c % CPR) * 16); \
+            ra[q] = *reinterpret_cast<const v4i *>((PA) + (int64_t)(c / CPR) * kp + (K0) + (c % CPR) * 16); \
         }                                                                                     \
         _Pragma("unroll") for (int q = 0; q < LB; ++q) {                                      \
             const int c = tid + 256 * q;                                                      \
-            rb[q] = *reinterpret_cast<const v4i *>(Bb + (int64_t)(c / CPR) * kp + (K0) + (c % CPR) * 16); \
+            rb[q] = *reinterpret_cast<const v4i *>((PB) + (int64_t)(c / CPR) * kp + (K0) + (c % CPR) * 16); \
         }                                                                                     \
     } while (0)
 #define OZ_LSTORE(BUF)                                                                        \
@@ -195,18 +198,64 @@ DDI void oz_gemm_tile(const int8_t *__restrict__ Ab, const int8_t *__restrict__ 
     } while (0)
 
     v16i acc[TM][TN];
+    auto clear = [&]() {
 #pragma unroll
-    for (int a = 0; a < TM; ++a)
+        for (int a = 0; a < TM; ++a)
 #pragma unroll
-        for (int b = 0; b < TN; ++b) acc[a][b] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int b = 0; b < TN; ++b)
+                acc[a][b] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    };
+    // Reduce the accumulators of modulus t and store them, four rows per 32-bit word.
+    auto flush = [&](int t) {
+        const ModConst mc = mcs[t];
+        uint8_t *U = Ub + (int64_t)t * sU;
+#pragma unroll
+        for (int a = 0; a < TM; ++a)
+#pragma unroll
+            for (int b = 0; b < TN; ++b) {
+                const int64_t col = j0 + wn * 32 * TN + b * 32 + (lane & 31);
+                const int64_t rowb = i0 + wm * 32 * TM + a * 32 + 4 * (lane >> 5);
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    uint32_t word = 0;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        word |= crt_residue(acc[a][b][4 * g + q], mc) << (8 * q);
+                    *reinterpret_cast<uint32_t *>(U + col * ldu + rowb + 8 * g) = word;
+                }
+            }
+    };
+    clear();
 
     const int64_t nk = kp / BK;
-    OZ_GLOAD(0);
+    const int64_t S = nk * nmod;
+    // Load cursor (the next slab to fetch) and compute cursor, as running counters.
+    const int8_t *la = Ab, *lb = Bb;
+    int64_t lkt = 0;
+    auto advance = [&]() {
+        if (++lkt == nk) {
+            lkt = 0;
+            la += sA;
+            lb += sB;
+        }
+    };
+    OZ_GLOAD(la, lb, lkt * BK);
+    advance();
     OZ_LSTORE(0);
     __syncthreads();
-    for (int64_t kt = 0; kt < nk; ++kt) {
-        const int cur = (int)(kt & 1);
-        if (kt + 1 < nk) OZ_GLOAD((kt + 1) * BK);
+    int ct = 0;
+    int64_t ckt = 0;
+    for (int64_t s = 0; s < S; ++s) {
+        const int cur = (int)(s & 1);
+        const bool more = s + 1 < S;
+        if (more) {
+            OZ_GLOAD(la, lb, lkt * BK);
+            advance();
+        }
+        if (ckt == 0 && s > 0) {   // first slab of modulus ct: modulus ct-1 is complete
+            flush(ct - 1);
+            clear();
+        }
         // Fragments are double-buffered: the LDS reads of step ks+1 are issued before the
         // MFMAs of step ks (scheduling barriers keep that order), so LDS latency overlaps
         // the matrix-core work instead of stalling it.
@@ -245,27 +294,17 @@ DDI void oz_gemm_tile(const int8_t *__restrict__ Ab, const int8_t *__restrict__ 
                     acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[cb][a], bf[cb][b], acc[a][b], 0, 0, 0);
             __builtin_amdgcn_sched_barrier(0);
         }
-        if (kt + 1 < nk) OZ_LSTORE(cur ^ 1);
+        if (more) OZ_LSTORE(cur ^ 1);
         __syncthreads();
+        if (++ckt == nk) {
+            ckt = 0;
+            ++ct;
+        }
     }
 
 #undef OZ_GLOAD
 #undef OZ_LSTORE
-#pragma unroll
-    for (int a = 0; a < TM; ++a)
-#pragma unroll
-        for (int b = 0; b < TN; ++b) {
-            const int64_t col = j0 + wn * 32 * TN + b * 32 + (lane & 31);
-            const int64_t rowb = i0 + wm * 32 * TM + a * 32 + 4 * (lane >> 5);
-#pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                uint32_t word = 0;
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    word |= crt_weight_residue(acc[a][b][4 * g + q], mc) << (8 * q);
-                *reinterpret_cast<uint32_t *>(Ub + col * ldu + rowb + 8 * g) = word;
-            }
-        }
+    flush(nmod - 1);
     __syncthreads();   // the LDS slabs may be refilled by the caller's next tile
 }
 
@@ -285,7 +324,10 @@ DDI void oz_gemm_tile(const int8_t *__restrict__ Ab, const int8_t *__restrict__ 
 // so; the s_barrier then publishes them to the workgroup.  The barrier is bracketed by
 // compiler-only memory clobbers instead of a fence, because a workgroup fence would also
 // wait for the stages still in flight.
-constexpr int RING_NST = 4, RING_BK = 64, RING_SLOT = 2 * 256 * RING_BK;
+constexpr int RING_NST = 4, RING_BK = 64;
+// LDS bytes of one ring stage (the A and the B slab) for BT x BT tiles.
+template <int BT> constexpr int ring_slot() { return 2 * BT * RING_BK; }
+constexpr int RING_SLOT = ring_slot<256>();
 constexpr int RING_THREADS = 512;
 
 // Wait until at most N of this wave's vector-memory operations are outstanding
@@ -308,42 +350,60 @@ template <int Q> DDI void wait_stages(int n)
 }
 DDI void wait_lds() { __builtin_amdgcn_s_waitcnt(15 | (7 << 4) | (0 << 8) | (3 << 14)); }
 
+// The ring tile for a WM x WN grid of waves, each owning TM x TN MFMA tiles of 32 x 32: a
+// BT x BT tile (BT = 32*TM*WM = 32*TN*WN), 64*WM*WN threads.  <4,2,2,4> is the 256 x 256
+// tile of the single products (one workgroup per CU); <2,2,2,2> the 128 x 128 tile of the
+// batched products (two per CU).  With MULTI, the tile runs the products of nmod moduli
+// (A_t = Ab + t*sA, B_t = Bb + t*sB, U_t = Ub + t*sU, constants mcs[t]) as one sequence of
+// stages, so the ring stays full across modulus boundaries; the accumulators of a modulus
+// are reduced and stored while later stages' DMA is in flight.
+template <int WM, int WN, int TM, int TN, bool MULTI>
 DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restrict__ Bb,
-                           uint8_t *__restrict__ Ub, int64_t kp, int64_t ldu, const ModConst &mc,
-                           int64_t i0, int64_t j0, int8_t *smem)
+                           uint8_t *__restrict__ Ub, int64_t kp, int64_t ldu,
+                           const ModConst *__restrict__ mcs, int nmod, int64_t sA, int64_t sB,
+                           int64_t sU, int64_t i0, int64_t j0, int8_t *smem)
 {
-    constexpr int NST = RING_NST, BK = RING_BK, SLOT = RING_SLOT;
-    constexpr int WM = 4, TM = 2, TN = 4;       // waves along M; MFMA tiles per wave
-    constexpr int DQ = 2;                       // DMA instructions per operand, wave and stage
+    constexpr int BT = 32 * TM * WM, NW = WM * WN;
+    static_assert(BT == 32 * TN * WN, "square tile");
+    constexpr int NST = RING_NST, BK = RING_BK, SLOT = ring_slot<BT>();
+    constexpr int DQ = BT / 16 / NW;            // DMA instructions per operand, wave and stage
+    static_assert(DQ * NW * 16 == BT, "DMA rows");
     constexpr int NR = TM + TN, NM = TM * TN;   // fragment reads and MFMAs per 32-deep step
-    static_assert(RING_THREADS == 64 * WM * (256 / (32 * TN)), "wave grid");
-    static_assert(DQ * (RING_THREADS / 64) * 16 == 256, "DMA rows");
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wm = w % WM, wn = w / WM;
     const int64_t nk = kp / BK;
+    const int64_t S = MULTI ? nk * nmod : nk;   // stages of the whole sequence
     const int drow = lane >> 2, dslot = lane & 3;
 
-    // DMA of stage kt into its ring slot: instruction q of wave w covers rows
-    // (2w + q) * 16 + lane / 4 of both panels.
-    auto issue = [&](int64_t kt) {
-        int8_t *slot = smem + (int)(kt % NST) * SLOT;
+    // DMA of the next stage into its ring slot (stages are issued in order, so the source
+    // is a running cursor: modulus base pointers and the K slab within the modulus).
+    // Instruction q of wave w covers rows (DQ*w + q) * 16 + lane / 4 of both panels.
+    const int8_t *ca = Ab, *cb = Bb;
+    int64_t ck = 0;
+    auto issue = [&](int64_t s) {
+        int8_t *slot = smem + (int)(s % NST) * SLOT;
 #pragma unroll
         for (int q = 0; q < DQ; ++q) {
             const int blk = DQ * w + q;
             const int r = blk * 16 + drow;
             const int c = dslot ^ ((r >> 2) & 3);
             __builtin_amdgcn_global_load_lds(
-                (const __attribute__((address_space(1))) void *)(Ab + (int64_t)r * kp + kt * BK + c * 16),
+                (const __attribute__((address_space(1))) void *)(ca + (int64_t)r * kp + ck * BK + c * 16),
                 (__attribute__((address_space(3))) void *)(slot + blk * 1024), 16, 0, 0);
             __builtin_amdgcn_global_load_lds(
-                (const __attribute__((address_space(1))) void *)(Bb + (int64_t)r * kp + kt * BK + c * 16),
-                (__attribute__((address_space(3))) void *)(slot + 256 * BK + blk * 1024), 16, 0, 0);
+                (const __attribute__((address_space(1))) void *)(cb + (int64_t)r * kp + ck * BK + c * 16),
+                (__attribute__((address_space(3))) void *)(slot + BT * BK + blk * 1024), 16, 0, 0);
+        }
+        if (++ck == nk && MULTI) {
+            ck = 0;
+            ca += sA;
+            cb += sB;
         }
     };
     v4i af[2][TM], bf[2][TN];
-    // Fragments of 32-deep step ks (0 or 1) of stage kt into fragment buffer d.
-    auto frag = [&](int64_t kt, int ks, int d) {
-        const int8_t *sl = smem + (int)(kt % NST) * SLOT;
+    // Fragments of 32-deep step ks (0 or 1) of stage s into fragment buffer d.
+    auto frag = [&](int64_t s, int ks, int d) {
+        const int8_t *sl = smem + (int)(s % NST) * SLOT;
         const int c = 2 * ks + (lane >> 5);
 #pragma unroll
         for (int a = 0; a < TM; ++a) {
@@ -353,14 +413,17 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
 #pragma unroll
         for (int b = 0; b < TN; ++b) {
             const int r = wn * 32 * TN + b * 32 + (lane & 31);
-            bf[d][b] = *reinterpret_cast<const v4i *>(sl + 256 * BK + r * 64 + ((c ^ ((r >> 2) & 3)) << 4));
+            bf[d][b] = *reinterpret_cast<const v4i *>(sl + BT * BK + r * 64 + ((c ^ ((r >> 2) & 3)) << 4));
         }
     };
     v16i acc[TM][TN];
+    auto clear = [&]() {
 #pragma unroll
-    for (int a = 0; a < TM; ++a)
+        for (int a = 0; a < TM; ++a)
 #pragma unroll
-        for (int b = 0; b < TN; ++b) acc[a][b] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int b = 0; b < TN; ++b)
+                acc[a][b] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    };
     auto mma = [&](int d) {
 #pragma unroll
         for (int a = 0; a < TM; ++a)
@@ -386,8 +449,40 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
         __builtin_amdgcn_s_barrier();
         asm volatile("" ::: "memory");
     };
+    // Reduce the accumulators of modulus t and store them, four rows per 32-bit word.
+    auto flush = [&](int t) {
+        const ModConst mc = mcs[t];
+        uint8_t *U = Ub + (int64_t)t * sU;
+#pragma unroll
+        for (int a = 0; a < TM; ++a)
+#pragma unroll
+            for (int b = 0; b < TN; ++b) {
+                const int64_t col = j0 + wn * 32 * TN + b * 32 + (lane & 31);
+                const int64_t rowb = i0 + wm * 32 * TM + a * 32 + 4 * (lane >> 5);
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    uint32_t word = 0;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        word |= crt_residue(acc[a][b][4 * g + q], mc) << (8 * q);
+                    *reinterpret_cast<uint32_t *>(U + col * ldu + rowb + 8 * g) = word;
+                }
+            }
+    };
+    // Compute cursor (MULTI): modulus and K slab of the stage being finished.
+    int ct = 0;
+    int64_t ckt = 0;
+    auto stage_done = [&]() {
+        if (MULTI && ++ckt == nk) {
+            flush(ct);
+            clear();
+            ckt = 0;
+            ++ct;
+        }
+    };
+    clear();
 
-    const int64_t npro = nk < NST ? nk : NST;
+    const int64_t npro = S < NST ? S : NST;
     for (int64_t s = 0; s < npro; ++s) issue(s);
     stage_barrier((int)(npro - 1));
     frag(0, 0, 0);
@@ -395,7 +490,7 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
     // Steady state.  Step 0 of stage kt runs while step 1's fragments are read; the barrier
     // before step 1 frees slot kt (its last fragments are in registers), which is refilled
     // with stage kt + NST while step 1 runs and stage kt + 1's first fragments are read.
-    for (; kt + NST < nk; ++kt) {
+    for (; kt + NST < S; ++kt) {
         frag(kt, 1, 1);
         mma(0);
         interleave(false);
@@ -404,39 +499,26 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
         frag(kt + 1, 0, 0);
         mma(1);
         interleave(true);
+        stage_done();
     }
     // Drain: the last NST - 1 stage transitions issue no refills.
-    for (; kt + 1 < nk; ++kt) {
+    for (; kt + 1 < S; ++kt) {
         frag(kt, 1, 1);
         mma(0);
         interleave(false);
-        const int64_t later = nk - kt - 2;
+        const int64_t later = S - kt - 2;
         stage_barrier((int)(later < NST - 2 ? later : NST - 2));
         frag(kt + 1, 0, 0);
         mma(1);
         interleave(false);
+        stage_done();
     }
     frag(kt, 1, 1);
     mma(0);
     interleave(false);
     wait_lds();
     mma(1);
-
-#pragma unroll
-    for (int a = 0; a < TM; ++a)
-#pragma unroll
-        for (int b = 0; b < TN; ++b) {
-            const int64_t col = j0 + wn * 32 * TN + b * 32 + (lane & 31);
-            const int64_t rowb = i0 + wm * 32 * TM + a * 32 + 4 * (lane >> 5);
-#pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                uint32_t word = 0;
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    word |= crt_weight_residue(acc[a][b][4 * g + q], mc) << (8 * q);
-                *reinterpret_cast<uint32_t *>(Ub + col * ldu + rowb + 8 * g) = word;
-            }
-        }
+    flush(MULTI ? nmod - 1 : 0);
     __syncthreads();   // the ring may be refilled by the caller's next tile
 }
 

@@ -265,8 +265,8 @@ MWGEMM_API int mwgemm_ozaki_set(int bits, int nmod);
 MWGEMM_API int mwgemm_ozaki_moduli(int nmod, int *m, int *y);
 
 /* Test hook: runs the production INT8 MFMA residue kernel once on device buffers,
- * U[j*mp + i] = ((sum_k A[i*kp + k] * B[j*kp + k]) mod m * y) mod m with m, y the
- * mod_index-th modulus and weight of the full 40-modulus table.  mp, np multiples of 256,
+ * U[j*mp + i] = (sum_k A[i*kp + k] * B[j*kp + k]) mod m with m the mod_index-th
+ * modulus of the full 40-modulus table.  mp, np multiples of 256,
  * kp a multiple of 64.  Returns 0 or an error code. */
 MWGEMM_API int mwgemm_testing_residue_gemm(const void *dA, const void *dB, void *dU, int64_t mp,
                                            int64_t np, int64_t kp, int mod_index, int variant,

@@ -47,27 +47,38 @@ def test_depth_limit_is_rejected():
 
 
 def test_fixed_point_crt_reconstruction_identity():
-    """C'/M == frac(sum_t u_t/m_t), u_t = (C' mod m_t) * y_t mod m_t, evaluated as
-    sum_t u_t * round(2^128/m_t) mod 2^128: the error is below M * 2^-115 for every |C'|
-    up to M/2 * (1 - 2^-100) (the plan keeps |C'| below M/2 by a factor 1 - 7e-7 or more)."""
+    """C'/M == frac(sum_t u_t y_t/m_t), u_t = C' mod m_t (what the residue GEMMs store),
+    evaluated as sum_t u_t * W_t mod 2^128 with the weight-folded limbs W_t =
+    round(2^128 y_t/m_t) mod 2^128: the error is below M * 2^-115 for every |C'| up to
+    M/2 * (1 - 2^-100) (the plan keeps |C'| below M/2 by a factor 1 - 7e-7 or more).  The
+    unfolded form (u_t = (C' mod m_t) y_t mod m_t with round(2^128/m_t)) is checked too:
+    both give C' to the same bound, so dropping y_t from one side only would fail."""
     nmod = 29
     mm, yy = mwblas.ozaki_moduli(nmod)
     mm = [int(v) for v in mm]
     yy = [int(v) for v in yy]
     M = math.prod(mm)
     W = [(2 ** 128 + m // 2) // m for m in mm]
+    Wy = [((2 ** 129 * y + m) // (2 * m)) % 2 ** 128 for m, y in zip(mm, yy)]
     rng = random.Random(5)
     lim = M // 2 - M // 2 ** 101
+
+    def recon(u, weights):
+        F = sum(ut * w for ut, w in zip(u, weights)) % 2 ** 128
+        if F >= 2 ** 127:
+            F -= 2 ** 128
+        return F * M / 2 ** 128       # exact rational
+
     for _ in range(2000):
         X = rng.randrange(-lim, lim + 1)
         if rng.random() < 0.1:
             X = rng.choice([0, 1, -1, lim, -lim])
-        u = [((X % m) * y) % m for m, y in zip(mm, yy)]
-        F = sum(ut * w for ut, w in zip(u, W)) % 2 ** 128
-        if F >= 2 ** 127:
-            F -= 2 ** 128
-        est = F * M / 2 ** 128       # exact rational
-        assert abs(est - X) <= M / 2 ** 115
+        plain = [X % m for m in mm]
+        assert abs(recon(plain, Wy) - X) <= M / 2 ** 115
+        assert abs(recon([(u * y) % m for u, y, m in zip(plain, yy, mm)], W) - X) <= M / 2 ** 115
+    # dropping the weight on one side is caught
+    X = lim // 3
+    assert abs(recon([X % m for m in mm], W) - X) > M / 2 ** 20
 
 
 def _inv_up(m):
