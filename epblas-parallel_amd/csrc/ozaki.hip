@@ -361,6 +361,11 @@ namespace {
 struct OzDevice {
     void *ws = nullptr;
     size_t ws_bytes = 0;
+    // Completion of the last call that used ws, and its stream: a call on another stream
+    // waits for it (on the device) before its kernels overwrite the workspace.
+    hipEvent_t ws_done = nullptr;
+    hipStream_t ws_stream = nullptr;
+    bool ws_recorded = false;
     std::map<int, ModConst *> tables;   // keyed by modulus count
 };
 std::mutex g_oz_mu;
@@ -377,10 +382,29 @@ int oz_device(OzDevice **out)
     return 0;
 }
 
+bool stream_capturing(hipStream_t s)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
+// The per-device workspace for a call on stream s, ordered after the previous call that
+// used it: on the same stream by stream order, on another stream by waiting (on the
+// device) for that call's completion event.  Under stream capture the workspace must
+// already be large enough (capture cannot free or synchronise), and captured work is
+// ordered only by how the graph is launched.
 int oz_workspace(OzDevice *d, size_t bytes, hipStream_t s, char **base)
 {
+    const bool capturing = stream_capturing(s);
+    if (d->ws_recorded && d->ws_stream != s && !capturing)
+        OZ_TRY(hipStreamWaitEvent(s, d->ws_done, 0));
     if (d->ws_bytes < bytes) {
+        if (capturing)
+            return record_error(-4, "ozaki: the workspace would have to grow to %zu bytes during "
+                                    "stream capture; run the call once outside capture first",
+                                bytes);
         if (d->ws) {
+            if (d->ws_recorded) OZ_TRY(hipEventSynchronize(d->ws_done));
             OZ_TRY(hipStreamSynchronize(s));
             OZ_TRY(hipFree(d->ws));
             d->ws = nullptr;
@@ -390,6 +414,17 @@ int oz_workspace(OzDevice *d, size_t bytes, hipStream_t s, char **base)
         d->ws_bytes = bytes;
     }
     *base = static_cast<char *>(d->ws);
+    return 0;
+}
+
+// Records the completion of the call just enqueued on s (after its last kernel).
+int oz_workspace_done(OzDevice *d, hipStream_t s)
+{
+    if (stream_capturing(s)) return 0;
+    if (!d->ws_done) OZ_TRY(hipEventCreateWithFlags(&d->ws_done, hipEventDisableTiming));
+    OZ_TRY(hipEventRecord(d->ws_done, s));
+    d->ws_stream = s;
+    d->ws_recorded = true;
     return 0;
 }
 
@@ -560,7 +595,7 @@ int ozaki_gemm_real(const GemmArgsR &g, hipStream_t s)
                            bpc, g.alpha, g.beta, g.beta_mode, g.C, g.ldc);
         OZ_TRY(hipGetLastError());
     }
-    return 0;
+    return oz_workspace_done(d, s);
 }
 
 int ozaki_gemm_cplx(const GemmArgsZ &g, hipStream_t s)
@@ -622,7 +657,7 @@ int ozaki_gemm_cplx(const GemmArgsZ &g, hipStream_t s)
                            bpc, g.alpha, g.beta, g.beta_mode, g.C, g.ldc);
         OZ_TRY(hipGetLastError());
     }
-    return 0;
+    return oz_workspace_done(d, s);
 }
 
 int ozaki_residue_gemm_test(const int8_t *dA, const int8_t *dB, uint8_t *dU, int64_t mp,

@@ -122,6 +122,40 @@ def test_special_values(gpu, ozaki_path):
     assert np.all(np.isfinite(C[:, mask, :]))                 # beta = 0 cleared C0's NaN
 
 
+def test_calls_on_two_streams_share_the_workspace_safely(gpu):
+    """The exact-product workspace (residues and residue products) is one per device.  Calls
+    alternating between two streams with no host synchronisation in between must each give
+    the result of the same call made alone: a call on another stream waits, on the device,
+    for the previous call's kernels before overwriting the workspace."""
+    import torch
+    rng = np.random.default_rng(77)
+    m = n = k = 1024
+    cases = [(ddgen.real_matrix(rng, m, k), ddgen.real_matrix(rng, k, n),
+              ddgen.real_matrix(rng, m, n), ddgen.real_scalar(rng, scale=2),
+              ddgen.real_scalar(rng, scale=2)) for _ in range(4)]
+    dev = [tuple(torch.from_numpy(x).to(gpu) for x in c[:3]) for c in cases]
+    ref = []
+    for (dA, dB, dC0), (_, _, _, al, be) in zip(dev, cases):
+        dC = dC0.clone()
+        mwblas.gemm_device(False, "N", "N", m, n, k, al, dA, m, dB, k, be, dC, m,
+                           path=mwblas.PATH_OZAKI, stream=torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        ref.append(dC.cpu().numpy())
+    streams = (torch.cuda.Stream(), torch.cuda.Stream())   # default priority
+    outs = [torch.empty_like(d[2]) for d in dev]
+    for rep in range(3):
+        print(f"two-stream round {rep}", flush=True)
+        for i, ((dA, dB, dC0), (_, _, _, al, be)) in enumerate(zip(dev, cases)):
+            st = streams[i % 2]
+            with torch.cuda.stream(st):
+                outs[i].copy_(dC0)
+                mwblas.gemm_device(False, "N", "N", m, n, k, al, dA, m, dB, k, be, outs[i], m,
+                                   path=mwblas.PATH_OZAKI, stream=st.cuda_stream)
+        torch.cuda.synchronize()
+        for i in range(len(cases)):
+            assert np.array_equal(outs[i].cpu().numpy(), ref[i]), (rep, i)
+
+
 def test_8192_wide_exponents_vs_binary128_and_plain(gpu):
     """The headline workload: M=N=K=8192, hi = +-U[1,2)*2^U_int[-20,20], alpha=1, beta=0.
     Ozaki and plain GPU results: 256 sampled entries each within 1e-30 of binary128."""
