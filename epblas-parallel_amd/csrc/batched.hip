@@ -309,6 +309,9 @@ struct BatchPlan {
     unsigned long long *rmax_base = nullptr;
     int64_t rmax_count = 0;
     int grid_persist = 0;
+    hipEvent_t done = nullptr;          // completion of the last run, and its stream: a run on
+    hipStream_t done_stream = nullptr;  // another stream waits for it (the runs share the
+    bool done_recorded = false;         // plan's work counters and workspace)
     int oz_tile = 128;                  // residue-GEMM tile edge (128 or 64)
     int mod_group = 0;                  // moduli per residue tile (all of them unless split)
     OzPlan oz{};
@@ -395,6 +398,7 @@ static void plan_free(BatchPlan *P)
 {
     if (!P) return;
     if (P->exec) (void)hipGraphExecDestroy(P->exec);
+    if (P->done) (void)hipEventDestroy(P->done);
     if (P->graph) (void)hipGraphDestroy(P->graph);
     if (P->dmem) (void)hipFree(P->dmem);
     if (P->mcs) (void)hipFree(P->mcs);
@@ -673,8 +677,24 @@ int mwgemm_batched_plan_create(mwgemm_batched_plan *plan, int64_t count, char tr
 int mwgemm_batched_plan_run(mwgemm_batched_plan plan, void *stream)
 {
     if (!plan || !plan->p || !plan->p->exec) return record_error(-3, "batched plan: invalid handle");
-    hipError_t e = hipGraphLaunch(plan->p->exec, static_cast<hipStream_t>(stream));
+    BatchPlan *P = plan->p;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    const bool capturing = hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+    if (!capturing && P->done_recorded && P->done_stream != s) {
+        hipError_t w = hipStreamWaitEvent(s, P->done, 0);
+        if (w != hipSuccess) return record_error((int)w, "hipStreamWaitEvent: %s", hipGetErrorString(w));
+    }
+    hipError_t e = hipGraphLaunch(P->exec, s);
     if (e != hipSuccess) return record_error((int)e, "hipGraphLaunch: %s", hipGetErrorString(e));
+    if (!capturing) {
+        if (!P->done && (e = hipEventCreateWithFlags(&P->done, hipEventDisableTiming)) != hipSuccess)
+            return record_error((int)e, "hipEventCreate: %s", hipGetErrorString(e));
+        if ((e = hipEventRecord(P->done, s)) != hipSuccess)
+            return record_error((int)e, "hipEventRecord: %s", hipGetErrorString(e));
+        P->done_stream = s;
+        P->done_recorded = true;
+    }
     return 0;
 }
 

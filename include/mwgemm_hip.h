@@ -212,7 +212,8 @@ MWGEMM_API int mwgemm_wgemv_device(char trans, int64_t m, int64_t n, const mw_co
  * HIP graph: the plain path is one persistent work-queue kernel over all output tiles
  * (results bit-identical to separate mgemm_ calls), the Ozaki path a fixed sequence of
  * flattened/persistent kernels.  plan_run replays the graph on a stream; the plan is bound
- * to the pointers it was created with.  mwgemm_mgemm_batched_device builds, runs (without
+ * to the pointers it was created with.  Runs of one plan share its work counters and
+ * workspace: a run on another stream than the previous run waits for it on the device.  mwgemm_mgemm_batched_device builds, runs (without
  * a graph) and frees a plan in one call and synchronises the stream.  Return 0 or an error
  * code.  No counterpart in the reference (its mgemm_ is one problem per call). */
 typedef struct mwgemm_batched_plan_s *mwgemm_batched_plan;

@@ -89,6 +89,33 @@ def test_plain_batched_graph_replays(gpu):
     plan.close()
 
 
+@pytest.mark.parametrize("path", [mwblas.PATH_PLAIN, mwblas.PATH_OZAKI], ids=["plain", "ozaki"])
+def test_plan_runs_alternating_streams(gpu, path):
+    """Runs of one plan share its work counters and workspace.  Three runs alternating
+    between two streams with no host synchronisation (each run accumulates into C, beta is
+    per problem) must leave C exactly as three runs on one stream do."""
+    import torch
+    rng = np.random.default_rng(34 + path)
+    probs = _problems(rng, 60, 1, 200)
+    dA, dB, dC = _device(probs, gpu)
+    plan = mwblas.BatchedPlan("N", "N", *_args(probs, dA, dB, dC), path=path)
+    one = torch.cuda.current_stream()
+    for _ in range(3):
+        plan.run(one.cuda_stream)
+    torch.cuda.synchronize()
+    expect = [c.cpu().numpy() for c in dC]
+    for i, p in enumerate(probs):
+        dC[i].copy_(torch.from_numpy(p["C0"]))
+    torch.cuda.synchronize()
+    streams = (torch.cuda.Stream(), torch.cuda.Stream())   # default priority
+    for rep in range(3):
+        plan.run(streams[rep % 2].cuda_stream)
+    torch.cuda.synchronize()
+    for i in range(len(probs)):
+        assert np.array_equal(dC[i].cpu().numpy(), expect[i], equal_nan=True), f"problem {i}"
+    plan.close()
+
+
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
 def test_ozaki_batched_vs_binary128(gpu, ta, tb):
     """Every operand layout of the residue split and maxima (row- and column-contiguous A
