@@ -14,7 +14,8 @@ Further objects on the same line (each with its own workload): "library_int8_gem
 vendor INT8 GEMM at one residue GEMM's shape, a reference point for the roofline),
 "batched" (2000 small GEMMs in one captured launch per path), "complex" (4096^3 with
 op(A)='C', both paths), "gemmtr" (the triangular-output GEMM, 4096 x 4096 upper triangle,
-k = 4096), "gemv" (16384 x 16384 matrix-vector products, HBM-bound), "host_boundary" (the
+k = 4096), "plain_orders" (the plain GEMM at 4096^3 in the reference's AVX2-kernel and
+scalar-kernel operation orders), "gemv" (16384 x 16384 matrix-vector products, HBM-bound), "host_boundary" (the
 drop-in mgemm_ with host operands, PCIe transfers inside the timing) and "serial_trailing"
 (a blocked SYRK's trailing updates through mgemm_serial on host arrays).
 
@@ -250,6 +251,43 @@ def gemmtr_bench(dev, n=4096, k=4096, reps=3, seed=41):
             # 18 FP64 lane-ops per DD multiply-add against 39.3e12 lane-ops/s
             "fp64_frac": round(18 * macs / t / 1e12 / (FP64_PEAK_TFLOPS / 2), 4),
             "kernels_ms_per_call": {kn: round(v[0] / reps, 3) for kn, v in kt.items()}}
+
+
+def plain_orders_bench(dev, n=4096, reps=2, seed=47):
+    """The plain real GEMM at n^3 (NN, random alpha/beta, inputs in HBM) in both operation
+    orders the reference has: its AVX2/FMA kernels (18 FP64 lane-ops per DD multiply-add)
+    and its scalar kernels for hosts without AVX2/FMA (29: a faithful multiply and a
+    faithful add per multiply-add).  fp64_frac = lane-ops / t against 39.3e12 lane-ops/s."""
+    import numpy as np
+    import torch
+    import mwblas
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    A, B, C0 = gen_uniform(n, n, gen, dev), gen_uniform(n, n, gen, dev), gen_uniform(n, n, gen, dev)
+    rng = np.random.default_rng(seed)
+    alpha, beta = random_scalar(rng, 1), random_scalar(rng, 1)
+    stream = torch.cuda.current_stream().cuda_stream
+    out = {"workload": f"plain real DD GEMM NN {n}^3, random alpha/beta"}
+    prev = mwblas.get_plain_order()
+    try:
+        for name, order, ops in (("simd", mwblas.ORDER_SIMD, 18), ("scalar", mwblas.ORDER_SCALAR, 29)):
+            mwblas.set_plain_order(order)
+            C = C0.clone()
+            mwblas.gemm_device(False, "N", "N", n, n, n, alpha, A, n, B, n, beta, C, n,
+                               path=mwblas.PATH_PLAIN, stream=stream)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                mwblas.gemm_device(False, "N", "N", n, n, n, alpha, A, n, B, n, beta, C, n,
+                                   path=mwblas.PATH_PLAIN, stream=stream)
+            torch.cuda.synchronize()
+            t = (time.perf_counter() - t0) / reps
+            out[name] = {"value": round(2.0 * n ** 3 / t / 1e9, 2), "unit": "DD-GFLOP/s",
+                         "ms_per_call": round(t * 1e3, 2),
+                         "fp64_frac": round(ops * n ** 3 / t / 1e12 / (FP64_PEAK_TFLOPS / 2), 4)}
+    finally:
+        mwblas.set_plain_order(prev)
+    return out
 
 
 def gemv_bench(dev, n=16384, reps=5, seed=43):
@@ -548,6 +586,7 @@ def main():
         result["complex"] = complex_bench(dev)
     if not args.no_gemmtr:
         result["gemmtr"] = gemmtr_bench(dev)
+        result["plain_orders"] = plain_orders_bench(dev)
     if not args.no_gemv:
         result["gemv"] = gemv_bench(dev)
     if not args.no_host:

@@ -19,6 +19,8 @@ ARCH = "gfx950"
 
 SOURCES = ["api.cpp", "host_copy.cpp", "plain_gemm.hip", "gemv.hip", "ozaki.hip", "batched.hip",
            "autotune.cpp"]
+# Pure host sources (no HIP constructs): compiled as C++ for the host only.
+HOST_ONLY = {"host_copy.cpp"}
 
 CFLAGS = [
     "-O3", f"--offload-arch={ARCH}", "-std=c++17", "-fPIC", "-ffp-contract=off",
@@ -32,7 +34,9 @@ CFLAGS = [
 def _compile(src: str) -> str:
     obj = os.path.join(OUT, os.path.splitext(src)[0] + ".o")
     cmd = [HIPCC, *CFLAGS]
-    if src.endswith(".cpp"):
+    if src in HOST_ONLY:
+        cmd = [HIPCC, *[f for f in CFLAGS if not f.startswith("--offload-arch")], "-x", "c++"]
+    elif src.endswith(".cpp"):
         cmd += ["-x", "hip"]
     cmd += ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)

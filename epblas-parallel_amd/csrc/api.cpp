@@ -102,6 +102,7 @@ std::mutex g_state_mu;
 std::deque<DevState> g_dev;   // a deque keeps each device's state at a fixed address
 std::mutex g_call_mu;        // host-pointer entry points are serialised
 std::atomic<int> g_path{-1};
+std::atomic<int> g_order{-1};
 
 int current_device(int *dev)
 {
@@ -303,6 +304,25 @@ int default_path()
     return p;
 }
 
+}  // namespace
+
+// Operation order of the plain path, so that results match the reference build this
+// library stands in for: MWGEMM_PLAIN_ORDER ("simd" | "scalar"), else what the reference
+// picks on this host.
+int plain_order()
+{
+    int o = g_order.load();
+    if (o < 0) {
+        const char *e = getenv("MWGEMM_PLAIN_ORDER");
+        if (e && strcmp(e, "scalar") == 0) o = ORDER_SCALAR;
+        else if (e && strcmp(e, "simd") == 0) o = ORDER_SIMD;
+        else o = host_has_avx2_fma() ? ORDER_SIMD : ORDER_SCALAR;
+        g_order.store(o);
+    }
+    return o;
+}
+
+namespace {
 // Letters: returns OP_N/OP_T/OP_C or -1.
 int op_real(char c)
 {
@@ -357,6 +377,7 @@ int choose_path(int path, int64_t m, int64_t n, int64_t k, bool cplx)
 // ----------------------------------------------------------------------------------
 int gemm_real_device(GemmArgsR g, int path, hipStream_t s)
 {
+    g.order = plain_order();
     if (g.m <= 0 || g.n <= 0) return 0;
     if (g.opa < 0 || g.opb < 0) return record_error(-2, "mgemm: invalid trans letter");
     const int64_t arows = g.opa == OP_N ? g.m : g.k;
@@ -400,6 +421,7 @@ int uplo_upper(char uplo)
 // is bit-for-bit the reference's).
 int gemmtr_real_device(GemmArgsR g, char uplo, hipStream_t s)
 {
+    g.order = plain_order();
     const int upper = uplo_upper(uplo);
     if (upper < 0) return record_error(-2, "mgemmtr: invalid uplo letter");
     if (g.opa < 0 || g.opb < 0) return record_error(-2, "mgemmtr: invalid trans letter");
@@ -417,6 +439,7 @@ int gemmtr_real_device(GemmArgsR g, char uplo, hipStream_t s)
 
 int gemmtr_cplx_device(GemmArgsZ g, char uplo, hipStream_t s)
 {
+    g.order = plain_order();
     const int upper = uplo_upper(uplo);
     if (upper < 0) return record_error(-2, "wgemmtr: invalid uplo letter");
     if (g.opa < 0 || g.opb < 0) return record_error(-2, "wgemmtr: invalid trans letter");
@@ -434,6 +457,7 @@ int gemmtr_cplx_device(GemmArgsZ g, char uplo, hipStream_t s)
 
 int gemm_cplx_device(GemmArgsZ g, int path, hipStream_t s)
 {
+    g.order = plain_order();
     if (g.m <= 0 || g.n <= 0) return 0;
     if (g.opa < 0 || g.opb < 0) return record_error(-2, "wgemm: invalid trans letter");
     const int64_t arows = g.opa == OP_N ? g.m : g.k;
@@ -758,7 +782,8 @@ int mgemv_host(char trans, int64_t m, int64_t n, const mw_float64x2 *alpha,
                             reinterpret_cast<const ddv *>(x), span_of(notrans ? n : m, incx),
                             reinterpret_cast<ddv *>(y), span_of(notrans ? m : n, incy),
                             [&](const ddv *gA, int64_t glda, const ddv *gx, ddv *gy, hipStream_t s) {
-                                return gemv_real(op, m, n, al, be, gA, glda, gx, incx, gy, incy, nb, s);
+                                return gemv_real(op, m, n, al, be, gA, glda, gx, incx, gy, incy, nb,
+                                                 plain_order(), s);
                             });
 }
 
@@ -777,7 +802,8 @@ int wgemv_host(char trans, int64_t m, int64_t n, const mw_complex64x2 *alpha,
                              reinterpret_cast<zddv *>(y), span_of(notrans ? m : n, incy),
                              [&](const zddv *gA, int64_t glda, const zddv *gx, zddv *gy,
                                  hipStream_t s) {
-                                 return gemv_cplx(op, m, n, al, be, gA, glda, gx, incx, gy, incy, s);
+                                 return gemv_cplx(op, m, n, al, be, gA, glda, gx, incx, gy, incy,
+                                                  plain_order(), s);
                              });
 }
 
@@ -1027,7 +1053,7 @@ int mwgemm_mgemv_device(char trans, int64_t m, int64_t n, const mw_float64x2 *al
     MW_TRY(gemv_real(op, m, n, ddv{alpha->limbs[0], alpha->limbs[1]},
                      ddv{beta->limbs[0], beta->limbs[1]}, reinterpret_cast<const ddv *>(dA), lda,
                      reinterpret_cast<const ddv *>(dx), incx, reinterpret_cast<ddv *>(dy), incy,
-                     nbands, s));
+                     nbands, plain_order(), s));
     return 0;
 }
 
@@ -1044,7 +1070,8 @@ int mwgemm_wgemv_device(char trans, int64_t m, int64_t n, const mw_complex64x2 *
     const zddv al{{alpha->re.limbs[0], alpha->re.limbs[1]}, {alpha->im.limbs[0], alpha->im.limbs[1]}};
     const zddv be{{beta->re.limbs[0], beta->re.limbs[1]}, {beta->im.limbs[0], beta->im.limbs[1]}};
     MW_TRY(gemv_cplx(op, m, n, al, be, reinterpret_cast<const zddv *>(dA), lda,
-                     reinterpret_cast<const zddv *>(dx), incx, reinterpret_cast<zddv *>(dy), incy, s));
+                     reinterpret_cast<const zddv *>(dx), incx, reinterpret_cast<zddv *>(dy), incy,
+                     plain_order(), s));
     return 0;
 }
 
@@ -1057,6 +1084,16 @@ int mwgemm_set_path(int path)
 }
 
 int mwgemm_get_path(void) { return default_path(); }
+
+int mwgemm_set_plain_order(int order)
+{
+    if (order != MWGEMM_ORDER_SIMD && order != MWGEMM_ORDER_SCALAR) return -1;
+    const int prev = plain_order();
+    g_order.store(order);
+    return prev;
+}
+
+int mwgemm_get_plain_order(void) { return plain_order(); }
 
 int mwgemm_last_error(char *buf, size_t len)
 {

@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/mwgemm_hip.h"
@@ -81,7 +82,8 @@ void k_batch_fill(unsigned long long *p, int64_t n, int *counters, int ncounters
 // Plain persistent kernel.
 // ---------------------------------------------------------------------------------
 // Two workgroups per CU (at most 256 VGPRs): the tile routine alone sits just below that.
-template <bool TA, bool TB>
+// SCALAR: the reference's scalar-kernel operation order (plain_tile.hpp), as for mgemm_.
+template <bool TA, bool TB, bool SCALAR>
 __global__ __launch_bounds__(256, 2)
 void k_batch_plain(const BatchProb *__restrict__ probs, const int4 *__restrict__ tiles,
                    int ntiles, int *__restrict__ counter)
@@ -101,7 +103,7 @@ void k_batch_plain(const BatchProb *__restrict__ probs, const int4 *__restrict__
         const BatchProb &p = probs[tr.x];
         const int64_t i0 = (int64_t)tr.y * 64, j0 = (int64_t)tr.z * 64;
         if (p.active) {
-            mgemm_plain_tile<4, 4, 16, TA, TB>(p.m, p.n, p.k, p.alpha, p.beta, p.beta_mode, p.A,
+            mgemm_plain_tile<4, 4, 16, TA, TB, TRI_NONE, SCALAR>(p.m, p.n, p.k, p.alpha, p.beta, p.beta_mode, p.A,
                                                p.lda, p.B, p.ldb, p.C, p.ldc, i0, j0, As, Bs);
         } else {
             const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
@@ -296,6 +298,7 @@ unsigned blocks_for(int64_t units) { return (unsigned)((units + 255) / 256); }
 
 struct BatchPlan {
     int path = 0;
+    int order = ORDER_SIMD;             // plain path: operation order (as for mgemm_)
     int ta = 0, tb = 0;                 // 1 = transposed
     int count = 0;
     int device = 0;
@@ -333,14 +336,20 @@ static int batch_enqueue(BatchPlan *P, hipStream_t s)
         B_TRY(hipGetLastError());
         if (P->ntiles == 0) return 0;
         const dim3 g((unsigned)P->grid_persist), b(256);
-        if (!P->ta && !P->tb)
-            hipLaunchKernelGGL((k_batch_plain<false, false>), g, b, 0, s, P->probs, P->tiles, P->ntiles, P->counters);
-        else if (P->ta && !P->tb)
-            hipLaunchKernelGGL((k_batch_plain<true, false>), g, b, 0, s, P->probs, P->tiles, P->ntiles, P->counters);
-        else if (!P->ta && P->tb)
-            hipLaunchKernelGGL((k_batch_plain<false, true>), g, b, 0, s, P->probs, P->tiles, P->ntiles, P->counters);
-        else
-            hipLaunchKernelGGL((k_batch_plain<true, true>), g, b, 0, s, P->probs, P->tiles, P->ntiles, P->counters);
+        auto go = [&](auto ta, auto tb) {
+            if (P->order == ORDER_SCALAR)
+                hipLaunchKernelGGL((k_batch_plain<decltype(ta)::value, decltype(tb)::value, true>), g,
+                                   b, 0, s, P->probs, P->tiles, P->ntiles, P->counters);
+            else
+                hipLaunchKernelGGL((k_batch_plain<decltype(ta)::value, decltype(tb)::value, false>), g,
+                                   b, 0, s, P->probs, P->tiles, P->ntiles, P->counters);
+        };
+        using F = std::false_type;
+        using T = std::true_type;
+        if (!P->ta && !P->tb) go(F{}, F{});
+        else if (P->ta && !P->tb) go(T{}, F{});
+        else if (!P->ta && P->tb) go(F{}, T{});
+        else go(T{}, T{});
         B_TRY(hipGetLastError());
         return 0;
     }
@@ -446,6 +455,7 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
 
     BatchPlan *P = new BatchPlan();
     P->path = path;
+    P->order = plain_order();
     P->ta = ta;
     P->tb = tb;
     P->count = (int)count;

@@ -8,13 +8,16 @@
 //   the x_j == 0 skip flags and a per-batch "any skip" flag for 1024 columns at a time in
 //   LDS; A is read column by column with consecutive lanes on consecutive rows.  With m
 //   rows there are only m/64 waves, so op 'N' is bound by the FP64 issue rate of the SIMDs
-//   it occupies rather than by HBM once m is below ~64 rows per SIMD.  Rows in whole 4-blocks of their band take the
-//   fast operations, the band tails the faithful ones (the band layout is the reference's
-//   OpenMP row split for mgemv, and "whole 4-blocks, last m % 4 rows" for wgemv).
+//   it occupies rather than by HBM once m is below ~64 rows per SIMD.  Rows in whole
+//   4-blocks of their band take the fast operations, the band tails the faithful ones (the
+//   band layout is the reference's OpenMP row split for mgemv, and "whole 4-blocks, last
+//   m % 4 rows" for wgemv).
 // op 'T'/'C': four lanes per output column, lane L accumulating the rows i = L (mod 4)
 //   below m - m%4 with the fast operations, reduced across the four lanes in the
 //   reference's order (s0+s2, s1+s3, then their sum); lane 0 adds the last m % 4 rows and
 //   alpha*s faithfully.
+// In the scalar order (the reference's kernels for hosts without AVX2/FMA) every op-'N' row
+// takes the faithful operations, and op 'T'/'C' is one sequential faithful dot per column.
 // beta is applied to y in the same kernel, before the accumulation (mf_kernels.h:63-74).
 #include "mw_internal.hpp"
 #include "timer.hpp"
@@ -150,7 +153,7 @@ template <typename V>
 __global__ __launch_bounds__(GEMV_NB)
 void k_gemv_n(int64_t m, int64_t n, V alpha, V beta, int beta_mode, int alpha0,
               const V *__restrict__ A, int64_t lda, const V *__restrict__ x, int64_t incx,
-              V *__restrict__ y, int64_t incy, int64_t nbands)
+              V *__restrict__ y, int64_t incy, int64_t nbands, int all_tail)
 {
     constexpr int PF = pf_n<V>();
     __shared__ V ts[GEMV_CHUNK];
@@ -159,7 +162,8 @@ void k_gemv_n(int64_t m, int64_t n, V alpha, V beta, int beta_mode, int alpha0,
     const int64_t i = (int64_t)blockIdx.x * GEMV_NB + threadIdx.x;
     const bool live = i < m;
     const int64_t by = vec_base(m, incy), bx = vec_base(n, incx);
-    const bool tail = live && (nbands > 0 ? band_tail(i, m, nbands) : i >= m / 4 * 4);
+    // all_tail: the scalar order, every row takes the faithful operations
+    const bool tail = live && (all_tail || (nbands > 0 ? band_tail(i, m, nbands) : i >= m / 4 * 4));
     V yi = zero_of(alpha);
     if (live) yi = prepass(y[by + i * incy], beta, beta_mode);
     if (!alpha0) {
@@ -249,15 +253,66 @@ void k_gemv_t(int64_t m, int64_t n, V alpha, V beta, int beta_mode, int alpha0,
     }
 }
 
+// op 'T' / 'C' in the scalar order (the reference's kernels for hosts without AVX2/FMA):
+// one sequential faithful chain per column, s := s + a_ij * x_i for i ascending, then
+// y_j := y_j + alpha * s (mgemv.cpp:213-223, wgemv.cpp:284-295).  A one-wave workgroup owns
+// 64 columns; blocks of ROWS rows of them are staged in LDS by coalesced column reads
+// (consecutive lanes on consecutive rows), then each lane walks its own column.
+template <typename V, bool CONJ>
+__global__ __launch_bounds__(64)
+void k_gemv_t_seq(int64_t m, int64_t n, V alpha, V beta, int beta_mode, int alpha0,
+                  const V *__restrict__ A, int64_t lda, const V *__restrict__ x, int64_t incx,
+                  V *__restrict__ y, int64_t incy)
+{
+    constexpr int ROWS = sizeof(V) == sizeof(ddv) ? 32 : 16;
+    __shared__ V tile[64][ROWS + 1];   // [column][row]; the odd pitch spreads the banks
+    __shared__ V xs[ROWS];
+    const int lane = threadIdx.x;
+    const int64_t j0 = (int64_t)blockIdx.x * 64, j = j0 + lane;
+    const bool live = j < n;
+    const int64_t bx = vec_base(m, incx), by = vec_base(n, incy);
+    V s = zero_of(alpha);
+    if (!alpha0) {
+        const int cols = (int)(n - j0 < 64 ? n - j0 : 64);
+        for (int64_t r0 = 0; r0 < m; r0 += ROWS) {
+            const int rows = (int)(m - r0 < ROWS ? m - r0 : ROWS);
+            __syncthreads();   // the previous block has been consumed
+            if (lane < rows) {
+                for (int c = 0; c < cols; ++c) tile[c][lane] = A[(j0 + c) * lda + r0 + lane];
+                xs[lane] = x[bx + (r0 + lane) * incx];
+            }
+            __syncthreads();
+            if (live)
+                for (int r = 0; r < rows; ++r)
+                    s = add_exact(s, mul_exact(conj_if(tile[lane][r], CONJ), xs[r]));
+        }
+    }
+    if (live) {
+        V yj = prepass(y[by + j * incy], beta, beta_mode);
+        if (!alpha0) yj = add_exact(yj, mul_exact(alpha, s));
+        y[by + j * incy] = yj;
+    }
+}
+
 template <typename V>
 hipError_t launch_gemv(int op, int64_t m, int64_t n, V alpha, V beta, int beta_mode, bool alpha0,
                        const V *A, int64_t lda, const V *x, int64_t incx, V *y, int64_t incy,
-                       int64_t nbands, hipStream_t s)
+                       int64_t nbands, int order, hipStream_t s)
 {
     ScopedTimer tm(KC_GEMV, s);
+    const int a0 = alpha0 ? 1 : 0;
     if (op == OP_N) {
         hipLaunchKernelGGL((k_gemv_n<V>), dim3((unsigned)((m + GEMV_NB - 1) / GEMV_NB)), dim3(GEMV_NB), 0, s, m, n,
-                           alpha, beta, beta_mode, alpha0 ? 1 : 0, A, lda, x, incx, y, incy, nbands);
+                           alpha, beta, beta_mode, a0, A, lda, x, incx, y, incy, nbands,
+                           order == ORDER_SCALAR ? 1 : 0);
+    } else if (order == ORDER_SCALAR) {
+        const dim3 g((unsigned)((n + 63) / 64));
+        if (op == OP_C)
+            hipLaunchKernelGGL((k_gemv_t_seq<V, true>), g, dim3(64), 0, s, m, n, alpha, beta,
+                               beta_mode, a0, A, lda, x, incx, y, incy);
+        else
+            hipLaunchKernelGGL((k_gemv_t_seq<V, false>), g, dim3(64), 0, s, m, n, alpha, beta,
+                               beta_mode, a0, A, lda, x, incx, y, incy);
     } else if (op == OP_C) {
         hipLaunchKernelGGL((k_gemv_t<V, true>), dim3((unsigned)((n + 63) / 64)), dim3(256), 0, s, m,
                            n, alpha, beta, beta_mode, alpha0 ? 1 : 0, A, lda, x, incx, y, incy);
@@ -272,23 +327,23 @@ hipError_t launch_gemv(int op, int64_t m, int64_t n, V alpha, V beta, int beta_m
 
 hipError_t gemv_real(int op, int64_t m, int64_t n, ddv alpha, ddv beta, const ddv *A, int64_t lda,
                      const ddv *x, int64_t incx, ddv *y, int64_t incy, int64_t nbands,
-                     hipStream_t s)
+                     int order, hipStream_t s)
 {
     if (m == 0 || n == 0) return hipSuccess;
     const int mode = is0(beta) ? BETA_ZERO : (beta.hi == 1.0 && beta.lo == 0.0) ? BETA_ONE : BETA_SCALE;
     return launch_gemv<ddv>(op == OP_C ? OP_T : op, m, n, alpha, beta, mode, is0(alpha), A, lda,
-                            x, incx, y, incy, nbands < 1 ? 1 : nbands, s);
+                            x, incx, y, incy, nbands < 1 ? 1 : nbands, order, s);
 }
 
 hipError_t gemv_cplx(int op, int64_t m, int64_t n, zddv alpha, zddv beta, const zddv *A,
-                     int64_t lda, const zddv *x, int64_t incx, zddv *y, int64_t incy,
+                     int64_t lda, const zddv *x, int64_t incx, zddv *y, int64_t incy, int order,
                      hipStream_t s)
 {
     if (m == 0 || n == 0) return hipSuccess;
     const bool b1 = beta.re.hi == 1.0 && beta.re.lo == 0.0 && is0(beta.im);
     const int mode = is0(beta) ? BETA_ZERO : b1 ? BETA_ONE : BETA_SCALE;
     return launch_gemv<zddv>(op, m, n, alpha, beta, mode, is0(alpha), A, lda, x, incx, y, incy,
-                             0, s);
+                             0, order, s);
 }
 
 }  // namespace mw

@@ -126,4 +126,14 @@ void host_copy_columns(void *dst, size_t dpitch, const void *src, size_t spitch,
     });
 }
 
+bool host_has_avx2_fma()
+{
+#if defined(__x86_64__)
+    __builtin_cpu_init();
+    return __builtin_cpu_supports("avx2") && __builtin_cpu_supports("fma");
+#else
+    return false;
+#endif
+}
+
 }  // namespace mw

@@ -19,6 +19,7 @@ struct GemmArgsR {
     const ddv *B; int64_t ldb;
     ddv *C; int64_t ldc;
     int variant = -1;        // tile variant; -1: autotuned choice or heuristic
+    int order = 0;           // plain path: ORDER_SIMD or ORDER_SCALAR (plain_tile.hpp)
 };
 
 struct GemmArgsZ {
@@ -30,7 +31,13 @@ struct GemmArgsZ {
     const zddv *B; int64_t ldb;
     zddv *C; int64_t ldc;
     int variant = -1;
+    int order = 0;
 };
+
+// Operation orders of the plain path: the reference's AVX2 kernel, or its scalar kernel
+// (hosts without AVX2/FMA).  Chosen per call from the library setting (api.cpp).
+enum PlainOrder { ORDER_SIMD = 0, ORDER_SCALAR = 1 };
+int plain_order();   // the current setting
 
 // Autotuner table (autotune.cpp).  Kinds of tuned decisions:
 enum TuneKind {
@@ -53,12 +60,12 @@ hipError_t plain_gemm_cplx(const GemmArgsZ &g, hipStream_t s);
 hipError_t plain_beta_only_real(const GemmArgsR &g, hipStream_t s);
 hipError_t plain_beta_only_cplx(const GemmArgsZ &g, hipStream_t s);
 // Matrix-vector products (gemv.hip), op in OP_N/OP_T/OP_C (real 'C' = 'T'); nbands: the
-// reference's OpenMP row split for real op 'N'.
+// reference's OpenMP row split for real op 'N'; order: ORDER_SIMD or ORDER_SCALAR.
 hipError_t gemv_real(int op, int64_t m, int64_t n, ddv alpha, ddv beta, const ddv *A, int64_t lda,
                      const ddv *x, int64_t incx, ddv *y, int64_t incy, int64_t nbands,
-                     hipStream_t s);
+                     int order, hipStream_t s);
 hipError_t gemv_cplx(int op, int64_t m, int64_t n, zddv alpha, zddv beta, const zddv *A,
-                     int64_t lda, const zddv *x, int64_t incx, zddv *y, int64_t incy,
+                     int64_t lda, const zddv *x, int64_t incx, zddv *y, int64_t incy, int order,
                      hipStream_t s);
 // Triangular-output GEMM on the UPLO triangle of the n x n C (g.m == g.n), column blocks of nb.
 hipError_t plain_gemmtr_real(const GemmArgsR &g, bool upper, int64_t nb, hipStream_t s);

@@ -15,6 +15,9 @@
 //       C := faithful_add(C, fast_mul(alpha, acc))   (mgemm_serial.cpp:173-191)
 // The running C lives in HBM between chunk flushes: one read-modify-write per 128-deep
 // chunk costs ~1 byte per CU-clock, far below what the FP64 ALU leaves idle.
+// The reference's scalar kernel (hosts without AVX2/FMA) is the second order, SCALAR:
+//   C := beta-prepass(C); for k ascending: C := faithful_add(C, faithful_mul(alpha*b_kj, a_ik))
+//                                             (mgemm_serial.cpp:87-99; wgemm_serial.cpp:102-115)
 #include "plain_tile.hpp"
 #include "timer.hpp"
 
@@ -23,7 +26,7 @@ namespace mw {
 // ----------------------------------------------------------------------------------
 // Real kernel.
 // ----------------------------------------------------------------------------------
-template <int TM, int TN, int BK, bool TA, bool TB, int TRI = TRI_NONE>
+template <int TM, int TN, int BK, bool TA, bool TB, int TRI = TRI_NONE, bool SCALAR = false>
 __global__ __launch_bounds__(256)
 void k_mgemm_plain(int64_t m, int64_t n, int64_t k, ddv alpha, ddv beta, int beta_mode,
                    const ddv *__restrict__ A, int64_t lda, const ddv *__restrict__ B,
@@ -36,14 +39,14 @@ void k_mgemm_plain(int64_t m, int64_t n, int64_t k, ddv alpha, ddv beta, int bet
     const int64_t j0 = (int64_t)(blockIdx.x / tiles_m) * BN;
     if constexpr (TRI != TRI_NONE)
         if (!tri_tile_live<TRI>(i0, j0, BM, BN, n, tri_nb)) return;
-    mgemm_plain_tile<TM, TN, BK, TA, TB, TRI>(m, n, k, alpha, beta, beta_mode, A, lda, B, ldb, C,
-                                              ldc, i0, j0, As, Bs, tri_nb);
+    mgemm_plain_tile<TM, TN, BK, TA, TB, TRI, SCALAR>(m, n, k, alpha, beta, beta_mode, A, lda, B,
+                                                      ldb, C, ldc, i0, j0, As, Bs, tri_nb);
 }
 
 // ----------------------------------------------------------------------------------
 // Complex kernel (same structure; op letters N/T/C with conjugation in the staging).
 // ----------------------------------------------------------------------------------
-template <int TM, int TN, int BK, int OPA, int OPB, int TRI = TRI_NONE>
+template <int TM, int TN, int BK, int OPA, int OPB, int TRI = TRI_NONE, bool SCALAR = false>
 __global__ __launch_bounds__(256)
 void k_wgemm_plain(int64_t m, int64_t n, int64_t k, zddv alpha, zddv beta, int beta_mode,
                    const zddv *__restrict__ A, int64_t lda, const zddv *__restrict__ B,
@@ -60,6 +63,11 @@ void k_wgemm_plain(int64_t m, int64_t n, int64_t k, zddv alpha, zddv beta, int b
     const int64_t j0 = (int64_t)(blockIdx.x / tiles_m) * BN;
     if constexpr (TRI != TRI_NONE)
         if (!tri_tile_live<TRI>(i0, j0, BM, BN, n, tri_nb)) return;
+    if constexpr (SCALAR) {
+        scalar_order_tile<TM, TN, BK, OPA != OP_N, OPB != OP_N, TRI, OPA == OP_C, OPB == OP_C>(
+            m, n, k, alpha, beta, beta_mode, A, lda, B, ldb, C, ldc, i0, j0, As, Bs, tri_nb);
+        return;
+    }
 
     SlabA<zddv, BM, BK, OPA != OP_N, OPA == OP_C> sa;
     SlabB<zddv, BN, BK, OPB != OP_N, OPB == OP_C> sb;
@@ -291,7 +299,7 @@ namespace {
 constexpr int RTM = 4, RTN = 4, RBK = 16;   // real tile: 64 x 64, 16-deep slabs
 constexpr int CTM = 2, CTN = 2, CBK = 16;   // complex tile: 32 x 32
 
-template <int TM, int TN, bool TA, bool TB, int TRI = TRI_NONE>
+template <int TM, int TN, bool TA, bool TB, int TRI = TRI_NONE, bool SCALAR = false>
 hipError_t launch_real_tile(const GemmArgsR &g, hipStream_t s, int64_t tri_nb = 0)
 {
     constexpr int BM = 16 * TM, BN = 16 * TN;
@@ -299,7 +307,7 @@ hipError_t launch_real_tile(const GemmArgsR &g, hipStream_t s, int64_t tri_nb = 
     int64_t tiles = (int64_t)tiles_m * ((g.n + BN - 1) / BN);
     if (tiles > INT32_MAX) return hipErrorInvalidValue;
     ScopedTimer tm(KC_PLAIN_REAL, s);
-    hipLaunchKernelGGL((k_mgemm_plain<TM, TN, RBK, TA, TB, TRI>), dim3((unsigned)tiles),
+    hipLaunchKernelGGL((k_mgemm_plain<TM, TN, RBK, TA, TB, TRI, SCALAR>), dim3((unsigned)tiles),
                        dim3(256), 0, s, g.m, g.n, g.k, g.alpha, g.beta, g.beta_mode, g.A, g.lda,
                        g.B, g.ldb, g.C, g.ldc, tiles_m, tri_nb);
     return hipGetLastError();
@@ -307,10 +315,12 @@ hipError_t launch_real_tile(const GemmArgsR &g, hipStream_t s, int64_t tri_nb = 
 
 // Variant 0: 64 x 64 tiles (4 x 4 per lane); variant 1: 32 x 64 (2 x 4 per lane), which
 // keeps more workgroups in flight on small products.  Without a tuned choice, variant 1 is
-// used when 64 x 64 tiles would not give every CU two workgroups.
+// used when 64 x 64 tiles would not give every CU two workgroups.  The scalar order always
+// takes 64 x 64 tiles.
 template <bool TA, bool TB>
 hipError_t launch_real(const GemmArgsR &g, hipStream_t s)
 {
+    if (g.order == ORDER_SCALAR) return launch_real_tile<RTM, RTN, TA, TB, TRI_NONE, true>(g, s);
     int v = g.variant;
     if (v < 0) v = tune_lookup(TK_PLAIN_REAL, g.m, g.n, g.k);
     if (v < 0) v = ((g.m + 63) / 64) * ((g.n + 63) / 64) < 512 ? 1 : 0;
@@ -318,18 +328,25 @@ hipError_t launch_real(const GemmArgsR &g, hipStream_t s)
     return launch_real_tile<RTM, RTN, TA, TB>(g, s);
 }
 
-template <int OA, int OB, int TRI = TRI_NONE>
-hipError_t launch_cplx(const GemmArgsZ &g, hipStream_t s, int64_t tri_nb = 0)
+template <int OA, int OB, int TRI = TRI_NONE, bool SCALAR = false>
+hipError_t launch_cplx_tile(const GemmArgsZ &g, hipStream_t s, int64_t tri_nb)
 {
     constexpr int BM = 16 * CTM, BN = 16 * CTN;
     int tiles_m = (int)((g.m + BM - 1) / BM);
     int64_t tiles = (int64_t)tiles_m * ((g.n + BN - 1) / BN);
     if (tiles > INT32_MAX) return hipErrorInvalidValue;
     ScopedTimer tm(KC_PLAIN_CPLX, s);
-    hipLaunchKernelGGL((k_wgemm_plain<CTM, CTN, CBK, OA, OB, TRI>), dim3((unsigned)tiles),
+    hipLaunchKernelGGL((k_wgemm_plain<CTM, CTN, CBK, OA, OB, TRI, SCALAR>), dim3((unsigned)tiles),
                        dim3(256), 0, s, g.m, g.n, g.k, g.alpha, g.beta, g.beta_mode, g.A, g.lda,
                        g.B, g.ldb, g.C, g.ldc, tiles_m, tri_nb);
     return hipGetLastError();
+}
+
+template <int OA, int OB, int TRI = TRI_NONE>
+hipError_t launch_cplx(const GemmArgsZ &g, hipStream_t s, int64_t tri_nb = 0)
+{
+    if (g.order == ORDER_SCALAR) return launch_cplx_tile<OA, OB, TRI, true>(g, s, tri_nb);
+    return launch_cplx_tile<OA, OB, TRI, false>(g, s, tri_nb);
 }
 
 template <int OA>
@@ -346,8 +363,13 @@ hipError_t launch_cplx_b(const GemmArgsZ &g, hipStream_t s)
 template <bool TA, bool TB>
 hipError_t launch_gemmtr_real(const GemmArgsR &g, bool upper, int64_t nb, hipStream_t s)
 {
-    hipError_t e = upper ? launch_real_tile<RTM, RTN, TA, TB, TRI_UPPER>(g, s, nb)
-                         : launch_real_tile<RTM, RTN, TA, TB, TRI_LOWER>(g, s, nb);
+    hipError_t e;
+    if (g.order == ORDER_SCALAR)
+        e = upper ? launch_real_tile<RTM, RTN, TA, TB, TRI_UPPER, true>(g, s, nb)
+                  : launch_real_tile<RTM, RTN, TA, TB, TRI_LOWER, true>(g, s, nb);
+    else
+        e = upper ? launch_real_tile<RTM, RTN, TA, TB, TRI_UPPER>(g, s, nb)
+                  : launch_real_tile<RTM, RTN, TA, TB, TRI_LOWER>(g, s, nb);
     if (e != hipSuccess) return e;
     ScopedTimer tm(KC_GEMMTR_DIAG, s);
     const int64_t lanes = g.n * nb;

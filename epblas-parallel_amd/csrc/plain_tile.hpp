@@ -21,6 +21,18 @@ template <> struct VecOf<ddv> { typedef v2d type; };
 template <> struct VecOf<zddv> { typedef v4d type; };
 DDI v2d conj_vec(v2d x) { return x; }
 DDI v4d conj_vec(v4d x) { return v4d{x.x, x.y, -x.z, -x.w}; }
+DDI ddv mul_exact_t(const ddv &a, const ddv &b) { return dd_mul_exact(a, b); }
+DDI zddv mul_exact_t(const zddv &a, const zddv &b) { return zdd_mul_exact(a, b); }
+DDI ddv add_exact_t(const ddv &a, const ddv &b) { return dd_add_exact(a, b); }
+DDI zddv add_exact_t(const zddv &a, const zddv &b) { return zdd_add_exact(a, b); }
+DDI ddv beta_prepass_t(const ddv &c, const ddv &beta, int mode) { return dd_beta_prepass(c, beta, mode); }
+DDI zddv beta_prepass_t(const zddv &c, const zddv &beta, int mode) { return zdd_beta_prepass(c, beta, mode); }
+
+// Plain-path operation orders.  ORDER_SIMD: the reference's AVX2 kernel (fast operations
+// accumulated per 128-deep chunk, then alpha and a faithful add into C; mgemm_serial.cpp:
+// 173-319).  ORDER_SCALAR: its scalar kernel for hosts without AVX2/FMA (every
+// multiply-add straight into C with the faithful operations, C += (alpha*b)*a;
+// mgemm_serial.cpp:87-99, wgemm_serial.cpp:102-115).
 
 // op(A) slab [i0, i0+BM) x [k0, k0+BK) -> regs.  TRANS: A is stored K x M.
 template <typename T, int BM, int BK, bool TRANS, bool CONJ>
@@ -96,6 +108,22 @@ struct SlabB {
             *reinterpret_cast<V *>(&S[c][r]) = v[q];
         }
     }
+    // As store, each entry replaced by alpha (x) entry with the faithful DD (or complex DD)
+    // multiply: the reference's scalar-order kernel forms t = alpha * b per B entry.
+    template <int LDW>
+    DDI void store_scaled(T (*S)[LDW], const T &alpha, int tid) const
+    {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            int e = tid + 256 * q;
+            int r, c;
+            if (!TRANS) { c = e % BK; r = e / BK; }
+            else        { r = e % BN; c = e / BN; }
+            T x;
+            __builtin_memcpy(&x, &v[q], sizeof(T));
+            S[c][r] = mul_exact_t(alpha, x);
+        }
+    }
 };
 
 // Triangular-output GEMM (gemmtr) masks.  The n x n C is walked in column blocks of nb;
@@ -126,16 +154,114 @@ template <int TRI> DDI bool tri_tile_live(int64_t i0, int64_t j0, int BM, int BN
     }
 }
 
-// One BM x BN tile of a real plain GEMM (the whole K loop with the reference's
-// 128-deep chunk flushes); As/Bs are the workgroup's LDS slabs.  Ends with a barrier, so the
-// caller may reuse the LDS for another tile right away.  With TRI, only the gemmtr rectangle
-// elements of the tile are read and written.
-template <int TM, int TN, int BK, bool TA, bool TB, int TRI = TRI_NONE>
+// The lane's TM x TN register-tile elements that a tile writes: in range and, for gemmtr,
+// in the rectangle; fixed for the tile, so one bit mask (one register) for the K loop.
+template <int TM, int TN, int TRI>
+DDI uint32_t tile_keep(int64_t m, int64_t n, int64_t i0, int64_t j0, int tx, int ty, int64_t tri_nb)
+{
+    static_assert(TM * TN <= 32, "store mask width");
+    uint32_t keep = 0;
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const int64_t gi = i0 + tx + 16 * i, gj = j0 + ty + 16 * j;
+            if (gi < m && gj < n && tri_rect<TRI>(gi, gj, tri_nb)) keep |= 1u << (i * TN + j);
+        }
+    return keep;
+}
+
+// One tile in the scalar order (real or complex elements T): C is pre-passed with beta
+// and kept in registers, and for k ascending C := C + t_kj * a_ik with t_kj = alpha * b_kj
+// formed once per B entry as the slab is staged (faithful operations throughout).
+template <int TM, int TN, int BK, bool OA_T, bool OB_T, int TRI, bool OA_C = false,
+          bool OB_C = false, typename T>
+DDI void scalar_order_tile(int64_t m, int64_t n, int64_t k, T alpha, T beta, int beta_mode,
+                           const T *__restrict__ A, int64_t lda, const T *__restrict__ B,
+                           int64_t ldb, T *__restrict__ C, int64_t ldc, int64_t i0, int64_t j0,
+                           T (&As)[BK][16 * TM], T (&Bs)[BK][16 * TN], int64_t tri_nb)
+{
+    constexpr int BM = 16 * TM, BN = 16 * TN;
+    const int tid = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
+    SlabA<T, BM, BK, OA_T, OA_C> sa;
+    SlabB<T, BN, BK, OB_T, OB_C> sb;
+    const uint32_t keep = tile_keep<TM, TN, TRI>(m, n, i0, j0, tx, ty, tri_nb);
+
+    T c[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const int64_t gi = i0 + tx + 16 * i, gj = j0 + ty + 16 * j;
+            T v{};
+            if ((keep >> (i * TN + j)) & 1u) v = beta_prepass_t(C[gj * ldc + gi], beta, beta_mode);
+            c[i][j] = v;
+        }
+
+    const int64_t ntiles = (k + BK - 1) / BK;
+    sa.load(A, lda, m, k, i0, 0, tid);
+    sb.load(B, ldb, n, k, j0, 0, tid);
+    sa.store(As, tid);
+    sb.store_scaled(Bs, alpha, tid);
+    __syncthreads();
+    for (int64_t t = 0; t < ntiles; ++t) {
+        const int64_t k0 = t * BK;
+        if (t + 1 < ntiles) {
+            sa.load(A, lda, m, k, i0, k0 + BK, tid);
+            sb.load(B, ldb, n, k, j0, k0 + BK, tid);
+        }
+        const int kmax = (k - k0 < BK) ? (int)(k - k0) : BK;
+        for (int kk = 0; kk < kmax; ++kk) {
+            T a[TM], b[TN];
+#pragma unroll
+            for (int i = 0; i < TM; ++i) a[i] = As[kk][tx + 16 * i];
+#pragma unroll
+            for (int j = 0; j < TN; ++j) b[j] = Bs[kk][ty + 16 * j];
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j) c[i][j] = add_exact_t(c[i][j], mul_exact_t(b[j], a[i]));
+        }
+        __syncthreads();
+        if (t + 1 < ntiles) {
+            sa.store(As, tid);
+            sb.store_scaled(Bs, alpha, tid);
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+            if ((keep >> (i * TN + j)) & 1u) C[(j0 + ty + 16 * j) * ldc + i0 + tx + 16 * i] = c[i][j];
+}
+
+template <int TM, int TN, int BK, bool TA, bool TB, int TRI>
+DDI void mgemm_scalar_tile(int64_t m, int64_t n, int64_t k, ddv alpha, ddv beta, int beta_mode,
+                           const ddv *__restrict__ A, int64_t lda, const ddv *__restrict__ B,
+                           int64_t ldb, ddv *__restrict__ C, int64_t ldc, int64_t i0, int64_t j0,
+                           ddv (&As)[BK][16 * TM], ddv (&Bs)[BK][16 * TN], int64_t tri_nb)
+{
+    scalar_order_tile<TM, TN, BK, TA, TB, TRI>(m, n, k, alpha, beta, beta_mode, A, lda, B, ldb, C,
+                                               ldc, i0, j0, As, Bs, tri_nb);
+}
+
+// One BM x BN tile of a real plain GEMM (the whole K loop, in the operation order SCALAR
+// selects: 128-deep chunk flushes, or every multiply-add into C); As/Bs are the workgroup's
+// LDS slabs.  Ends with a barrier, so the caller may reuse the LDS for another tile right
+// away.  With TRI, only the gemmtr rectangle elements of the tile are read and written.
+template <int TM, int TN, int BK, bool TA, bool TB, int TRI = TRI_NONE, bool SCALAR = false>
 DDI void mgemm_plain_tile(int64_t m, int64_t n, int64_t k, ddv alpha, ddv beta, int beta_mode,
                           const ddv *__restrict__ A, int64_t lda, const ddv *__restrict__ B,
                           int64_t ldb, ddv *__restrict__ C, int64_t ldc, int64_t i0, int64_t j0,
                           ddv (&As)[BK][16 * TM], ddv (&Bs)[BK][16 * TN], int64_t tri_nb = 0)
 {
+    if constexpr (SCALAR) {
+        mgemm_scalar_tile<TM, TN, BK, TA, TB, TRI>(m, n, k, alpha, beta, beta_mode, A, lda, B, ldb,
+                                                   C, ldc, i0, j0, As, Bs, tri_nb);
+        return;
+    }
     constexpr int BM = 16 * TM, BN = 16 * TN;
     static_assert(KCHUNK % BK == 0, "BK must divide the flush period");
     const int tid = threadIdx.x;
@@ -149,17 +275,7 @@ DDI void mgemm_plain_tile(int64_t m, int64_t n, int64_t k, ddv alpha, ddv beta, 
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[i][j] = ddv{0.0, 0.0};
 
-    // Which of the lane's TM x TN elements are written: in range and, for gemmtr, in the
-    // rectangle.  Fixed for the tile, so it is one bit mask (one register) for the K loop.
-    static_assert(TM * TN <= 32, "store mask width");
-    uint32_t keep = 0;
-#pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-        for (int j = 0; j < TN; ++j) {
-            const int64_t gi = i0 + tx + 16 * i, gj = j0 + ty + 16 * j;
-            if (gi < m && gj < n && tri_rect<TRI>(gi, gj, tri_nb)) keep |= 1u << (i * TN + j);
-        }
+    const uint32_t keep = tile_keep<TM, TN, TRI>(m, n, i0, j0, tx, ty, tri_nb);
 
     const int64_t ntiles = (k + BK - 1) / BK;
     sa.load(A, lda, m, k, i0, 0, tid);

@@ -67,6 +67,9 @@ def lib():
         f.restype = ctypes.c_int
         f.argtypes = [c, c, _i64, _i64, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _i64,
                       ctypes.c_int, _vp]
+    L.mwgemm_set_plain_order.restype = ctypes.c_int
+    L.mwgemm_set_plain_order.argtypes = [ctypes.c_int]
+    L.mwgemm_get_plain_order.restype = ctypes.c_int
     L.mwgemm_set_path.restype = ctypes.c_int
     L.mwgemm_set_path.argtypes = [ctypes.c_int]
     L.mwgemm_get_path.restype = ctypes.c_int
@@ -111,6 +114,22 @@ def set_path(path):
     if prev < 0:
         raise ValueError(f"bad path {path}")
     return prev
+
+
+ORDER_SIMD, ORDER_SCALAR = 0, 1
+
+
+def set_plain_order(order):
+    """Operation order of the plain path: ORDER_SIMD (the reference's AVX2/FMA kernels) or
+    ORDER_SCALAR (its kernels for hosts without AVX2/FMA).  Returns the previous order."""
+    prev = lib().mwgemm_set_plain_order(int(order))
+    if prev < 0:
+        raise ValueError(f"bad order {order}")
+    return prev
+
+
+def get_plain_order():
+    return lib().mwgemm_get_plain_order()
 
 
 def _ptr(x):

@@ -164,6 +164,19 @@ enum { MWGEMM_PATH_PLAIN = 0, MWGEMM_PATH_OZAKI = 1, MWGEMM_PATH_AUTO = 2 };
 MWGEMM_API int mwgemm_set_path(int path);          /* returns the previous path, -1 on a bad value */
 MWGEMM_API int mwgemm_get_path(void);
 
+/* Operation order of the plain path (mgemm_/wgemm_ and their _serial cores, the gemmtr
+ * rectangle, batched plain plans, mgemv_/wgemv_).  The reference computes with one of two
+ * kernels, chosen on its host: the AVX2/FMA kernels (SIMD: fast operations accumulated in
+ * 128-deep chunks, mgemm_serial.cpp:173-319) where the CPU has AVX2 and FMA, else scalar
+ * kernels (SCALAR: every multiply-add into C with the faithful operations,
+ * mgemm_serial.cpp:87-99, wgemm_serial.cpp:102-115, mgemv.cpp:147-153, :213-223).  The two
+ * round differently, so the plain path reproduces whichever the reference would run: by
+ * default the same host test the reference makes (mf_have_avx2_fma, mf_cpu.cpp:10-23),
+ * overridden by MWGEMM_PLAIN_ORDER ("simd" | "scalar") or this setter. */
+enum { MWGEMM_ORDER_SIMD = 0, MWGEMM_ORDER_SCALAR = 1 };
+MWGEMM_API int mwgemm_set_plain_order(int order);  /* returns the previous order, -1 on a bad value */
+MWGEMM_API int mwgemm_get_plain_order(void);
+
 /* Error handling.  Every failure is reported on stderr and recorded; the drop-in entry
  * points then abort the process unless MWGEMM_ON_ERROR=return (or mwgemm_set_abort(0)),
  * in which case C is left unmodified and the error stays readable here.  The device entry

@@ -313,6 +313,39 @@ static void rblock_avx2(ptrdiff_t ib, ptrdiff_t jb, ptrdiff_t pb, dd alpha, cons
 
 static int g_force_scalar = 0;
 
+/* Operation order of the reference's plain kernels: 0 = its AVX2/FMA kernels (the blocked
+ * code below), 1 = the scalar kernels it runs on hosts without AVX2/FMA
+ * (mgemm_serial.cpp:87-99, wgemm_serial.cpp:102-115, mgemv.cpp:147-153, :213-223,
+ * wgemv.cpp:170-188, :284-295), restated by the direct loops further down. */
+static int g_order = 0;
+void port_set_order(int order) { g_order = order; }
+int port_get_order(void) { return g_order; }
+
+/* The scalar kernels' per-element sequence: for column j and p ascending, t = alpha*b_pj
+ * (faithful), then C_ij := C_ij + t*a_ip (faithful) for every row i.  The reference runs
+ * this inside its GotoBLAS blocking (jc, pc, ic), which leaves each element's p order
+ * ascending, so a direct loop over p is the same sequence. */
+static void rgemm_scalar_order(char ta, char tb, ptrdiff_t m, ptrdiff_t n, ptrdiff_t k,
+                               dd alpha, const dd *a, ptrdiff_t lda, const dd *b,
+                               ptrdiff_t ldb, dd *c, ptrdiff_t ldc, int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+#endif
+    for (ptrdiff_t j = 0; j < n; ++j) {
+        dd *cj = c + (size_t)j * ldc;
+        for (ptrdiff_t p = 0; p < k; ++p) {
+            const dd bpj = (tb == 'N') ? b[(size_t)j * ldb + p] : b[(size_t)p * ldb + j];
+            const dd t = dd_mul_exact(alpha, bpj);
+            for (ptrdiff_t i = 0; i < m; ++i) {
+                const dd aip = (ta == 'N') ? a[(size_t)p * lda + i] : a[(size_t)i * lda + p];
+                cj[i] = dd_add_exact(cj[i], dd_mul_exact(t, aip));
+            }
+        }
+    }
+}
+
 static int have_avx2_fma(void)
 {
     static int cached = -1;
@@ -343,6 +376,10 @@ int port_mgemm(char transa, char transb, ptrdiff_t m, ptrdiff_t n, ptrdiff_t k,
             else if (!dd_is1(beta)) *cij = dd_mul_exact(*cij, beta);
         }
     if (dd_is0(alpha) || k == 0) return 0;
+    if (g_order == 1) {
+        rgemm_scalar_order(ta, tb, m, n, k, alpha, a, lda, b, ldb, c, ldc, nthreads);
+        return 0;
+    }
 
     const int vec = have_avx2_fma();
     int failed = 0;
@@ -439,6 +476,28 @@ int port_wgemm(char transa, char transb, ptrdiff_t m, ptrdiff_t n, ptrdiff_t k,
             else if (!zdd_is1(beta)) *cij = zdd_mul_exact(*cij, beta);
         }
     if (zdd_is0(alpha) || k == 0) return 0;
+    if (g_order == 1) {
+        /* scalar kernels (wgemm_serial.cpp:77-115): t = cmul(alpha, op(B)_pj), then
+         * C_ij := cadd(C_ij, cmul(t, op(A)_ip)), p ascending; 'C' operands conjugated */
+#ifdef _OPENMP
+        if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+#endif
+        for (ptrdiff_t j = 0; j < n; ++j) {
+            zdd *cj = c + (size_t)j * ldc;
+            for (ptrdiff_t p = 0; p < k; ++p) {
+                zdd bpj = (tb == 'N') ? b[(size_t)j * ldb + p] : b[(size_t)p * ldb + j];
+                if (tb == 'C') bpj = zdd_conj(bpj);
+                const zdd t = zdd_mul_exact(alpha, bpj);
+                for (ptrdiff_t i = 0; i < m; ++i) {
+                    zdd aip = (ta == 'N') ? a[(size_t)p * lda + i] : a[(size_t)i * lda + p];
+                    if (ta == 'C') aip = zdd_conj(aip);
+                    cj[i] = zdd_add_exact(cj[i], zdd_mul_exact(t, aip));
+                }
+            }
+        }
+        return 0;
+    }
 
     int failed = 0;
 #ifdef _OPENMP
@@ -715,7 +774,25 @@ int port_mgemv(char trans, ptrdiff_t m, ptrdiff_t n, const dd *alpha_, const dd 
         else if (!dd_is1(beta)) v = dd_mul_exact(beta, v);
         ys[i] = v;
     }
-    if (!dd_is0(alpha)) {
+    if (!dd_is0(alpha) && g_order == 1) {
+        /* scalar kernels (mgemv.cpp:147-153, :213-223): every row and every column term
+         * with the faithful operations, one sequential dot per column for op 'T' */
+        if (notrans) {
+            for (ptrdiff_t j = 0; j < n; ++j) {
+                if (dd_is0(xs[j])) continue;
+                const dd tj = dd_mul_exact(alpha, xs[j]);
+                const dd *aj = a + (size_t)j * lda;
+                for (ptrdiff_t i = 0; i < m; ++i) ys[i] = dd_add_exact(ys[i], dd_mul_exact(tj, aj[i]));
+            }
+        } else {
+            for (ptrdiff_t j = 0; j < n; ++j) {
+                const dd *aj = a + (size_t)j * lda;
+                dd s = { 0.0, 0.0 };
+                for (ptrdiff_t i = 0; i < m; ++i) s = dd_add_exact(s, dd_mul_exact(aj[i], xs[i]));
+                ys[j] = dd_add_exact(ys[j], dd_mul_exact(alpha, s));
+            }
+        }
+    } else if (!dd_is0(alpha)) {
         if (notrans) {
             if (nbands < 1) nbands = 1;
             for (ptrdiff_t t = 0; t < nbands; ++t) {
@@ -765,7 +842,26 @@ int port_wgemv(char trans, ptrdiff_t m, ptrdiff_t n, const zdd *alpha_, const zd
         else if (!zdd_is1(beta)) v = zdd_mul_exact(beta, v);
         ys[i] = v;
     }
-    if (!zdd_is0(alpha)) {
+    if (!zdd_is0(alpha) && g_order == 1) {
+        /* scalar kernels (wgemv.cpp:170-188, :284-295) */
+        if (notrans) {
+            for (ptrdiff_t j = 0; j < n; ++j) {
+                if (zdd_is0(xs[j])) continue;
+                const zdd tj = zdd_mul_exact(alpha, xs[j]);
+                const zdd *aj = a + (size_t)j * lda;
+                for (ptrdiff_t i = 0; i < m; ++i) ys[i] = zdd_add_exact(ys[i], zdd_mul_exact(tj, aj[i]));
+            }
+        } else {
+            for (ptrdiff_t j = 0; j < n; ++j) {
+                const zdd *aj = a + (size_t)j * lda;
+                zdd s;
+                memset(&s, 0, sizeof s);
+                for (ptrdiff_t i = 0; i < m; ++i)
+                    s = zdd_add_exact(s, zdd_mul_exact(zop(aj[i], conj_a), xs[i]));
+                ys[j] = zdd_add_exact(ys[j], zdd_mul_exact(alpha, s));
+            }
+        }
+    } else if (!zdd_is0(alpha)) {
         const ptrdiff_t m4 = m / 4 * 4;
         if (notrans) {
             for (ptrdiff_t j = 0; j < n; ++j) {

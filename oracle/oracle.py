@@ -84,6 +84,8 @@ def lib():
         _LIB.q128_max_abs.restype = ctypes.c_double
         _LIB.q128_max_abs.argtypes = [_vp, _c_i64]
         _LIB.port_force_scalar.argtypes = [ctypes.c_int]
+        _LIB.port_set_order.argtypes = [ctypes.c_int]
+        _LIB.port_get_order.restype = ctypes.c_int
         _LIB.port_max_threads.restype = ctypes.c_int
         _LIB.port_has_avx2.restype = ctypes.c_int
     return _LIB
@@ -102,6 +104,18 @@ def ref_lib():
 
 def _p(x):
     return x.ctypes.data_as(ctypes.c_void_p)
+
+
+ORDER_SIMD, ORDER_SCALAR = 0, 1
+
+
+def set_order(order):
+    """Operation order of the restated plain kernels: ORDER_SIMD (the reference's AVX2/FMA
+    kernels) or ORDER_SCALAR (its kernels for hosts without AVX2/FMA).  Returns the previous
+    order."""
+    prev = lib().port_get_order()
+    lib().port_set_order(int(order))
+    return prev
 
 
 def _ch(t):
