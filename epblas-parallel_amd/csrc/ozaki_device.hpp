@@ -569,22 +569,41 @@ DDI void oz_crt_accumulate(const uint8_t *__restrict__ U, int64_t ldu, int64_t s
         for (int p = 0; p < P; ++p)
 #pragma unroll
             for (int l = 0; l < 4; ++l) acc[e][p][l] = 0.0;
-#pragma unroll 2
-    for (int t = 0; t < nmod; ++t) {
+    // One modulus: the residue bytes of RPL rows times the four weight limbs.
+    auto step = [&](int t, const uint32_t (&word)[P]) {
         const double w0 = mcs[t].w[0], w1 = mcs[t].w[1], w2 = mcs[t].w[2], w3 = mcs[t].w[3];
 #pragma unroll
-        for (int p = 0; p < P; ++p) {
-            const uint32_t word =
-                *reinterpret_cast<const uint32_t *>(U + (int64_t)(t * P + p) * sU + j * ldu + i0);
+        for (int p = 0; p < P; ++p)
 #pragma unroll
             for (int e = 0; e < RPL; ++e) {
-                const double u = (double)((word >> (8 * e)) & 0xffu);
+                const double u = (double)((word[p] >> (8 * e)) & 0xffu);
                 acc[e][p][0] = __builtin_fma(u, w0, acc[e][p][0]);
                 acc[e][p][1] = __builtin_fma(u, w1, acc[e][p][1]);
                 acc[e][p][2] = __builtin_fma(u, w2, acc[e][p][2]);
                 acc[e][p][3] = __builtin_fma(u, w3, acc[e][p][3]);
             }
-        }
+    };
+    auto load = [&](int t, uint32_t (&word)[P]) {
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+            word[p] = *reinterpret_cast<const uint32_t *>(U + (int64_t)(t * P + p) * sU + j * ldu + i0);
+    };
+    // The residue words of TB moduli are loaded before any of their FMAs, so each lane
+    // keeps TB*P loads in flight.  The limb sums are exact integers, so the grouping does not
+    // change the result.
+    constexpr int TB = 8;
+    int t = 0;
+    for (; t + TB <= nmod; t += TB) {
+        uint32_t words[TB][P];
+#pragma unroll
+        for (int q = 0; q < TB; ++q) load(t + q, words[q]);
+#pragma unroll
+        for (int q = 0; q < TB; ++q) step(t + q, words[q]);
+    }
+    for (; t < nmod; ++t) {
+        uint32_t word[P];
+        load(t, word);
+        step(t, word);
     }
 }
 
