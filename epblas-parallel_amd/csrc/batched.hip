@@ -124,8 +124,8 @@ void k_batch_plain(const BatchProb *__restrict__ probs, const int4 *__restrict__
 // ---------------------------------------------------------------------------------
 // Row maxima.  Column-contiguous input: unit = (row r, 64-deep k chunk), rows fastest so
 // lanes read adjacent rows.  Row-contiguous input (KC): every problem's unit count is a
-// multiple of 64 and a wave takes the 64 consecutive k of one (row, chunk), reduced across
-// the wave before a single atomic.
+// multiple of 64 and a wave takes the 256 consecutive k of one (row, chunk), four per lane,
+// reduced across the wave before a single atomic.
 template <bool A_SIDE, bool KC>
 __global__ __launch_bounds__(256)
 void k_batch_rowmax(const BatchProb *__restrict__ probs, UnitSpace us)
@@ -140,11 +140,18 @@ void k_batch_rowmax(const BatchProb *__restrict__ probs, UnitSpace us)
     const int64_t ld = A_SIDE ? p.lda : p.ldb;
     unsigned long long *rmax = A_SIDE ? p.rA : p.rB;
     if constexpr (KC) {
-        const int64_t nch = (p.k + 63) / 64, w = local / 64;
+        // a wave takes 256 consecutive k of one row, four coalesced steps of 64
+        const int64_t nch = (p.k + 255) / 256, w = local / 64;
         const int lane = (int)(local % 64);
-        const int64_t r = w / nch, kk = (w % nch) * 64 + lane;
-        const double mx = kk < p.k ? mag(X[r * ld + kk]) : 0.0;
-        unsigned long long bits = (unsigned long long)__double_as_longlong(mx);
+        const int64_t r = w / nch, kb = (w % nch) * 256 + lane;
+        unsigned long long bits = 0ull;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int64_t kk = kb + 64 * q;
+            const double mx = kk < p.k ? mag(X[r * ld + kk]) : 0.0;
+            const unsigned long long b = (unsigned long long)__double_as_longlong(mx);
+            bits = b > bits ? b : bits;
+        }
         // NaN-aware maximum on the bit patterns of non-negative values.
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
@@ -248,9 +255,13 @@ void k_batch_oz_ring(const BatchProb *__restrict__ probs, const int4 *__restrict
         const int64_t i0 = (int64_t)tr.z * 128, j0 = (int64_t)tr.w * 128;
         const int64_t sA = p.Mp * p.Kc, sB = p.Np * p.Kc, sU = p.Np * p.Mp;
         const int nm = nmod - tr.y < group ? nmod - tr.y : group;
-        oz_gemm_tile_ring<2, 2, 2, 2, true>(p.resA + tr.y * sA + i0 * p.Kc,
-                                            p.resB + tr.y * sB + j0 * p.Kc, p.U + tr.y * sU,
-                                            p.Kc, p.Mp, mcs + tr.y, nm, sA, sB, sU, i0, j0, smem);
+        // residue rows exist up to Mp / Np (multiples of 32), possibly short of the tile
+        const int ra = (int)(p.Mp - i0 < 128 ? p.Mp - i0 : 128);
+        const int rb = (int)(p.Np - j0 < 128 ? p.Np - j0 : 128);
+        oz_gemm_tile_ring<2, 2, 2, 2, true, true>(p.resA + tr.y * sA + i0 * p.Kc,
+                                                  p.resB + tr.y * sB + j0 * p.Kc, p.U + tr.y * sU,
+                                                  p.Kc, p.Mp, mcs + tr.y, nm, sA, sB, sU, i0, j0,
+                                                  smem, ra, rb);
     }
 }
 
@@ -505,7 +516,10 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
         }
         cr[p + 1] = empty ? 0 : rup(q.n * ((q.m + 3) / 4), 64);
         if (empty || !q.active) continue;
-        q.Mp = rup(q.m, OT); q.Np = rup(q.n, OT); q.Kc = rup(q.k, 64);
+        // Residue rows are padded to 32 for the ring's 128-row tiles (it clamps its loads and
+        // stores to the rows that exist), to 64 for the 64-row register-staged tiles.
+        const int64_t PAD = OT == 128 ? 32 : OT;
+        q.Mp = rup(q.m, PAD); q.Np = rup(q.n, PAD); q.Kc = rup(q.k, 64);
         offR[p] = rmax_n;
         rmax_n += q.Mp + q.Np;
         offA[p] = ws; ws += rup((int64_t)nmod * q.Mp * q.Kc, 256);
@@ -513,13 +527,13 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
         offU[p] = ws; ws += rup((int64_t)nmod * q.Np * q.Mp, 256);
         // unit counts of the maxima and split kernels (layouts documented there), each
         // padded to whole waves so that a wave never spans two problems
-        const int64_t nch = (q.k + 63) / 64, G = q.Kc / 16;
-        ra[p + 1] = a_kc ? q.m * nch * 64 : rup(q.m * nch, 64);
-        rb[p + 1] = b_kc ? q.n * nch * 64 : rup(q.n * nch, 64);
+        const int64_t nch = (q.k + 63) / 64, nch4 = (q.k + 255) / 256, G = q.Kc / 16;
+        ra[p + 1] = a_kc ? q.m * nch4 * 64 : rup(q.m * nch, 64);
+        rb[p + 1] = b_kc ? q.n * nch4 * 64 : rup(q.n * nch, 64);
         sa[p + 1] = a_kc ? rup(q.m * G, 64) : rup(q.m, 8) * rup(G, 8);
         sb[p + 1] = b_kc ? rup(q.n * G, 64) : rup(q.n, 8) * rup(G, 8);
-        for (int64_t tn = 0; tn < q.Np / OT; ++tn)
-            for (int64_t tm = 0; tm < q.Mp / OT; ++tm) tl.push_back(int4{(int)p, 0, (int)tm, (int)tn});
+        for (int64_t tn = 0; tn < (q.Np + OT - 1) / OT; ++tn)
+            for (int64_t tm = 0; tm < (q.Mp + OT - 1) / OT; ++tm) tl.push_back(int4{(int)p, 0, (int)tm, (int)tn});
     }
     // Residue tiles run all moduli in one slab pipeline; only when that leaves fewer than
     // about four tiles per CU are the moduli split into groups (tile .y = first modulus).

@@ -356,12 +356,16 @@ DDI void wait_lds() { __builtin_amdgcn_s_waitcnt(15 | (7 << 4) | (0 << 8) | (3 <
 // batched products (two per CU).  With MULTI, the tile runs the products of nmod moduli
 // (A_t = Ab + t*sA, B_t = Bb + t*sB, U_t = Ub + t*sU, constants mcs[t]) as one sequence of
 // stages, so the ring stays full across modulus boundaries; the accumulators of a modulus
-// are reduced and stored while later stages' DMA is in flight.
-template <int WM, int WN, int TM, int TN, bool MULTI>
+// are reduced and stored while later stages' DMA is in flight.  With CLAMP, only the first
+// rows_a rows of the A panel and rows_b rows of the B panel exist (multiples of 32): rows
+// beyond them are fetched as copies of the last row (valid, cached memory) and their
+// results are not stored.
+template <int WM, int WN, int TM, int TN, bool MULTI, bool CLAMP = false>
 DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restrict__ Bb,
                            uint8_t *__restrict__ Ub, int64_t kp, int64_t ldu,
                            const ModConst *__restrict__ mcs, int nmod, int64_t sA, int64_t sB,
-                           int64_t sU, int64_t i0, int64_t j0, int8_t *smem)
+                           int64_t sU, int64_t i0, int64_t j0, int8_t *smem, int rows_a = 0,
+                           int rows_b = 0)
 {
     constexpr int BT = 32 * TM * WM, NW = WM * WN;
     static_assert(BT == 32 * TN * WN, "square tile");
@@ -387,11 +391,13 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
             const int blk = DQ * w + q;
             const int r = blk * 16 + drow;
             const int c = dslot ^ ((r >> 2) & 3);
+            const int ra = CLAMP && r >= rows_a ? rows_a - 1 : r;
+            const int rb = CLAMP && r >= rows_b ? rows_b - 1 : r;
             __builtin_amdgcn_global_load_lds(
-                (const __attribute__((address_space(1))) void *)(ca + (int64_t)r * kp + ck * BK + c * 16),
+                (const __attribute__((address_space(1))) void *)(ca + (int64_t)ra * kp + ck * BK + c * 16),
                 (__attribute__((address_space(3))) void *)(slot + blk * 1024), 16, 0, 0);
             __builtin_amdgcn_global_load_lds(
-                (const __attribute__((address_space(1))) void *)(cb + (int64_t)r * kp + ck * BK + c * 16),
+                (const __attribute__((address_space(1))) void *)(cb + (int64_t)rb * kp + ck * BK + c * 16),
                 (__attribute__((address_space(3))) void *)(slot + BT * BK + blk * 1024), 16, 0, 0);
         }
         if (++ck == nk && MULTI) {
@@ -457,10 +463,13 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
         for (int a = 0; a < TM; ++a)
 #pragma unroll
             for (int b = 0; b < TN; ++b) {
-                const int64_t col = j0 + wn * 32 * TN + b * 32 + (lane & 31);
-                const int64_t rowb = i0 + wm * 32 * TM + a * 32 + 4 * (lane >> 5);
+                const int cl = wn * 32 * TN + b * 32 + (lane & 31);
+                const int rl = wm * 32 * TM + a * 32 + 4 * (lane >> 5);
+                const int64_t col = j0 + cl, rowb = i0 + rl;
 #pragma unroll
                 for (int g = 0; g < 4; ++g) {
+                    // rows rl+8g .. +3 lie all inside or all outside rows_a (a multiple of 32)
+                    if (CLAMP && (rl + 8 * g >= rows_a || cl >= rows_b)) continue;
                     uint32_t word = 0;
 #pragma unroll
                     for (int q = 0; q < 4; ++q)
