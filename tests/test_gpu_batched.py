@@ -197,3 +197,30 @@ def test_batched_rejects_bad_arguments(gpu):
     with pytest.raises(mwblas.MwgemmError, match="trans"):
         mwblas.mgemm_batched_device("Q", "N", [4], [4], [4], np.array([[1.0, 0.0]]), [A], [4],
                                     [A], [4], np.array([[0.0, 0.0]]), [A], [4])
+
+
+def test_auto_without_a_tuned_choice_goes_by_work(gpu):
+    """Untuned AUTO takes the exact product for a batch of at least 1e9 multiply-adds whose
+    problems have a mean size of 96 or more (the 2000-problem workload), plain for a batch of
+    small problems.  The plans are only built, never run, so one small device tensor stands
+    in for every operand."""
+    import torch
+    mwblas.autotune_clear()
+    dummy = torch.zeros((1, 1, 2), dtype=torch.float64, device=gpu)
+
+    def auto_path(dims):
+        cnt = len(dims)
+        ms, ns, ks = ([d[i] for d in dims] for i in range(3))
+        one = np.tile([1.0, 0.0], (cnt, 1))
+        plan = mwblas.BatchedPlan("N", "N", ms, ns, ks, one, [dummy] * cnt, ms, [dummy] * cnt,
+                                  ks, one, [dummy] * cnt, ms, path=mwblas.PATH_AUTO)
+        try:
+            return plan.info()["path"]
+        finally:
+            plan.close()
+
+    rng = np.random.default_rng(2000)
+    big = [tuple(ddgen.rand_int_log(rng, 16, 512) for _ in range(3)) for _ in range(2000)]
+    assert auto_path(big) == mwblas.PATH_OZAKI
+    small = [tuple(int(rng.integers(16, 48)) for _ in range(3)) for _ in range(200)]
+    assert auto_path(small) == mwblas.PATH_PLAIN
