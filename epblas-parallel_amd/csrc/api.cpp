@@ -75,11 +75,15 @@ int record_error(int code, const char *fmt, ...)
     return code ? code : -1;
 }
 
+// A failed HIP call also leaves its error as the thread's last error: clear it, so that the
+// launch check of a later call does not report it again (sticky errors persist anyway).
 #define MW_TRY(expr)                                                                   \
     do {                                                                               \
         hipError_t e_ = (expr);                                                        \
-        if (e_ != hipSuccess)                                                          \
+        if (e_ != hipSuccess) {                                                        \
+            (void)hipGetLastError();                                                   \
             return record_error((int)e_, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+        }                                                                              \
     } while (0)
 
 // ----------------------------------------------------------------------------------

@@ -142,7 +142,10 @@ int mwgemm_autotune(int64_t m, int64_t n, int64_t k, int cplx, int path, int *va
     if (hipMalloc(&A, esz * m * k) != hipSuccess || hipMalloc(&B, esz * k * n) != hipSuccess ||
         hipMalloc(&C, esz * m * n) != hipSuccess || hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    {
+        (void)hipGetLastError();
         rc = record_error(-6, "autotune: cannot allocate %zu bytes of probe operands", esz * (m * k + k * n + m * n));
+    }
     const int f = cplx ? 2 : 1;
     if (!rc) rc = fill((ddv *)A, f * m * k, s);
     if (!rc) rc = fill((ddv *)B, f * k * n, s);
@@ -226,7 +229,10 @@ int mwgemm_batched_autotune(int64_t count, char transa, char transb, const int64
     hipStream_t s = nullptr;
     int rc = 0;
     if (hipMalloc(&buf, total * sizeof(ddv)) != hipSuccess || hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+    {
+        (void)hipGetLastError();
         rc = record_error(-6, "batched autotune: cannot allocate %zu bytes", total * sizeof(ddv));
+    }
     if (!rc) rc = fill(buf, (int64_t)total, s);
     std::vector<const mw_float64x2 *> A(count), B(count);
     std::vector<mw_float64x2 *> C(count);

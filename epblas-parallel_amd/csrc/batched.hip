@@ -294,11 +294,15 @@ void k_batch_crt(const BatchProb *__restrict__ probs, UnitSpace us, int nmod,
 // ---------------------------------------------------------------------------------
 namespace {
 
+// A failed HIP call also leaves its error as the thread's last error: clear it, so that the
+// launch check of a later call does not report it again (sticky errors persist anyway).
 #define B_TRY(expr)                                                                    \
     do {                                                                               \
         hipError_t e_ = (expr);                                                        \
-        if (e_ != hipSuccess)                                                          \
+        if (e_ != hipSuccess) {                                                        \
+            (void)hipGetLastError();                                                   \
             return record_error((int)e_, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+        }                                                                              \
     } while (0)
 
 int64_t rup(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
@@ -610,11 +614,17 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
         hipMemcpy(P->us_sa, sa.data(), pre, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(P->us_sb, sb.data(), pre, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(P->us_crt, cr.data(), pre, hipMemcpyHostToDevice) != hipSuccess)
+    {
+        (void)hipGetLastError();
         rc = record_error(-1, "batched mgemm: plan upload failed");
+    }
     if (!rc && oz && nmod > 0) {
         if (hipMalloc(&P->mcs, sizeof(ModConst) * OZ_MAX_MOD) != hipSuccess ||
             hipMemcpy(P->mcs, P->oz.mc, sizeof(ModConst) * nmod, hipMemcpyHostToDevice) != hipSuccess)
+        {
+            (void)hipGetLastError();
             rc = record_error(-1, "batched mgemm: modulus table upload failed");
+        }
     }
     if (rc) { plan_free(P); return rc; }
     *out = P;

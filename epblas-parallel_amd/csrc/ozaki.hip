@@ -351,11 +351,15 @@ void k_oz_crt(const uint8_t *__restrict__ U, int64_t ldu, int64_t sU, int nmod,
 // ==================================================================================
 namespace {
 
+// A failed HIP call also leaves its error as the thread's last error: clear it, so that the
+// launch check of a later call does not report it again (sticky errors persist anyway).
 #define OZ_TRY(expr)                                                                   \
     do {                                                                               \
         hipError_t e_ = (expr);                                                        \
-        if (e_ != hipSuccess)                                                          \
+        if (e_ != hipSuccess) {                                                        \
+            (void)hipGetLastError();                                                   \
             return record_error((int)e_, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+        }                                                                              \
     } while (0)
 
 struct OzDevice {
