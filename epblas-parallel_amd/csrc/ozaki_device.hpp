@@ -50,17 +50,34 @@ DDI void int_chunks(double hi, double lo, int sigma, double &c0, double &c1, dou
     c2 = sg * a2;
 }
 
-// Symmetric residue of c0 + c1*2^35 + c2*2^70 modulo m, in [-(m-1)/2, (m-1)/2] for odd m
-// and [-128, 127] for m = 256 (half = (m-1) >> 1).  s = c2*p2 + c1*p1 + c0 is an exact
-// integer with |s| < 2^45, and q = floor(s * inv_up) is floor(s/m) for s >= 0.  For s < 0
-// it is too, except when m divides s, where q is one lower and r = s - q*m comes out as m
-// instead of 0; r lies in [0, m] either way and the symmetric step maps m to 0.
-DDI int sym_residue(double c0, double c1, double c2, const ModConst &mc, int mi, int half)
+// Symmetric residue of c0 + c1*2^35 + c2*2^70 modulo m: r = s - m*rint(s/m), where
+// s = c2*p2 + c1*p1 + c0 is an exact integer with |s| < 2^45.  The computed s*inv_m is
+// within |s/m|*2^-52 < 2^-7/m of s/m.  For odd m, s/m is at least 1/(2m) away from every
+// half-integer, so rint picks the same integer as for the exact quotient and r lies in
+// [-(m-1)/2, (m-1)/2].  For m = 256 the product is exact and a tie gives r = +-128, whose
+// byte 0x80 is -128 == 128 (mod 256).  Three FP64 operations and a conversion per entry
+// and modulus; no integer fix-up.
+DDI int sym_residue(double c0, double c1, double c2, const ModConst &mc)
 {
     const double s = __builtin_fma(c2, mc.p2, __builtin_fma(c1, mc.p1, c0));
-    const double q = floor(s * mc.inv_up);
-    const int r = (int)__builtin_fma(-q, mc.m, s);
-    return r > half ? r - mi : r;
+    const double q = rint(s * mc.inv_m);
+    return (int)__builtin_fma(-q, mc.m, s);
+}
+
+// The low bytes of v0..v3 as one word, v0 in byte 0 (v_perm_b32: selector bytes 0-3 take
+// bytes of the second source, 4-7 of the first, 0x0c gives zero).
+DDI uint32_t pack_low_bytes(int v0, int v1, int v2, int v3)
+{
+    const uint32_t lo = __builtin_amdgcn_perm((uint32_t)v1, (uint32_t)v0, 0x0c0c0400u);
+    const uint32_t hi = __builtin_amdgcn_perm((uint32_t)v3, (uint32_t)v2, 0x0c0c0400u);
+    return __builtin_amdgcn_perm(hi, lo, 0x05040100u);
+}
+
+// Bytewise negation modulo 256 of a packed word: 0x80 - (b & 0x7f) never borrows from the
+// next byte, and flipping bit 7 where b's bit 7 is clear turns it into -b.
+DDI uint32_t neg_bytes(uint32_t x)
+{
+    return (0x80808080u - (x & 0x7f7f7f7fu)) ^ (~x & 0x80808080u);
 }
 
 DDI void comp(const ddv &x, int part, double &hi, double &lo)
@@ -96,8 +113,8 @@ DDI void oz_split_unit(const T *__restrict__ X, int64_t ld, int64_t K,
         if (k < K && !bad) comp(KC ? X[r * ld + k] : X[k * ld + r], part, hi, lo);
         int_chunks(hi, lo, sigma, c0[s], c1[s], c2[s]);
     }
-    // The sign s1 is applied to the chunks once; out2 differs from out1 by the sign mask g
-    // ((v ^ g) - g negates when g is -1).  Every modulus but 256 is odd, so the symmetric
+    // The sign s1 is applied to the chunks once; out2 differs from out1 by a bytewise
+    // negation when the signs differ.  Every modulus but 256 is odd, so the symmetric
     // residue of -x is minus that of x; for 256 both are -x mod 256 as a byte.
     if (s1 < 0) {
 #pragma unroll
@@ -107,16 +124,18 @@ DDI void oz_split_unit(const T *__restrict__ X, int64_t ld, int64_t K,
             c2[s] = -c2[s];
         }
     }
-    const int g = (s1 < 0) != (s2 < 0) ? -1 : 0;
+    const bool flip = (s1 < 0) != (s2 < 0);
     for (int t = 0; t < nmod; ++t) {
         const ModConst mc = mcs[t];
-        const int mi = (int)mc.m, half = (mi - 1) >> 1;
-        uint32_t w1[4] = {0, 0, 0, 0}, w2[4] = {0, 0, 0, 0};
+        int res[16];
 #pragma unroll
-        for (int s = 0; s < 16; ++s) {
-            const int res = sym_residue(c0[s], c1[s], c2[s], mc, mi, half);
-            w1[s >> 2] |= ((uint32_t)res & 0xffu) << (8 * (s & 3));
-            if constexpr (DUAL) w2[s >> 2] |= ((uint32_t)((res ^ g) - g) & 0xffu) << (8 * (s & 3));
+        for (int s = 0; s < 16; ++s) res[s] = sym_residue(c0[s], c1[s], c2[s], mc);
+        uint32_t w1[4], w2[4];
+        (void)w2;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            w1[q] = pack_low_bytes(res[4 * q], res[4 * q + 1], res[4 * q + 2], res[4 * q + 3]);
+            if constexpr (DUAL) w2[q] = flip ? neg_bytes(w1[q]) : w1[q];
         }
         const int64_t off = t * mod_stride + r * kp + kg * 16;
         *reinterpret_cast<uint4 *>(out1 + off) = make_uint4(w1[0], w1[1], w1[2], w1[3]);

@@ -105,37 +105,43 @@ def test_rounded_up_reciprocal_gives_exact_quotients():
                 assert math.floor(float(d) * inv) == d // m, (m, d)
 
 
-def _sym_residue(s, m, inv):
+def _sym_residue(s, m):
     """The split kernel's residue step in binary64: s is an exact integer (|s| < 2^45), the
-    FMAs around the quotient are exact integer operations, only s * inv rounds."""
-    q = math.floor(float(s) * inv)
-    r = s - q * m
-    half = (m - 1) >> 1
-    return r - m if r > half else r
+    FMA after the quotient is an exact integer operation, only s * fl(1/m) rounds, and the
+    quotient is rounded to the nearest integer (ties to even, as rint)."""
+    q = round(float(s) * (1.0 / m))
+    return s - q * m
 
 
 def test_symmetric_residue_is_exact_for_both_signs():
-    """Split residues need no non-negative offset: floor(s * RU(1/m)) is floor(s/m) for
-    s >= 0, and for s < 0 it is one lower exactly when m divides s, where the remainder m is
-    mapped to 0 by the symmetric step.  Checked on random and worst-case integers of either
-    sign over the whole chunk range |s| < 2^45 (c0 < 2^35, |c1| < 2^36, |c2| <= 2^34, chunk
-    weights below 256), for every modulus."""
+    """r = s - m*rint(s*fl(1/m)) needs no offset and no fix-up: s*fl(1/m) is within
+    2^-7/m of s/m, and for odd m every s/m is at least 1/(2m) from a half-integer, so r is
+    the unique residue in [-(m-1)/2, (m-1)/2]; for m = 256 the product is exact and a tie
+    gives +-128, whose byte is 0x80 either way.  Checked over the whole chunk range
+    |s| < 2^45 (c0 < 2^35, |c1| < 2^36, |c2| <= 2^34, chunk weights below 256) for every
+    modulus, with the integers next to every half-integer multiple of m, against the
+    canonical residue's byte (the int8 the residue GEMM reads)."""
     m_all, _ = mwblas.ozaki_moduli(40)
     rng = random.Random(11)
     lim = 2 ** 45
     for m in (int(v) for v in m_all):
-        inv = _inv_up(m)
         top = (lim - 1) // m
+        h = m // 2
         cases = [rng.randrange(-lim + 1, lim) for _ in range(2000)]
-        cases += [q * m + d for q in (0, 1, 2, top - 1, top) for d in (-1, 0, 1)]
-        cases += [rng.randrange(0, top) * m + d for d in (-1, 0, 1, m - 1) for _ in range(100)]
+        cases += [q * m + d for q in (0, 1, 2, top - 1, top) for d in (-1, 0, 1, h, h + 1)]
+        cases += [rng.randrange(0, top) * m + d for d in (-1, 0, 1, h, h + 1, m - 1)
+                  for _ in range(100)]
         cases += [-c for c in cases]
-        lo_r, hi_r = (-128, 127) if m == 256 else (-((m - 1) // 2), (m - 1) // 2)
+        bound = 128 if m == 256 else (m - 1) // 2
         for s in cases:
             if abs(s) >= lim:
                 continue
-            r = _sym_residue(s, m, inv)
-            assert (r - s) % m == 0 and lo_r <= r <= hi_r, (m, s, r)
+            r = _sym_residue(s, m)
+            assert (r - s) % m == 0 and -bound <= r <= bound, (m, s, r)
+            canon = s % m
+            canon = canon - m if canon > (m - 1) // 2 else canon
+            assert (r & 0xff) == (canon & 0xff), (m, s, r, canon)
+            assert ((-r) & 0xff) == ((-canon) & 0xff), (m, s, r, canon)
 
 
 def _chunks(hi, lo, sigma):
