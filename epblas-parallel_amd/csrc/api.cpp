@@ -121,9 +121,12 @@ DevState *dev_state(int dev)
     return &g_dev[dev];
 }
 
+// The host-pointer entry points' stream.  It is a blocking stream: its work is ordered
+// after what the caller queued on HIP's default stream (device-pointer operands of a
+// drop-in call produced there are complete before they are read).
 int ensure_stream(DevState *st)
 {
-    if (!st->stream) MW_TRY(hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking));
+    if (!st->stream) MW_TRY(hipStreamCreateWithFlags(&st->stream, hipStreamDefault));
     return 0;
 }
 
@@ -415,6 +418,18 @@ int64_t gemmtr_nb(bool cplx)
     return cplx ? nb_cplx : nb_real;
 }
 
+// Rectangle tile of the real triangular-output GEMM: MGEMMTR_TILE=64 (64 x 64) or 32
+// (32 x 64); unset: chosen per call.  Tiling does not change any result bit.
+int gemmtr_tile()
+{
+    static const int v = [] {
+        const char *e = getenv("MGEMMTR_TILE");
+        const long t = e ? atol(e) : 0;
+        return t == 64 ? 0 : t == 32 ? 1 : -1;
+    }();
+    return v;
+}
+
 int uplo_upper(char uplo)
 {
     const char u = (char)toupper((unsigned char)uplo);
@@ -437,6 +452,7 @@ int gemmtr_real_device(GemmArgsR g, char uplo, hipStream_t s)
         return record_error(-3, "mgemmtr: leading dimension too small (lda=%lld ldb=%lld ldc=%lld)",
                             (long long)g.lda, (long long)g.ldb, (long long)g.ldc);
     g.beta_mode = dd_is0(g.beta) ? BETA_ZERO : dd_is1(g.beta) ? BETA_ONE : BETA_SCALE;
+    g.variant = gemmtr_tile();
     MW_TRY(plain_gemmtr_real(g, upper == 1, gemmtr_nb(false), s));
     return 0;
 }
@@ -811,18 +827,6 @@ int wgemv_host(char trans, int64_t m, int64_t n, const mw_complex64x2 *alpha,
                              });
 }
 
-hipStream_t resolve_stream(void *stream, int *rc)
-{
-    *rc = 0;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    if (s) return s;
-    int dev = 0;
-    if ((*rc = current_device(&dev))) return nullptr;
-    DevState *st = dev_state(dev);
-    if ((*rc = ensure_stream(st))) return nullptr;
-    return st->stream;
-}
-
 void finish(int rc)
 {
     if (rc != 0 && abort_on_error()) {
@@ -994,9 +998,7 @@ int mwgemm_mgemmtr_device(char uplo, char transa, char transb, int64_t n, int64_
                           const mw_float64x2 *dB, int64_t ldb, const mw_float64x2 *beta,
                           mw_float64x2 *dC, int64_t ldc, void *stream)
 {
-    int rc = 0;
-    hipStream_t s = resolve_stream(stream, &rc);
-    if (rc) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);   // NULL: HIP's default stream
     return gemmtr_real_device(real_args(transa, transb, n, n, k, alpha, dA, lda, dB, ldb, beta,
                                         dC, ldc), uplo, s);
 }
@@ -1006,9 +1008,7 @@ int mwgemm_wgemmtr_device(char uplo, char transa, char transb, int64_t n, int64_
                           const mw_complex64x2 *dB, int64_t ldb, const mw_complex64x2 *beta,
                           mw_complex64x2 *dC, int64_t ldc, void *stream)
 {
-    int rc = 0;
-    hipStream_t s = resolve_stream(stream, &rc);
-    if (rc) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);   // NULL: HIP's default stream
     return gemmtr_cplx_device(cplx_args(transa, transb, n, n, k, alpha, dA, lda, dB, ldb, beta,
                                         dC, ldc), uplo, s);
 }
@@ -1050,9 +1050,7 @@ int mwgemm_mgemv_device(char trans, int64_t m, int64_t n, const mw_float64x2 *al
 {
     const int op = op_real(trans);
     if (int rc = gemv_check("mgemv", op, m, n, lda, incx, incy)) return rc;
-    int rc = 0;
-    hipStream_t s = resolve_stream(stream, &rc);
-    if (rc) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);   // NULL: HIP's default stream
     if (nbands <= 0) nbands = op == OP_N ? gemv_bands(m) : 1;
     MW_TRY(gemv_real(op, m, n, ddv{alpha->limbs[0], alpha->limbs[1]},
                      ddv{beta->limbs[0], beta->limbs[1]}, reinterpret_cast<const ddv *>(dA), lda,
@@ -1068,9 +1066,7 @@ int mwgemm_wgemv_device(char trans, int64_t m, int64_t n, const mw_complex64x2 *
 {
     const int op = op_cplx(trans);
     if (int rc = gemv_check("wgemv", op, m, n, lda, incx, incy)) return rc;
-    int rc = 0;
-    hipStream_t s = resolve_stream(stream, &rc);
-    if (rc) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);   // NULL: HIP's default stream
     const zddv al{{alpha->re.limbs[0], alpha->re.limbs[1]}, {alpha->im.limbs[0], alpha->im.limbs[1]}};
     const zddv be{{beta->re.limbs[0], beta->re.limbs[1]}, {beta->im.limbs[0], beta->im.limbs[1]}};
     MW_TRY(gemv_cplx(op, m, n, al, be, reinterpret_cast<const zddv *>(dA), lda,
@@ -1136,14 +1132,7 @@ int mwgemm_mgemm_device(char transa, char transb, int64_t m, int64_t n, int64_t 
     g.A = reinterpret_cast<const ddv *>(dA); g.lda = lda;
     g.B = reinterpret_cast<const ddv *>(dB); g.ldb = ldb;
     g.C = reinterpret_cast<ddv *>(dC); g.ldc = ldc;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    if (!s) {
-        int dev = 0;
-        if (int rc = current_device(&dev)) return rc;
-        DevState *st = dev_state(dev);
-        if (int rc = ensure_stream(st)) return rc;
-        s = st->stream;
-    }
+    hipStream_t s = static_cast<hipStream_t>(stream);   // NULL: HIP's default stream
     return gemm_real_device(g, path, s);
 }
 
@@ -1160,14 +1149,7 @@ int mwgemm_wgemm_device(char transa, char transb, int64_t m, int64_t n, int64_t 
     g.A = reinterpret_cast<const zddv *>(dA); g.lda = lda;
     g.B = reinterpret_cast<const zddv *>(dB); g.ldb = ldb;
     g.C = reinterpret_cast<zddv *>(dC); g.ldc = ldc;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    if (!s) {
-        int dev = 0;
-        if (int rc = current_device(&dev)) return rc;
-        DevState *st = dev_state(dev);
-        if (int rc = ensure_stream(st)) return rc;
-        s = st->stream;
-    }
+    hipStream_t s = static_cast<hipStream_t>(stream);   // NULL: HIP's default stream
     return gemm_cplx_device(g, path, s);
 }
 

@@ -322,7 +322,8 @@ def gemv_device(cplx, trans, m, n, alpha, dA, lda, dx, incx, beta, dy, incy, nba
 def gemm_device(cplx, transa, transb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc,
                 path=PATH_PLAIN, stream=None):
     """Device-pointer GEMM (tensors on the GPU), asynchronous on ``stream`` (an int handle
-    such as ``torch.cuda.current_stream().cuda_stream``; None = the library's stream)."""
+    such as ``torch.cuda.current_stream().cuda_stream``; None or 0 = HIP's default stream,
+    which is torch's default stream)."""
     L = lib()
     w = 4 if cplx else 2
     al, be = _scalar(alpha, w), _scalar(beta, w)

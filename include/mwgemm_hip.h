@@ -185,9 +185,11 @@ MWGEMM_API int mwgemm_last_error(char *buf, size_t len);   /* last error code (0
 MWGEMM_API void mwgemm_clear_error(void);
 MWGEMM_API int mwgemm_set_abort(int on);                  /* returns the previous setting */
 
-/* Device-pointer GEMMs on an explicit stream (NULL = the library's stream).  alpha/beta
- * are host values.  Return 0 or a hipError_t code.  The call is asynchronous w.r.t. the
- * host; path is one of MWGEMM_PATH_*. */
+/* Device-pointer GEMMs on the given stream; NULL is HIP's default (null) stream, as for
+ * every HIP launch, so the call is ordered after work queued there.  alpha/beta are host
+ * values.  Return 0 or a hipError_t code.  The call is asynchronous w.r.t. the host; path
+ * is one of MWGEMM_PATH_*.  The device-pointer gemmtr, gemv and batched entry points below
+ * take their stream the same way. */
 MWGEMM_API int mwgemm_mgemm_device(char transa, char transb, int64_t m, int64_t n, int64_t k,
                         const mw_float64x2 *alpha, const mw_float64x2 *dA, int64_t lda,
                         const mw_float64x2 *dB, int64_t ldb, const mw_float64x2 *beta,
