@@ -848,7 +848,7 @@ double g_timer_ms[KC_COUNT];
 int64_t g_timer_cnt[KC_COUNT];
 const char *kClassNames[KC_COUNT] = {
     "plain_real", "plain_cplx", "beta_only", "oz_exponent", "oz_split", "oz_gemm", "oz_crt",
-    "batch_plain", "batch_oz_split", "batch_oz_gemm", "batch_oz_crt", "gemmtr_diag", "gemv"};
+    "batch_plain", "batch_oz_split", "batch_oz_gemm", "batch_oz_crt", "gemmtr", "gemv"};
 
 void drain_slots()
 {

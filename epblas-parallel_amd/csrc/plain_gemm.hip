@@ -24,17 +24,146 @@
 namespace mw {
 
 // ----------------------------------------------------------------------------------
+// Triangular-output GEMM (gemmtr): the nb x nb diagonal triangles, one lane per element,
+// with the reference's scalar faithful sequence (mgemmtr_serial.cpp:48-95 / complex
+// wgemmtr_serial.cpp:48-95): the beta scaling, then for op(A) = 'N'
+//     per l ascending: t = alpha * b_lj; skipped when t == 0; c += t * a_il
+// and otherwise  s = sum_l a_li * b_lj (ascending, faithful), c += alpha * s.
+// Lane (j, r): column j, r-th row of its block's triangle part of that column.
+// ----------------------------------------------------------------------------------
+DDI ddv mul_exact_any(const ddv &a, const ddv &b) { return dd_mul_exact(a, b); }
+DDI zddv mul_exact_any(const zddv &a, const zddv &b) { return zdd_mul_exact(a, b); }
+DDI ddv add_exact_any(const ddv &a, const ddv &b) { return dd_add_exact(a, b); }
+DDI zddv add_exact_any(const zddv &a, const zddv &b) { return zdd_add_exact(a, b); }
+DDI bool is_zero_any(const ddv &v) { return v.hi == 0.0 && v.lo == 0.0; }
+DDI bool is_zero_any(const zddv &v) { return is_zero_any(v.re) && is_zero_any(v.im); }
+// p ? a : b field by field (a select of whole structs goes through scratch memory).
+DDI ddv select_any(bool p, const ddv &a, const ddv &b)
+{
+    return ddv{p ? a.hi : b.hi, p ? a.lo : b.lo};
+}
+DDI zddv select_any(bool p, const zddv &a, const zddv &b)
+{
+    return zddv{select_any(p, a.re, b.re), select_any(p, a.im, b.im)};
+}
+
+DDI bool tri_diag_row(int64_t j, int64_t r, int64_t n, int64_t nb, bool upper, int64_t &i)
+{
+    const int64_t jc = (j / nb) * nb;
+    if (upper) {
+        i = jc + r;
+        return i <= j;
+    }
+    i = j + r;
+    return i < jc + nb && i < n;
+}
+
+// The chain over l is sequential, so operands are loaded DIAG_PF steps ahead of their use:
+// each lane then waits once per DIAG_PF steps on memory instead of once per step.  A batch
+// is branch-free (a zero t_l keeps c by a select, which also keeps a NaN or Inf of a
+// skipped column out of c), so the compiler issues all of its loads first.
+constexpr int DIAG_PF = 8;
+
+template <bool TA, typename V>
+DDI void tr_step(V &c, V &acc, const V &alpha, const V &a, const V &b)
+{
+    if constexpr (!TA) {
+        const V tl = mul_exact_any(alpha, b);
+        const V upd = add_exact_any(c, mul_exact_any(tl, a));
+        c = select_any(is_zero_any(tl), c, upd);
+    } else {
+        acc = add_exact_any(acc, mul_exact_any(a, b));
+    }
+}
+
+template <bool TA, bool TB>
+DDI void mgemmtr_diag_lane(int64_t t, int64_t n, int64_t k, int64_t nb, bool upper, ddv alpha,
+                           ddv beta, int beta_mode, const ddv *__restrict__ A, int64_t lda,
+                           const ddv *__restrict__ B, int64_t ldb, ddv *__restrict__ C,
+                           int64_t ldc)
+{
+    const int64_t j = t / nb, r = t % nb;
+    int64_t i;
+    if (j >= n || !tri_diag_row(j, r, n, nb, upper, i)) return;
+    ddv c = dd_beta_prepass(C[j * ldc + i], beta, beta_mode);
+    ddv acc{0.0, 0.0};
+    auto a_at = [&](int64_t l) { return TA ? A[i * lda + l] : A[l * lda + i]; };
+    auto b_at = [&](int64_t l) { return TB ? B[l * ldb + j] : B[j * ldb + l]; };
+    int64_t l0 = 0;
+    for (; l0 + DIAG_PF <= k; l0 += DIAG_PF) {
+        ddv av[DIAG_PF], bv[DIAG_PF];
+#pragma unroll
+        for (int q = 0; q < DIAG_PF; ++q) {
+            av[q] = a_at(l0 + q);
+            bv[q] = b_at(l0 + q);
+        }
+        __builtin_amdgcn_sched_barrier(0);   // all loads of the batch before its steps
+#pragma unroll
+        for (int q = 0; q < DIAG_PF; ++q) tr_step<TA>(c, acc, alpha, av[q], bv[q]);
+    }
+    for (; l0 < k; ++l0) tr_step<TA>(c, acc, alpha, a_at(l0), b_at(l0));
+    if (TA) c = dd_add_exact(c, dd_mul_exact(alpha, acc));
+    C[j * ldc + i] = c;
+}
+
+template <int OPA, int OPB>
+DDI void wgemmtr_diag_lane(int64_t t, int64_t n, int64_t k, int64_t nb, bool upper,
+                           zddv alpha, zddv beta, int beta_mode, const zddv *__restrict__ A,
+                           int64_t lda, const zddv *__restrict__ B, int64_t ldb,
+                           zddv *__restrict__ C, int64_t ldc)
+{
+    const int64_t j = t / nb, r = t % nb;
+    int64_t i;
+    if (j >= n || !tri_diag_row(j, r, n, nb, upper, i)) return;
+    zddv c = zdd_beta_prepass(C[j * ldc + i], beta, beta_mode);
+    zddv acc{{0.0, 0.0}, {0.0, 0.0}};
+    auto a_at = [&](int64_t l) {
+        const zddv a = OPA == OP_N ? A[l * lda + i] : A[i * lda + l];
+        return OPA == OP_C ? zdd_conj(a) : a;
+    };
+    auto b_at = [&](int64_t l) {
+        const zddv b = OPB == OP_N ? B[j * ldb + l] : B[l * ldb + j];
+        return OPB == OP_C ? zdd_conj(b) : b;
+    };
+    int64_t l0 = 0;
+    for (; l0 + DIAG_PF <= k; l0 += DIAG_PF) {
+        zddv av[DIAG_PF], bv[DIAG_PF];
+#pragma unroll
+        for (int q = 0; q < DIAG_PF; ++q) {
+            av[q] = a_at(l0 + q);
+            bv[q] = b_at(l0 + q);
+        }
+        __builtin_amdgcn_sched_barrier(0);   // all loads of the batch before its steps
+#pragma unroll
+        for (int q = 0; q < DIAG_PF; ++q) tr_step<OPA != OP_N>(c, acc, alpha, av[q], bv[q]);
+    }
+    for (; l0 < k; ++l0) tr_step<OPA != OP_N>(c, acc, alpha, a_at(l0), b_at(l0));
+    if (OPA != OP_N) c = zdd_add_exact(c, zdd_mul_exact(alpha, acc));
+    C[j * ldc + i] = c;
+}
+
+// ----------------------------------------------------------------------------------
 // Real kernel.
 // ----------------------------------------------------------------------------------
 template <int TM, int TN, int BK, bool TA, bool TB, int TRI = TRI_NONE, bool SCALAR = false>
 __global__ __launch_bounds__(256)
 void k_mgemm_plain(int64_t m, int64_t n, int64_t k, ddv alpha, ddv beta, int beta_mode,
                    const ddv *__restrict__ A, int64_t lda, const ddv *__restrict__ B,
-                   int64_t ldb, ddv *__restrict__ C, int64_t ldc, int tiles_m, int64_t tri_nb)
+                   int64_t ldb, ddv *__restrict__ C, int64_t ldc, int tiles_m, int rect_tiles,
+                   int64_t tri_nb)
 {
     constexpr int BM = 16 * TM, BN = 16 * TN;
     __shared__ __attribute__((aligned(16))) ddv As[BK][BM];
     __shared__ __attribute__((aligned(16))) ddv Bs[BK][BN];
+    if constexpr (TRI != TRI_NONE) {
+        // gemmtr: workgroups past the rectangle's tiles take the diagonal triangles.
+        if ((int)blockIdx.x >= rect_tiles) {
+            mgemmtr_diag_lane<TA, TB>((int64_t)(blockIdx.x - rect_tiles) * 256 + threadIdx.x,
+                                      n, k, tri_nb, TRI == TRI_UPPER, alpha, beta, beta_mode,
+                                      A, lda, B, ldb, C, ldc);
+            return;
+        }
+    }
     const int64_t i0 = (int64_t)(blockIdx.x % tiles_m) * BM;
     const int64_t j0 = (int64_t)(blockIdx.x / tiles_m) * BN;
     if constexpr (TRI != TRI_NONE)
@@ -50,12 +179,21 @@ template <int TM, int TN, int BK, int OPA, int OPB, int TRI = TRI_NONE, bool SCA
 __global__ __launch_bounds__(256)
 void k_wgemm_plain(int64_t m, int64_t n, int64_t k, zddv alpha, zddv beta, int beta_mode,
                    const zddv *__restrict__ A, int64_t lda, const zddv *__restrict__ B,
-                   int64_t ldb, zddv *__restrict__ C, int64_t ldc, int tiles_m, int64_t tri_nb)
+                   int64_t ldb, zddv *__restrict__ C, int64_t ldc, int tiles_m, int rect_tiles,
+                   int64_t tri_nb)
 {
     constexpr int BM = 16 * TM, BN = 16 * TN;
     static_assert(KCHUNK % BK == 0, "BK must divide the flush period");
     __shared__ __attribute__((aligned(32))) zddv As[BK][BM];
     __shared__ __attribute__((aligned(32))) zddv Bs[BK][BN];
+    if constexpr (TRI != TRI_NONE) {
+        if ((int)blockIdx.x >= rect_tiles) {
+            wgemmtr_diag_lane<OPA, OPB>((int64_t)(blockIdx.x - rect_tiles) * 256 + threadIdx.x,
+                                        n, k, tri_nb, TRI == TRI_UPPER, alpha, beta, beta_mode,
+                                        A, lda, B, ldb, C, ldc);
+            return;
+        }
+    }
 
     const int tid = threadIdx.x;
     const int tx = tid & 15, ty = tid >> 4;
@@ -165,118 +303,6 @@ void k_wgemm_beta_only(int64_t m, int64_t n, zddv beta, int beta_mode, zddv *__r
     *cp = zdd_beta_prepass(*cp, beta, beta_mode);
 }
 
-// ----------------------------------------------------------------------------------
-// Triangular-output GEMM (gemmtr): the nb x nb diagonal triangles, one lane per element,
-// with the reference's scalar faithful sequence (mgemmtr_serial.cpp:48-95 / complex
-// wgemmtr_serial.cpp:48-95): the beta scaling, then for op(A) = 'N'
-//     per l ascending: t = alpha * b_lj; skipped when t == 0; c += t * a_il
-// and otherwise  s = sum_l a_li * b_lj (ascending, faithful), c += alpha * s.
-// Lane (j, r): column j, r-th row of its block's triangle part of that column.
-// ----------------------------------------------------------------------------------
-DDI ddv mul_exact_any(const ddv &a, const ddv &b) { return dd_mul_exact(a, b); }
-DDI zddv mul_exact_any(const zddv &a, const zddv &b) { return zdd_mul_exact(a, b); }
-DDI ddv add_exact_any(const ddv &a, const ddv &b) { return dd_add_exact(a, b); }
-DDI zddv add_exact_any(const zddv &a, const zddv &b) { return zdd_add_exact(a, b); }
-DDI bool is_zero_any(const ddv &v) { return v.hi == 0.0 && v.lo == 0.0; }
-DDI bool is_zero_any(const zddv &v) { return is_zero_any(v.re) && is_zero_any(v.im); }
-
-DDI bool tri_diag_row(int64_t j, int64_t r, int64_t n, int64_t nb, bool upper, int64_t &i)
-{
-    const int64_t jc = (j / nb) * nb;
-    if (upper) {
-        i = jc + r;
-        return i <= j;
-    }
-    i = j + r;
-    return i < jc + nb && i < n;
-}
-
-// The chain over l is sequential, so operands are loaded DIAG_PF steps ahead of their use:
-// each lane then waits once per DIAG_PF steps on memory instead of once per step.  A batch
-// is branch-free (a zero t_l keeps c by a select, which also keeps a NaN or Inf of a
-// skipped column out of c), so the compiler issues all of its loads first.
-constexpr int DIAG_PF = 8;
-
-template <bool TA, typename V>
-DDI void tr_step(V &c, V &acc, const V &alpha, const V &a, const V &b)
-{
-    if constexpr (!TA) {
-        const V tl = mul_exact_any(alpha, b);
-        const V upd = add_exact_any(c, mul_exact_any(tl, a));
-        c = is_zero_any(tl) ? c : upd;
-    } else {
-        acc = add_exact_any(acc, mul_exact_any(a, b));
-    }
-}
-
-template <bool TA, bool TB>
-__global__ __launch_bounds__(256)
-void k_mgemmtr_diag(int64_t n, int64_t k, int64_t nb, int upper, ddv alpha, ddv beta,
-                    int beta_mode, const ddv *__restrict__ A, int64_t lda,
-                    const ddv *__restrict__ B, int64_t ldb, ddv *__restrict__ C, int64_t ldc)
-{
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t j = t / nb, r = t % nb;
-    int64_t i;
-    if (j >= n || !tri_diag_row(j, r, n, nb, upper != 0, i)) return;
-    ddv c = dd_beta_prepass(C[j * ldc + i], beta, beta_mode);
-    ddv acc{0.0, 0.0};
-    auto a_at = [&](int64_t l) { return TA ? A[i * lda + l] : A[l * lda + i]; };
-    auto b_at = [&](int64_t l) { return TB ? B[l * ldb + j] : B[j * ldb + l]; };
-    int64_t l0 = 0;
-    for (; l0 + DIAG_PF <= k; l0 += DIAG_PF) {
-        ddv av[DIAG_PF], bv[DIAG_PF];
-#pragma unroll
-        for (int q = 0; q < DIAG_PF; ++q) {
-            av[q] = a_at(l0 + q);
-            bv[q] = b_at(l0 + q);
-        }
-        __builtin_amdgcn_sched_barrier(0);   // all loads of the batch before its steps
-#pragma unroll
-        for (int q = 0; q < DIAG_PF; ++q) tr_step<TA>(c, acc, alpha, av[q], bv[q]);
-    }
-    for (; l0 < k; ++l0) tr_step<TA>(c, acc, alpha, a_at(l0), b_at(l0));
-    if (TA) c = dd_add_exact(c, dd_mul_exact(alpha, acc));
-    C[j * ldc + i] = c;
-}
-
-template <int OPA, int OPB>
-__global__ __launch_bounds__(256)
-void k_wgemmtr_diag(int64_t n, int64_t k, int64_t nb, int upper, zddv alpha, zddv beta,
-                    int beta_mode, const zddv *__restrict__ A, int64_t lda,
-                    const zddv *__restrict__ B, int64_t ldb, zddv *__restrict__ C, int64_t ldc)
-{
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t j = t / nb, r = t % nb;
-    int64_t i;
-    if (j >= n || !tri_diag_row(j, r, n, nb, upper != 0, i)) return;
-    zddv c = zdd_beta_prepass(C[j * ldc + i], beta, beta_mode);
-    zddv acc{{0.0, 0.0}, {0.0, 0.0}};
-    auto a_at = [&](int64_t l) {
-        const zddv a = OPA == OP_N ? A[l * lda + i] : A[i * lda + l];
-        return OPA == OP_C ? zdd_conj(a) : a;
-    };
-    auto b_at = [&](int64_t l) {
-        const zddv b = OPB == OP_N ? B[j * ldb + l] : B[l * ldb + j];
-        return OPB == OP_C ? zdd_conj(b) : b;
-    };
-    int64_t l0 = 0;
-    for (; l0 + DIAG_PF <= k; l0 += DIAG_PF) {
-        zddv av[DIAG_PF], bv[DIAG_PF];
-#pragma unroll
-        for (int q = 0; q < DIAG_PF; ++q) {
-            av[q] = a_at(l0 + q);
-            bv[q] = b_at(l0 + q);
-        }
-        __builtin_amdgcn_sched_barrier(0);   // all loads of the batch before its steps
-#pragma unroll
-        for (int q = 0; q < DIAG_PF; ++q) tr_step<OPA != OP_N>(c, acc, alpha, av[q], bv[q]);
-    }
-    for (; l0 < k; ++l0) tr_step<OPA != OP_N>(c, acc, alpha, a_at(l0), b_at(l0));
-    if (OPA != OP_N) c = zdd_add_exact(c, zdd_mul_exact(alpha, acc));
-    C[j * ldc + i] = c;
-}
-
 // alpha == 0 or k == 0: the triangle's beta scaling alone (mgemmtr_parallel.cpp:43-52).
 template <typename V>
 __global__ __launch_bounds__(256)
@@ -305,11 +331,13 @@ hipError_t launch_real_tile(const GemmArgsR &g, hipStream_t s, int64_t tri_nb = 
     constexpr int BM = 16 * TM, BN = 16 * TN;
     int tiles_m = (int)((g.m + BM - 1) / BM);
     int64_t tiles = (int64_t)tiles_m * ((g.n + BN - 1) / BN);
-    if (tiles > INT32_MAX) return hipErrorInvalidValue;
-    ScopedTimer tm(KC_PLAIN_REAL, s);
-    hipLaunchKernelGGL((k_mgemm_plain<TM, TN, RBK, TA, TB, TRI, SCALAR>), dim3((unsigned)tiles),
-                       dim3(256), 0, s, g.m, g.n, g.k, g.alpha, g.beta, g.beta_mode, g.A, g.lda,
-                       g.B, g.ldb, g.C, g.ldc, tiles_m, tri_nb);
+    const int64_t diag_blocks = TRI != TRI_NONE ? (g.n * tri_nb + 255) / 256 : 0;
+    if (tiles + diag_blocks > INT32_MAX) return hipErrorInvalidValue;
+    ScopedTimer tm(TRI != TRI_NONE ? KC_GEMMTR : KC_PLAIN_REAL, s);
+    hipLaunchKernelGGL((k_mgemm_plain<TM, TN, RBK, TA, TB, TRI, SCALAR>),
+                       dim3((unsigned)(tiles + diag_blocks)), dim3(256), 0, s, g.m, g.n, g.k,
+                       g.alpha, g.beta, g.beta_mode, g.A, g.lda, g.B, g.ldb, g.C, g.ldc, tiles_m,
+                       (int)tiles, tri_nb);
     return hipGetLastError();
 }
 
@@ -334,11 +362,13 @@ hipError_t launch_cplx_tile(const GemmArgsZ &g, hipStream_t s, int64_t tri_nb)
     constexpr int BM = 16 * CTM, BN = 16 * CTN;
     int tiles_m = (int)((g.m + BM - 1) / BM);
     int64_t tiles = (int64_t)tiles_m * ((g.n + BN - 1) / BN);
-    if (tiles > INT32_MAX) return hipErrorInvalidValue;
-    ScopedTimer tm(KC_PLAIN_CPLX, s);
-    hipLaunchKernelGGL((k_wgemm_plain<CTM, CTN, CBK, OA, OB, TRI, SCALAR>), dim3((unsigned)tiles),
-                       dim3(256), 0, s, g.m, g.n, g.k, g.alpha, g.beta, g.beta_mode, g.A, g.lda,
-                       g.B, g.ldb, g.C, g.ldc, tiles_m, tri_nb);
+    const int64_t diag_blocks = TRI != TRI_NONE ? (g.n * tri_nb + 255) / 256 : 0;
+    if (tiles + diag_blocks > INT32_MAX) return hipErrorInvalidValue;
+    ScopedTimer tm(TRI != TRI_NONE ? KC_GEMMTR : KC_PLAIN_CPLX, s);
+    hipLaunchKernelGGL((k_wgemm_plain<CTM, CTN, CBK, OA, OB, TRI, SCALAR>),
+                       dim3((unsigned)(tiles + diag_blocks)), dim3(256), 0, s, g.m, g.n, g.k,
+                       g.alpha, g.beta, g.beta_mode, g.A, g.lda, g.B, g.ldb, g.C, g.ldc, tiles_m,
+                       (int)tiles, tri_nb);
     return hipGetLastError();
 }
 
@@ -369,41 +399,27 @@ int gemmtr_tile_variant(const GemmArgsR &g)
     return g.variant == 0 ? 0 : 1;
 }
 
-// gemmtr: the masked rectangle kernel, then the diagonal triangles.
+// gemmtr in one launch: the masked rectangle tiles, then (workgroups past the tiles) the
+// diagonal triangles, one lane per element.  The two parts write disjoint elements, and the
+// diagonal workgroups, dispatched last, fill the slots that the rectangle's final, partly
+// filled round of tiles leaves idle.
 template <bool TA, bool TB>
 hipError_t launch_gemmtr_real(const GemmArgsR &g, bool upper, int64_t nb, hipStream_t s)
 {
-    hipError_t e;
     if (g.order == ORDER_SCALAR)
-        e = upper ? launch_real_tile<RTM, RTN, TA, TB, TRI_UPPER, true>(g, s, nb)
-                  : launch_real_tile<RTM, RTN, TA, TB, TRI_LOWER, true>(g, s, nb);
-    else if (gemmtr_tile_variant(g) == 1)
-        e = upper ? launch_real_tile<2, 4, TA, TB, TRI_UPPER>(g, s, nb)
-                  : launch_real_tile<2, 4, TA, TB, TRI_LOWER>(g, s, nb);
-    else
-        e = upper ? launch_real_tile<RTM, RTN, TA, TB, TRI_UPPER>(g, s, nb)
-                  : launch_real_tile<RTM, RTN, TA, TB, TRI_LOWER>(g, s, nb);
-    if (e != hipSuccess) return e;
-    ScopedTimer tm(KC_GEMMTR_DIAG, s);
-    const int64_t lanes = g.n * nb;
-    hipLaunchKernelGGL((k_mgemmtr_diag<TA, TB>), dim3((unsigned)((lanes + 255) / 256)),
-                       dim3(256), 0, s, g.n, g.k, nb, upper ? 1 : 0, g.alpha, g.beta,
-                       g.beta_mode, g.A, g.lda, g.B, g.ldb, g.C, g.ldc);
-    return hipGetLastError();
+        return upper ? launch_real_tile<RTM, RTN, TA, TB, TRI_UPPER, true>(g, s, nb)
+                     : launch_real_tile<RTM, RTN, TA, TB, TRI_LOWER, true>(g, s, nb);
+    if (gemmtr_tile_variant(g) == 1)
+        return upper ? launch_real_tile<2, 4, TA, TB, TRI_UPPER>(g, s, nb)
+                     : launch_real_tile<2, 4, TA, TB, TRI_LOWER>(g, s, nb);
+    return upper ? launch_real_tile<RTM, RTN, TA, TB, TRI_UPPER>(g, s, nb)
+                 : launch_real_tile<RTM, RTN, TA, TB, TRI_LOWER>(g, s, nb);
 }
 
 template <int OA, int OB>
 hipError_t launch_gemmtr_cplx(const GemmArgsZ &g, bool upper, int64_t nb, hipStream_t s)
 {
-    hipError_t e = upper ? launch_cplx<OA, OB, TRI_UPPER>(g, s, nb)
-                         : launch_cplx<OA, OB, TRI_LOWER>(g, s, nb);
-    if (e != hipSuccess) return e;
-    ScopedTimer tm(KC_GEMMTR_DIAG, s);
-    const int64_t lanes = g.n * nb;
-    hipLaunchKernelGGL((k_wgemmtr_diag<OA, OB>), dim3((unsigned)((lanes + 255) / 256)),
-                       dim3(256), 0, s, g.n, g.k, nb, upper ? 1 : 0, g.alpha, g.beta,
-                       g.beta_mode, g.A, g.lda, g.B, g.ldb, g.C, g.ldc);
-    return hipGetLastError();
+    return upper ? launch_cplx<OA, OB, TRI_UPPER>(g, s, nb) : launch_cplx<OA, OB, TRI_LOWER>(g, s, nb);
 }
 
 template <int OA>

@@ -16,7 +16,7 @@ enum KernelClass {
     KC_BATCH_OZ_SPLIT,
     KC_BATCH_OZ_GEMM,
     KC_BATCH_OZ_CRT,
-    KC_GEMMTR_DIAG,
+    KC_GEMMTR,
     KC_GEMV,
     KC_COUNT
 };
