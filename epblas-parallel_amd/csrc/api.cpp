@@ -101,6 +101,13 @@ struct DevState {
     int pin_slot = 0;                            // chunk the next region is taken from
     size_t pin_off = 0;                          // first free byte in that chunk
     hipEvent_t dl_ev[2] = {nullptr, nullptr};    // completion of download chunks, in turn
+    // Column-panel pipeline of large host-pointer GEMMs: its upload and download streams,
+    // per-panel events, and pinned chunks of its own for the downloads (the upload ring's
+    // per-chunk events name one stream).
+    hipStream_t up = nullptr, dn = nullptr;
+    hipEvent_t up_ev[64] = {}, comp_ev[64] = {};
+    char *dpin[2] = {nullptr, nullptr};
+    hipEvent_t dpin_ev[2] = {nullptr, nullptr};
 };
 std::mutex g_state_mu;
 std::deque<DevState> g_dev;   // a deque keeps each device's state at a fixed address
@@ -500,6 +507,141 @@ int gemm_cplx_device(GemmArgsZ g, int path, hipStream_t s)
 // Host-pointer staging shared by the real and complex drop-in entry points.
 // ----------------------------------------------------------------------------------
 namespace {
+// Large host-pointer GEMMs on the plain path, in column panels of C.  A goes up first;
+// then, for each panel, its columns of op(B) (and of C when beta != 0) go up on the upload
+// stream while the previous panel computes, and each finished panel of C comes back on the
+// download stream while the next one computes.  A panel is a whole set of columns, and
+// every element of C is computed by the same per-element sequence as in one call, so the
+// result is bitwise that of one call.  All of A, B and C are host memory here.
+constexpr int MAX_PANELS = 64;
+
+int ensure_panel_state(DevState *st)
+{
+    if (!st->up) MW_TRY(hipStreamCreateWithFlags(&st->up, hipStreamDefault));
+    if (!st->dn) MW_TRY(hipStreamCreateWithFlags(&st->dn, hipStreamDefault));
+    for (int i = 0; i < MAX_PANELS; ++i) {
+        if (!st->up_ev[i]) MW_TRY(hipEventCreateWithFlags(&st->up_ev[i], hipEventDisableTiming));
+        if (!st->comp_ev[i]) MW_TRY(hipEventCreateWithFlags(&st->comp_ev[i], hipEventDisableTiming));
+    }
+    for (int i = 0; i < 2; ++i) {
+        if (!st->dpin_ev[i]) MW_TRY(hipEventCreateWithFlags(&st->dpin_ev[i], hipEventDisableTiming));
+        if (!st->dpin[i])
+            MW_TRY(hipHostMalloc(reinterpret_cast<void **>(&st->dpin[i]), PIN_CHUNK, hipHostMallocDefault));
+    }
+    return 0;
+}
+
+// Packed device columns (pitch seg) -> host (pitch hp) through the download chunks, on
+// stream dn: the DMA of chunk i+1 runs while chunk i is copied out.  Returns once the last
+// chunk is on the host.
+int download_panel(DevState *st, char *dst, size_t hp, const char *src, size_t seg, int64_t cols)
+{
+    const int64_t per = (int64_t)std::max<size_t>(1, PIN_CHUNK / seg);
+    if (seg > PIN_CHUNK) return record_error(-6, "host pipeline: a column of %zu bytes exceeds the pinned chunk", seg);
+    const int64_t nchunk = (cols + per - 1) / per;
+    auto issue = [&](int64_t i) -> int {
+        const int64_t c0 = i * per, nc = std::min(per, cols - c0);
+        MW_TRY(hipMemcpyAsync(st->dpin[i & 1], src + c0 * seg, (size_t)nc * seg,
+                              hipMemcpyDeviceToHost, st->dn));
+        MW_TRY(hipEventRecord(st->dpin_ev[i & 1], st->dn));
+        return 0;
+    };
+    if (int rc = issue(0)) return rc;
+    for (int64_t i = 0; i < nchunk; ++i) {
+        // chunk i+1 reuses the buffer of chunk i-1, already copied out
+        if (i + 1 < nchunk)
+            if (int rc = issue(i + 1)) return rc;
+        MW_TRY(hipEventSynchronize(st->dpin_ev[i & 1]));
+        const int64_t c0 = i * per, nc = std::min(per, cols - c0);
+        host_copy_columns(dst + c0 * hp, hp, st->dpin[i & 1], seg, seg, nc);
+    }
+    return 0;
+}
+
+// b_trans: op(B) is B^T or B^H, so a panel of op(B)'s columns is a band of B's rows.
+template <typename T, typename Args, typename Run>
+int staged_panels(Args g, int64_t a_rows, int64_t a_cols, bool b_trans, bool beta_zero, Run run)
+{
+    const size_t sz = sizeof(T);
+    const int64_t m = g.m, n = g.n, k = g.k;
+    const int64_t nb = std::max<int64_t>(256, ((n + 7) / 8 + 63) / 64 * 64);
+    const int64_t npan = (n + nb - 1) / nb;
+    if (npan > MAX_PANELS) return record_error(-3, "host pipeline: too many panels");
+    std::lock_guard<std::mutex> call_lk(g_call_mu);
+    int dev = 0;
+    if (int rc = current_device(&dev)) return rc;
+    DevState *st = dev_state(dev);
+    if (int rc = ensure_stream(st)) return rc;
+    if (int rc = ensure_panel_state(st)) return rc;
+    hipStream_t s = st->stream;
+    const size_t offA = 0;
+    const size_t offB = offA + pin_align((size_t)(a_rows * a_cols) * sz);
+    const size_t offC = offB + pin_align((size_t)(k * n) * sz);
+    if (int rc = ensure_stage(st, offC + pin_align((size_t)(m * n) * sz))) return rc;
+    char *base = static_cast<char *>(st->stage);
+    const char *hA = reinterpret_cast<const char *>(g.A);
+    const char *hB = reinterpret_cast<const char *>(g.B);
+    char *hC = reinterpret_cast<char *>(g.C);
+    const size_t lda = (size_t)g.lda * sz, ldb = (size_t)g.ldb * sz, ldc = (size_t)g.ldc * sz;
+    auto fail = [&](int rc) {
+        (void)hipStreamSynchronize(st->up);
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamSynchronize(st->dn);
+        return rc;
+    };
+    // panel p: upload its operands, then queue its computation behind them
+    auto issue = [&](int64_t p) -> int {
+        const int64_t j0 = p * nb, w = std::min(nb, n - j0);
+        char *dB = base + offB + (size_t)(j0 * k) * sz;
+        char *dC = base + offC + (size_t)(j0 * m) * sz;
+        if (b_trans) {
+            if (int rc = upload_pinned(st, st->up, dB, hB + (size_t)j0 * sz, ldb, (size_t)w * sz, k)) return rc;
+        } else {
+            if (int rc = upload_pinned(st, st->up, dB, hB + (size_t)j0 * ldb, ldb, (size_t)k * sz, w)) return rc;
+        }
+        if (!beta_zero)
+            if (int rc = upload_pinned(st, st->up, dC, hC + (size_t)j0 * ldc, ldc, (size_t)m * sz, w)) return rc;
+        MW_TRY(hipEventRecord(st->up_ev[p], st->up));
+        MW_TRY(hipStreamWaitEvent(s, st->up_ev[p], 0));
+        Args gp = g;
+        gp.n = w;
+        gp.A = reinterpret_cast<decltype(gp.A)>(base + offA);
+        gp.lda = a_rows;
+        gp.B = reinterpret_cast<decltype(gp.B)>(dB);
+        gp.ldb = b_trans ? w : k;
+        gp.C = reinterpret_cast<decltype(gp.C)>(dC);
+        gp.ldc = m;
+        if (int rc = run(gp, s)) return rc;
+        MW_TRY(hipEventRecord(st->comp_ev[p], s));
+        return 0;
+    };
+    if (int rc = upload_pinned(st, st->up, base + offA, hA, lda, (size_t)a_rows * sz, a_cols)) return fail(rc);
+    if (int rc = issue(0)) return fail(rc);
+    for (int64_t p = 0; p < npan; ++p) {
+        if (p + 1 < npan)
+            if (int rc = issue(p + 1)) return fail(rc);
+        const int64_t j0 = p * nb, w = std::min(nb, n - j0);
+        if (hipStreamWaitEvent(st->dn, st->comp_ev[p], 0) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(record_error(-1, "host pipeline: stream wait failed"));
+        }
+        if (int rc = download_panel(st, hC + (size_t)j0 * ldc, ldc,
+                                    base + offC + (size_t)(j0 * m) * sz, (size_t)m * sz, w))
+            return fail(rc);
+    }
+    return 0;
+}
+
+// Whether a host-pointer GEMM goes through the column-panel pipeline: A, B and C all in
+// host memory, a product of at least 2^30 multiply-adds, at least two 256-column panels.
+bool panel_worthy(const void *a, const void *b, const void *c, int64_t m, int64_t n, int64_t k,
+                  bool need_ab)
+{
+    if (!need_ab || n < 512 || (double)m * (double)n * (double)k < 1073741824.0) return false;
+    if (getenv("MWGEMM_HOST_PANELS") && atoi(getenv("MWGEMM_HOST_PANELS")) == 0) return false;
+    return !is_device_ptr(a) && !is_device_ptr(b) && !is_device_ptr(c);
+}
+
 template <typename T, typename Args, typename Run>
 int staged_call(Args g, int64_t a_rows, int64_t a_cols, int64_t b_rows, int64_t b_cols,
                 bool beta_zero, Run run)
@@ -591,6 +733,10 @@ int mgemm_host(char transa, char transb, int64_t m, int64_t n, int64_t k,
         return record_error(-3, "mgemm: leading dimension too small (lda=%lld ldb=%lld ldc=%lld)",
                             (long long)lda, (long long)ldb, (long long)ldc);
     const int path = default_path();
+    if (choose_path(path, m, n, k, false) == MWGEMM_PATH_PLAIN && m * (size_t)sizeof(ddv) <= PIN_CHUNK &&
+        panel_worthy(a, b, c, m, n, k, need_ab))
+        return staged_panels<ddv>(g, ar, ac, g.opb != OP_N, dd_is0(g.beta),
+                                  [](GemmArgsR gg, hipStream_t s) { return gemm_real_device(gg, MWGEMM_PATH_PLAIN, s); });
     return staged_call<ddv>(g, ar, ac, br, bc, dd_is0(g.beta),
                             [path](GemmArgsR gg, hipStream_t s) { return gemm_real_device(gg, path, s); });
 }
@@ -617,6 +763,10 @@ int wgemm_host(char transa, char transb, int64_t m, int64_t n, int64_t k,
         return record_error(-3, "wgemm: leading dimension too small (lda=%lld ldb=%lld ldc=%lld)",
                             (long long)lda, (long long)ldb, (long long)ldc);
     const int path = default_path();
+    if (choose_path(path, m, n, k, true) == MWGEMM_PATH_PLAIN && m * (size_t)sizeof(zddv) <= PIN_CHUNK &&
+        panel_worthy(a, b, c, m, n, k, need_ab))
+        return staged_panels<zddv>(g, ar, ac, g.opb != OP_N, zdd_is0(g.beta),
+                                   [](GemmArgsZ gg, hipStream_t s) { return gemm_cplx_device(gg, MWGEMM_PATH_PLAIN, s); });
     return staged_call<zddv>(g, ar, ac, br, bc, zdd_is0(g.beta),
                              [path](GemmArgsZ gg, hipStream_t s) { return gemm_cplx_device(gg, path, s); });
 }

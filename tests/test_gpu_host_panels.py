@@ -1,0 +1,73 @@
+"""Large host-pointer GEMMs on the plain path go through the column-panel pipeline: A up
+first, then per panel its columns of op(B) (rows of B when op(B) is transposed) and of C up,
+its computation, and its columns of C down, on three streams.
+
+Bars: bitwise equal to the same call through the device-pointer entry point and to the
+single-shot staging (MWGEMM_HOST_PANELS=0).  Cases cover real and complex, op(B) = 'N' and
+transposed, padded leading dimensions, beta = 0 and beta != 0, and a ragged last panel.
+C's padding rows must stay untouched."""
+import os
+
+import numpy as np
+import pytest
+
+import ddgen
+import mwblas
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(cplx, ta, tb, m, n, k, seed, beta_zero):
+    import torch
+    rng = np.random.default_rng(seed)
+    gen = ddgen.cplx_matrix if cplx else ddgen.real_matrix
+    scal = ddgen.cplx_scalar if cplx else ddgen.real_scalar
+    ar, ac = (m, k) if ta == "N" else (k, m)
+    br, bc = (k, n) if tb == "N" else (n, k)
+    lda, ldb, ldc = ar + 3, br + 5, m + 2
+    A, B, C0 = gen(rng, ar, ac, lda), gen(rng, br, bc, ldb), gen(rng, m, n, ldc)
+    al = scal(rng, scale=2)
+    be = np.zeros(4 if cplx else 2) if beta_zero else scal(rng, scale=2)
+    print(f"host panels {'wgemm' if cplx else 'mgemm'} {ta}{tb} m={m} n={n} k={k} "
+          f"beta_zero={beta_zero}", flush=True)
+    entry = mwblas.wgemm if cplx else mwblas.mgemm
+    args = (ta, tb, m, n, k, al, A, lda, B, ldb, be)
+    Cp = C0.copy()
+    entry(*args, Cp, ldc)                               # panel pipeline
+    prev = os.environ.get("MWGEMM_HOST_PANELS")
+    os.environ["MWGEMM_HOST_PANELS"] = "0"
+    try:
+        Cs = C0.copy()
+        entry(*args, Cs, ldc)                           # one staged call
+    finally:
+        if prev is None:
+            del os.environ["MWGEMM_HOST_PANELS"]
+        else:
+            os.environ["MWGEMM_HOST_PANELS"] = prev
+    dA, dB, dC = (torch.from_numpy(x).to("cuda:0") for x in (A, B, C0))
+    mwblas.gemm_device(cplx, ta, tb, m, n, k, al, dA, lda, dB, ldb, be, dC, ldc,
+                       path=mwblas.PATH_PLAIN, stream=0)
+    torch.cuda.synchronize()
+    Cd = dC.cpu().numpy()
+    assert np.array_equal(Cp, Cs, equal_nan=True)
+    assert np.array_equal(Cp, Cd, equal_nan=True)
+    assert np.array_equal(Cp[:, m:, :], C0[:, m:, :])
+
+
+@pytest.mark.parametrize("ta,tb,beta_zero", [("N", "N", False), ("T", "N", True),
+                                              ("N", "T", False), ("T", "T", True)])
+def test_real_panels_match_one_call(gpu, ta, tb, beta_zero):
+    prev = mwblas.set_path(mwblas.PATH_PLAIN)
+    try:
+        _run(False, ta, tb, 600, 1100, 1700, 1400 + 2 * (ta == "T") + (tb == "T"), beta_zero)
+    finally:
+        mwblas.set_path(prev)
+
+
+@pytest.mark.parametrize("ta,tb", [("C", "N"), ("N", "C")])
+def test_complex_panels_match_one_call(gpu, ta, tb):
+    prev = mwblas.set_path(mwblas.PATH_PLAIN)
+    try:
+        _run(True, ta, tb, 400, 700, 4000, 1410 + (tb == "C"), False)
+    finally:
+        mwblas.set_path(prev)
