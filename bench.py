@@ -357,8 +357,9 @@ def host_boundary_bench(size=4096, reps=3, seed=77):
     finally:
         mwblas.set_path(prev)
     moved = (A.nbytes + B.nbytes + C.nbytes)
-    # The default (plain) path on the same host operands: large calls go through the
-    # column-panel pipeline, which overlaps the panels' transfers with their computation.
+    # The default (plain) path on the same host operands.  Large calls on either path go
+    # through the column-panel pipeline, which overlaps the panels' transfers with their
+    # computation.
     prev = mwblas.set_path(mwblas.PATH_PLAIN)
     try:
         mwblas.mgemm("N", "N", m, n, k, [1.0, 0.0], A, m, B, k, [0.0, 0.0], C, m)

@@ -389,7 +389,7 @@ int choose_path(int path, int64_t m, int64_t n, int64_t k, bool cplx)
 // ----------------------------------------------------------------------------------
 // Device-resident cores.
 // ----------------------------------------------------------------------------------
-int gemm_real_device(GemmArgsR g, int path, hipStream_t s)
+int gemm_real_device(GemmArgsR g, int path, hipStream_t s, uint64_t a_tag)
 {
     g.order = plain_order();
     if (g.m <= 0 || g.n <= 0) return 0;
@@ -405,7 +405,7 @@ int gemm_real_device(GemmArgsR g, int path, hipStream_t s)
         MW_TRY(plain_beta_only_real(g, s));
         return 0;
     }
-    if (choose_path(path, g.m, g.n, g.k, false) == MWGEMM_PATH_OZAKI) return ozaki_gemm_real(g, s);
+    if (choose_path(path, g.m, g.n, g.k, false) == MWGEMM_PATH_OZAKI) return ozaki_gemm_real(g, s, a_tag);
     MW_TRY(plain_gemm_real(g, s));
     return 0;
 }
@@ -733,10 +733,15 @@ int mgemm_host(char transa, char transb, int64_t m, int64_t n, int64_t k,
         return record_error(-3, "mgemm: leading dimension too small (lda=%lld ldb=%lld ldc=%lld)",
                             (long long)lda, (long long)ldb, (long long)ldc);
     const int path = default_path();
-    if (choose_path(path, m, n, k, false) == MWGEMM_PATH_PLAIN && m * (size_t)sizeof(ddv) <= PIN_CHUNK &&
-        panel_worthy(a, b, c, m, n, k, need_ab))
+    const int eff = choose_path(path, m, n, k, false);
+    if (m * (size_t)sizeof(ddv) <= PIN_CHUNK && panel_worthy(a, b, c, m, n, k, need_ab)) {
+        // The path is fixed for the whole call; the exact-product panels share op(A)'s
+        // maxima and residues through a tag unique to this call.
+        static std::atomic<uint64_t> next_tag{0};
+        const uint64_t tag = eff == MWGEMM_PATH_OZAKI ? ++next_tag : 0;
         return staged_panels<ddv>(g, ar, ac, g.opb != OP_N, dd_is0(g.beta),
-                                  [](GemmArgsR gg, hipStream_t s) { return gemm_real_device(gg, MWGEMM_PATH_PLAIN, s); });
+                                  [eff, tag](GemmArgsR gg, hipStream_t s) { return gemm_real_device(gg, eff, s, tag); });
+    }
     return staged_call<ddv>(g, ar, ac, br, bc, dd_is0(g.beta),
                             [path](GemmArgsR gg, hipStream_t s) { return gemm_real_device(gg, path, s); });
 }

@@ -26,8 +26,6 @@
 
 namespace mw {
 
-int gemm_real_device(GemmArgsR g, int path, hipStream_t s);
-int gemm_cplx_device(GemmArgsZ g, int path, hipStream_t s);
 double batch_time_variant(int64_t count, char ta, char tb, const int64_t *m, const int64_t *n,
                           const int64_t *k, const mw_float64x2 *alpha,
                           const mw_float64x2 *const *dA, const int64_t *lda,

@@ -39,6 +39,10 @@ struct GemmArgsZ {
 enum PlainOrder { ORDER_SIMD = 0, ORDER_SCALAR = 1 };
 int plain_order();   // the current setting
 
+// Device-pointer GEMM cores (api.cpp).  a_tag: see ozaki_gemm_real.
+int gemm_real_device(GemmArgsR g, int path, hipStream_t s, uint64_t a_tag = 0);
+int gemm_cplx_device(GemmArgsZ g, int path, hipStream_t s);
+
 // Autotuner table (autotune.cpp).  Kinds of tuned decisions:
 enum TuneKind {
     TK_PLAIN_REAL = 0,   // variant: 0 = 64x64 tile, 1 = 32x64 tile

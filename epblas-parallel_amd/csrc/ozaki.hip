@@ -371,6 +371,11 @@ struct OzDevice {
     hipStream_t ws_stream = nullptr;
     bool ws_recorded = false;
     std::map<int, ModConst *> tables;   // keyed by modulus count
+    // The op(A) part of the workspace (maxima, then residues, at its start) as the last real
+    // call left it: its tag (0: no reuse), shape and modulus count.
+    uint64_t a_tag = 0;
+    int64_t a_m = 0, a_k = 0;
+    int a_nmod = 0, a_bits = 0;
 };
 std::mutex g_oz_mu;
 std::vector<OzDevice> g_oz;
@@ -397,8 +402,9 @@ bool stream_capturing(hipStream_t s)
 // device) for that call's completion event.  Under stream capture the workspace must
 // already be large enough (capture cannot free or synchronise), and captured work is
 // ordered only by how the graph is launched.
-int oz_workspace(OzDevice *d, size_t bytes, hipStream_t s, char **base)
+int oz_workspace(OzDevice *d, size_t bytes, hipStream_t s, char **base, bool *moved = nullptr)
 {
+    if (moved) *moved = false;
     const bool capturing = stream_capturing(s);
     if (d->ws_recorded && d->ws_stream != s && !capturing)
         OZ_TRY(hipStreamWaitEvent(s, d->ws_done, 0));
@@ -416,6 +422,7 @@ int oz_workspace(OzDevice *d, size_t bytes, hipStream_t s, char **base)
         }
         OZ_TRY(hipMalloc(&d->ws, bytes));
         d->ws_bytes = bytes;
+        if (moved) *moved = true;
     }
     *base = static_cast<char *>(d->ws);
     return 0;
@@ -548,7 +555,7 @@ hipError_t fill_zero(unsigned long long *p, int64_t n, hipStream_t s)
 
 }  // namespace
 
-int ozaki_gemm_real(const GemmArgsR &g, hipStream_t s)
+int ozaki_gemm_real(const GemmArgsR &g, hipStream_t s, uint64_t a_tag)
 {
     std::lock_guard<std::mutex> lk(g_oz_mu);
     OzPlan plan;
@@ -559,34 +566,51 @@ int ozaki_gemm_real(const GemmArgsR &g, hipStream_t s)
     ModConst *mcs;
     if (int rc = oz_table(d, plan, &mcs)) return rc;
 
-    // K padded to 128: the 256 x 256 residue tile consumes 128-deep slabs.
+    // K padded to 64 (the residue tiles consume 64-deep steps), rows and columns to 256.
+    // Layout: [A maxima | A residues | B maxima | B residues | reduced products]: the op(A)
+    // part comes first, so that it stays in place for calls with a different n.
     const int64_t Mp = round_up(g.m, 256), Np = round_up(g.n, 256), Kc = round_up(g.k, 64);
     const int nmod = plan.nmod;
     const size_t szA = (size_t)nmod * Mp * Kc, szB = (size_t)nmod * Np * Kc;
-    const size_t szU = (size_t)nmod * Np * Mp, szR = (size_t)(Mp + Np) * 8;
+    const size_t szU = (size_t)nmod * Np * Mp;
+    const size_t oRA = 0, oA = round_up(Mp * 8, 256), oRB = oA + round_up(szA, 256);
+    const size_t oB = oRB + round_up(Np * 8, 256), oU = oB + round_up(szB, 256);
     char *base;
-    if (int rc = oz_workspace(d, szR + szA + szB + szU + 1024, s, &base)) return rc;
-    unsigned long long *rA = reinterpret_cast<unsigned long long *>(base);
-    unsigned long long *rB = rA + Mp;
-    int8_t *resA = reinterpret_cast<int8_t *>(base + round_up(szR, 256));
-    int8_t *resB = resA + round_up(szA, 256);
-    uint8_t *Ub = reinterpret_cast<uint8_t *>(resB + round_up(szB, 256));
+    bool moved = false;
+    if (int rc = oz_workspace(d, oU + szU + 1024, s, &base, &moved)) return rc;
+    unsigned long long *rA = reinterpret_cast<unsigned long long *>(base + oRA);
+    unsigned long long *rB = reinterpret_cast<unsigned long long *>(base + oRB);
+    int8_t *resA = reinterpret_cast<int8_t *>(base + oA);
+    int8_t *resB = reinterpret_cast<int8_t *>(base + oB);
+    uint8_t *Ub = reinterpret_cast<uint8_t *>(base + oU);
+    const bool reuse_a = a_tag != 0 && !moved && d->a_tag == a_tag && d->a_m == g.m &&
+                         d->a_k == g.k && d->a_nmod == nmod && d->a_bits == plan.bits;
+    d->a_tag = 0;   // set again below once this call's op(A) part is enqueued
 
     const bool a_kc = g.opa != OP_N;   // op(A) row i contiguous in k when A is transposed
     const bool b_kc = g.opb == OP_N;   // op(B) column j contiguous in k when B is not
     {
         ScopedTimer tm(KC_OZ_EXPONENT, s);
-        OZ_TRY(fill_zero(rA, Mp + Np, s));
-        OZ_TRY(launch_rowmax(g.A, g.lda, g.m, g.k, a_kc, rA, s));
+        if (!reuse_a) {
+            OZ_TRY(fill_zero(rA, Mp, s));
+            OZ_TRY(launch_rowmax(g.A, g.lda, g.m, g.k, a_kc, rA, s));
+        }
+        OZ_TRY(fill_zero(rB, Np, s));
         OZ_TRY(launch_rowmax(g.B, g.ldb, g.n, g.k, b_kc, rB, s));
     }
     {
         ScopedTimer tm(KC_OZ_SPLIT, s);
-        OZ_TRY(launch_split(g.A, g.lda, g.m, g.k, Kc, a_kc, rA, plan.bits, 0, mcs, nmod, resA, 1,
-                            (int8_t *)nullptr, 0, Kc, Mp * Kc, s));
+        if (!reuse_a)
+            OZ_TRY(launch_split(g.A, g.lda, g.m, g.k, Kc, a_kc, rA, plan.bits, 0, mcs, nmod, resA,
+                                1, (int8_t *)nullptr, 0, Kc, Mp * Kc, s));
         OZ_TRY(launch_split(g.B, g.ldb, g.n, g.k, Kc, b_kc, rB, plan.bits, 0, mcs, nmod, resB, 1,
                             (int8_t *)nullptr, 0, Kc, Np * Kc, s));
     }
+    d->a_tag = a_tag;
+    d->a_m = g.m;
+    d->a_k = g.k;
+    d->a_nmod = nmod;
+    d->a_bits = plan.bits;
     {
         ScopedTimer tm(KC_OZ_GEMM, s);
         OZ_TRY(launch_gemm(resA, resB, Ub, Mp, Np, Kc, Mp * Kc, Np * Kc, Np * Mp, mcs, 1, nmod, g.variant, s));
@@ -610,6 +634,7 @@ int ozaki_gemm_cplx(const GemmArgsZ &g, hipStream_t s)
         return record_error(-5, "ozaki: contraction depth %lld out of range", (long long)g.k);
     OzDevice *d;
     if (int rc = oz_device(&d)) return rc;
+    d->a_tag = 0;   // this call's layout overwrites the real calls' op(A) part
     ModConst *mcs;
     if (int rc = oz_table(d, plan, &mcs)) return rc;
 

@@ -39,7 +39,11 @@ constexpr int64_t OZ_MAX_KEFF = 65536;
 
 bool ozaki_supported(int64_t k);
 bool ozaki_supported_cplx(int64_t k);
-int ozaki_gemm_real(const GemmArgsR &g, hipStream_t s);
+// a_tag != 0: the caller runs a sequence of calls that share op(A) (the column panels of one
+// host-pointer GEMM) and names it by a_tag.  A call whose tag matches the last call's on this
+// device, with the same m, k and modulus count and an unmoved workspace, reuses op(A)'s
+// maxima and residues instead of recomputing them; any other call recomputes them.
+int ozaki_gemm_real(const GemmArgsR &g, hipStream_t s, uint64_t a_tag = 0);
 int ozaki_gemm_cplx(const GemmArgsZ &g, hipStream_t s);
 
 // Test hook: U[j*ldu + i] = (sum_k A[i*kp + k] * B[j*kp + k]) mod m for one

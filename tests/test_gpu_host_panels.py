@@ -4,8 +4,9 @@ its computation, and its columns of C down, on three streams.
 
 Bars: bitwise equal to the same call through the device-pointer entry point and to the
 single-shot staging (MWGEMM_HOST_PANELS=0).  Cases cover real and complex, op(B) = 'N' and
-transposed, padded leading dimensions, beta = 0 and beta != 0, and a ragged last panel.
-C's padding rows must stay untouched."""
+transposed, padded leading dimensions, beta = 0 and beta != 0, a ragged last panel, and
+the real exact-product path (whose panels share op(A)'s maxima and residues).  C's padding
+rows must stay untouched."""
 import os
 
 import numpy as np
@@ -17,7 +18,7 @@ import mwblas
 pytestmark = pytest.mark.gpu
 
 
-def _run(cplx, ta, tb, m, n, k, seed, beta_zero):
+def _run(cplx, ta, tb, m, n, k, seed, beta_zero, path=mwblas.PATH_PLAIN):
     import torch
     rng = np.random.default_rng(seed)
     gen = ddgen.cplx_matrix if cplx else ddgen.real_matrix
@@ -29,7 +30,7 @@ def _run(cplx, ta, tb, m, n, k, seed, beta_zero):
     al = scal(rng, scale=2)
     be = np.zeros(4 if cplx else 2) if beta_zero else scal(rng, scale=2)
     print(f"host panels {'wgemm' if cplx else 'mgemm'} {ta}{tb} m={m} n={n} k={k} "
-          f"beta_zero={beta_zero}", flush=True)
+          f"beta_zero={beta_zero} path={path}", flush=True)
     entry = mwblas.wgemm if cplx else mwblas.mgemm
     args = (ta, tb, m, n, k, al, A, lda, B, ldb, be)
     Cp = C0.copy()
@@ -46,7 +47,7 @@ def _run(cplx, ta, tb, m, n, k, seed, beta_zero):
             os.environ["MWGEMM_HOST_PANELS"] = prev
     dA, dB, dC = (torch.from_numpy(x).to("cuda:0") for x in (A, B, C0))
     mwblas.gemm_device(cplx, ta, tb, m, n, k, al, dA, lda, dB, ldb, be, dC, ldc,
-                       path=mwblas.PATH_PLAIN, stream=0)
+                       path=path, stream=0)
     torch.cuda.synchronize()
     Cd = dC.cpu().numpy()
     assert np.array_equal(Cp, Cs, equal_nan=True)
@@ -69,5 +70,23 @@ def test_complex_panels_match_one_call(gpu, ta, tb):
     prev = mwblas.set_path(mwblas.PATH_PLAIN)
     try:
         _run(True, ta, tb, 400, 700, 4000, 1410 + (tb == "C"), False)
+    finally:
+        mwblas.set_path(prev)
+
+
+@pytest.mark.parametrize("ta,tb,beta_zero", [("N", "N", False), ("T", "T", True)])
+def test_exact_product_panels_share_a_and_match_one_call(gpu, ta, tb, beta_zero):
+    """The exact-product panels of one call reuse op(A)'s maxima and residues (a tag unique
+    to the call); every panel's entries are computed exactly as in one call."""
+    prev = mwblas.set_path(mwblas.PATH_OZAKI)
+    try:
+        _run(False, ta, tb, 600, 1100, 1700, 1420 + (ta == "T"), beta_zero, mwblas.PATH_OZAKI)
+        # a complex exact-product call in between overwrites the workspace's op(A) part;
+        # the next panelled call recomputes it under its own tag
+        rng = np.random.default_rng(1425)
+        A, B, C = (ddgen.cplx_matrix(rng, 40, 40) for _ in range(3))
+        mwblas.wgemm("N", "N", 40, 40, 40, ddgen.cplx_scalar(rng), A, 40, B, 40,
+                     ddgen.cplx_scalar(rng), C, 40)
+        _run(False, ta, tb, 600, 1100, 1700, 1430 + (ta == "T"), beta_zero, mwblas.PATH_OZAKI)
     finally:
         mwblas.set_path(prev)
