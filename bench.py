@@ -77,7 +77,8 @@ def cpu_baseline(A, B, n, k, target_s=20.0):
     busy = min(threads, (n + 255) // 256)   # the reference splits N into 256-wide panels
     return {"value": round(gflops, 3), "unit": "DD-GFLOP/s", "cores": busy, "kind": "port",
             "sample": f"rows 0..{rows} of the {A.shape[1]}x{n}x{k} NN problem (same inputs), "
-                      f"{t:.1f} s, OMP threads={threads} (jc split keeps {busy} busy), "
+                      f"{t:.1f} s after a 64-row warm-up, OMP threads={threads} (jc split keeps "
+                      f"{busy} busy), OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND', 'unset')}, "
                       f"AVX2={bool(L.port_has_avx2())}"}
 
 
@@ -615,4 +616,8 @@ def main():
 
 
 if __name__ == "__main__":
+    # The CPU baseline's OpenMP team runs pinned, one thread per core (the reference's
+    # benchmark protocol); OpenMP reads these when the oracle library loads.
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     main()
