@@ -482,7 +482,7 @@ int gemmtr_cplx_device(GemmArgsZ g, char uplo, hipStream_t s)
     return 0;
 }
 
-int gemm_cplx_device(GemmArgsZ g, int path, hipStream_t s)
+int gemm_cplx_device(GemmArgsZ g, int path, hipStream_t s, uint64_t a_tag)
 {
     g.order = plain_order();
     if (g.m <= 0 || g.n <= 0) return 0;
@@ -498,7 +498,7 @@ int gemm_cplx_device(GemmArgsZ g, int path, hipStream_t s)
         MW_TRY(plain_beta_only_cplx(g, s));
         return 0;
     }
-    if (choose_path(path, g.m, g.n, g.k, true) == MWGEMM_PATH_OZAKI) return ozaki_gemm_cplx(g, s);
+    if (choose_path(path, g.m, g.n, g.k, true) == MWGEMM_PATH_OZAKI) return ozaki_gemm_cplx(g, s, a_tag);
     MW_TRY(plain_gemm_cplx(g, s));
     return 0;
 }
@@ -632,6 +632,13 @@ int staged_panels(Args g, int64_t a_rows, int64_t a_cols, bool b_trans, bool bet
     return 0;
 }
 
+// A tag unique to one panelled host call (never 0).
+uint64_t next_panel_tag()
+{
+    static std::atomic<uint64_t> next{0};
+    return ++next;
+}
+
 // Whether a host-pointer GEMM goes through the column-panel pipeline: A, B and C all in
 // host memory, a product of at least 2^30 multiply-adds, at least two 256-column panels.
 bool panel_worthy(const void *a, const void *b, const void *c, int64_t m, int64_t n, int64_t k,
@@ -737,8 +744,7 @@ int mgemm_host(char transa, char transb, int64_t m, int64_t n, int64_t k,
     if (m * (size_t)sizeof(ddv) <= PIN_CHUNK && panel_worthy(a, b, c, m, n, k, need_ab)) {
         // The path is fixed for the whole call; the exact-product panels share op(A)'s
         // maxima and residues through a tag unique to this call.
-        static std::atomic<uint64_t> next_tag{0};
-        const uint64_t tag = eff == MWGEMM_PATH_OZAKI ? ++next_tag : 0;
+        const uint64_t tag = eff == MWGEMM_PATH_OZAKI ? next_panel_tag() : 0;
         return staged_panels<ddv>(g, ar, ac, g.opb != OP_N, dd_is0(g.beta),
                                   [eff, tag](GemmArgsR gg, hipStream_t s) { return gemm_real_device(gg, eff, s, tag); });
     }
@@ -768,10 +774,12 @@ int wgemm_host(char transa, char transb, int64_t m, int64_t n, int64_t k,
         return record_error(-3, "wgemm: leading dimension too small (lda=%lld ldb=%lld ldc=%lld)",
                             (long long)lda, (long long)ldb, (long long)ldc);
     const int path = default_path();
-    if (choose_path(path, m, n, k, true) == MWGEMM_PATH_PLAIN && m * (size_t)sizeof(zddv) <= PIN_CHUNK &&
-        panel_worthy(a, b, c, m, n, k, need_ab))
+    const int eff = choose_path(path, m, n, k, true);
+    if (m * (size_t)sizeof(zddv) <= PIN_CHUNK && panel_worthy(a, b, c, m, n, k, need_ab)) {
+        const uint64_t tag = eff == MWGEMM_PATH_OZAKI ? next_panel_tag() : 0;
         return staged_panels<zddv>(g, ar, ac, g.opb != OP_N, zdd_is0(g.beta),
-                                   [](GemmArgsZ gg, hipStream_t s) { return gemm_cplx_device(gg, MWGEMM_PATH_PLAIN, s); });
+                                   [eff, tag](GemmArgsZ gg, hipStream_t s) { return gemm_cplx_device(gg, eff, s, tag); });
+    }
     return staged_call<zddv>(g, ar, ac, br, bc, zdd_is0(g.beta),
                              [path](GemmArgsZ gg, hipStream_t s) { return gemm_cplx_device(gg, path, s); });
 }

@@ -41,7 +41,7 @@ int plain_order();   // the current setting
 
 // Device-pointer GEMM cores (api.cpp).  a_tag: see ozaki_gemm_real.
 int gemm_real_device(GemmArgsR g, int path, hipStream_t s, uint64_t a_tag = 0);
-int gemm_cplx_device(GemmArgsZ g, int path, hipStream_t s);
+int gemm_cplx_device(GemmArgsZ g, int path, hipStream_t s, uint64_t a_tag = 0);
 
 // Autotuner table (autotune.cpp).  Kinds of tuned decisions:
 enum TuneKind {

@@ -376,6 +376,7 @@ struct OzDevice {
     uint64_t a_tag = 0;
     int64_t a_m = 0, a_k = 0;
     int a_nmod = 0, a_bits = 0;
+    bool a_cplx = false;
 };
 std::mutex g_oz_mu;
 std::vector<OzDevice> g_oz;
@@ -584,7 +585,8 @@ int ozaki_gemm_real(const GemmArgsR &g, hipStream_t s, uint64_t a_tag)
     int8_t *resB = reinterpret_cast<int8_t *>(base + oB);
     uint8_t *Ub = reinterpret_cast<uint8_t *>(base + oU);
     const bool reuse_a = a_tag != 0 && !moved && d->a_tag == a_tag && d->a_m == g.m &&
-                         d->a_k == g.k && d->a_nmod == nmod && d->a_bits == plan.bits;
+                         d->a_k == g.k && d->a_nmod == nmod && d->a_bits == plan.bits &&
+                         !d->a_cplx;
     d->a_tag = 0;   // set again below once this call's op(A) part is enqueued
 
     const bool a_kc = g.opa != OP_N;   // op(A) row i contiguous in k when A is transposed
@@ -611,6 +613,7 @@ int ozaki_gemm_real(const GemmArgsR &g, hipStream_t s, uint64_t a_tag)
     d->a_k = g.k;
     d->a_nmod = nmod;
     d->a_bits = plan.bits;
+    d->a_cplx = false;
     {
         ScopedTimer tm(KC_OZ_GEMM, s);
         OZ_TRY(launch_gemm(resA, resB, Ub, Mp, Np, Kc, Mp * Kc, Np * Kc, Np * Mp, mcs, 1, nmod, g.variant, s));
@@ -626,7 +629,7 @@ int ozaki_gemm_real(const GemmArgsR &g, hipStream_t s, uint64_t a_tag)
     return oz_workspace_done(d, s);
 }
 
-int ozaki_gemm_cplx(const GemmArgsZ &g, hipStream_t s)
+int ozaki_gemm_cplx(const GemmArgsZ &g, hipStream_t s, uint64_t a_tag)
 {
     std::lock_guard<std::mutex> lk(g_oz_mu);
     OzPlan plan;
@@ -634,46 +637,64 @@ int ozaki_gemm_cplx(const GemmArgsZ &g, hipStream_t s)
         return record_error(-5, "ozaki: contraction depth %lld out of range", (long long)g.k);
     OzDevice *d;
     if (int rc = oz_device(&d)) return rc;
-    d->a_tag = 0;   // this call's layout overwrites the real calls' op(A) part
     ModConst *mcs;
     if (int rc = oz_table(d, plan, &mcs)) return rc;
 
     const int64_t Mp = round_up(g.m, 256), Np = round_up(g.n, 256), Kc = round_up(g.k, 64);
     const int64_t Kp = 2 * Kc;
     const int nmod = plan.nmod;
-    // batch b = 2t + part (part 0: real part of C, 1: imaginary part)
+    // batch b = 2t + part (part 0: real part of C, 1: imaginary part).  Layout as for real
+    // calls: [A maxima | A residues | B maxima | B residues | reduced products].
     const size_t szA = (size_t)2 * nmod * Mp * Kp, szB = (size_t)2 * nmod * Np * Kp;
-    const size_t szU = (size_t)2 * nmod * Np * Mp, szR = (size_t)(Mp + Np) * 8;
+    const size_t szU = (size_t)2 * nmod * Np * Mp;
+    const size_t oRA = 0, oA = round_up(Mp * 8, 256), oRB = oA + round_up(szA, 256);
+    const size_t oB = oRB + round_up(Np * 8, 256), oU = oB + round_up(szB, 256);
     char *base;
-    if (int rc = oz_workspace(d, szR + szA + szB + szU + 1024, s, &base)) return rc;
-    unsigned long long *rA = reinterpret_cast<unsigned long long *>(base);
-    unsigned long long *rB = rA + Mp;
-    int8_t *resA = reinterpret_cast<int8_t *>(base + round_up(szR, 256));
-    int8_t *resB = resA + round_up(szA, 256);
-    uint8_t *Ub = reinterpret_cast<uint8_t *>(resB + round_up(szB, 256));
+    bool moved = false;
+    if (int rc = oz_workspace(d, oU + szU + 1024, s, &base, &moved)) return rc;
+    unsigned long long *rA = reinterpret_cast<unsigned long long *>(base + oRA);
+    unsigned long long *rB = reinterpret_cast<unsigned long long *>(base + oRB);
+    int8_t *resA = reinterpret_cast<int8_t *>(base + oA);
+    int8_t *resB = reinterpret_cast<int8_t *>(base + oB);
+    uint8_t *Ub = reinterpret_cast<uint8_t *>(base + oU);
     const int64_t bsA = Mp * Kp, bsB = Np * Kp;   // per-batch strides
+    const bool reuse_a = a_tag != 0 && !moved && d->a_tag == a_tag && d->a_m == g.m &&
+                         d->a_k == g.k && d->a_nmod == nmod && d->a_bits == plan.bits &&
+                         d->a_cplx;
+    d->a_tag = 0;   // set again below once this call's op(A) part is enqueued
 
     const bool a_kc = g.opa != OP_N, b_kc = g.opb == OP_N;
     const int sa = g.opa == OP_C ? -1 : 1, sb = g.opb == OP_C ? -1 : 1;
     {
         ScopedTimer tm(KC_OZ_EXPONENT, s);
-        OZ_TRY(fill_zero(rA, Mp + Np, s));
-        OZ_TRY(launch_rowmax(g.A, g.lda, g.m, g.k, a_kc, rA, s));
+        if (!reuse_a) {
+            OZ_TRY(fill_zero(rA, Mp, s));
+            OZ_TRY(launch_rowmax(g.A, g.lda, g.m, g.k, a_kc, rA, s));
+        }
+        OZ_TRY(fill_zero(rB, Np, s));
         OZ_TRY(launch_rowmax(g.B, g.ldb, g.n, g.k, b_kc, rB, s));
     }
     {
         ScopedTimer tm(KC_OZ_SPLIT, s);
-        // A: part 0 row = [Ar | -Ai], part 1 row = [Ar | Ai]
-        OZ_TRY(launch_split(g.A, g.lda, g.m, g.k, Kc, a_kc, rA, plan.bits, 0, mcs, nmod,
-                            resA, 1, resA + bsA, 1, Kp, 2 * bsA, s));
-        OZ_TRY(launch_split(g.A, g.lda, g.m, g.k, Kc, a_kc, rA, plan.bits, 1, mcs, nmod,
-                            resA + Kc, -sa, resA + bsA + Kc, sa, Kp, 2 * bsA, s));
+        if (!reuse_a) {
+            // A: part 0 row = [Ar | -Ai], part 1 row = [Ar | Ai]
+            OZ_TRY(launch_split(g.A, g.lda, g.m, g.k, Kc, a_kc, rA, plan.bits, 0, mcs, nmod,
+                                resA, 1, resA + bsA, 1, Kp, 2 * bsA, s));
+            OZ_TRY(launch_split(g.A, g.lda, g.m, g.k, Kc, a_kc, rA, plan.bits, 1, mcs, nmod,
+                                resA + Kc, -sa, resA + bsA + Kc, sa, Kp, 2 * bsA, s));
+        }
         // B: part 0 column = [Br ; Bi], part 1 column = [Bi ; Br]
         OZ_TRY(launch_split(g.B, g.ldb, g.n, g.k, Kc, b_kc, rB, plan.bits, 0, mcs, nmod,
                             resB, 1, resB + bsB + Kc, 1, Kp, 2 * bsB, s));
         OZ_TRY(launch_split(g.B, g.ldb, g.n, g.k, Kc, b_kc, rB, plan.bits, 1, mcs, nmod,
                             resB + Kc, sb, resB + bsB, sb, Kp, 2 * bsB, s));
     }
+    d->a_tag = a_tag;
+    d->a_m = g.m;
+    d->a_k = g.k;
+    d->a_nmod = nmod;
+    d->a_bits = plan.bits;
+    d->a_cplx = true;
     {
         ScopedTimer tm(KC_OZ_GEMM, s);
         OZ_TRY(launch_gemm(resA, resB, Ub, Mp, Np, Kp, bsA, bsB, Np * Mp, mcs, 2, 2 * nmod, g.variant, s));

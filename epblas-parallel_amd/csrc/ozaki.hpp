@@ -41,10 +41,11 @@ bool ozaki_supported(int64_t k);
 bool ozaki_supported_cplx(int64_t k);
 // a_tag != 0: the caller runs a sequence of calls that share op(A) (the column panels of one
 // host-pointer GEMM) and names it by a_tag.  A call whose tag matches the last call's on this
-// device, with the same m, k and modulus count and an unmoved workspace, reuses op(A)'s
-// maxima and residues instead of recomputing them; any other call recomputes them.
+// device (real or complex alike), with the same m, k, modulus count and integer width and an
+// unmoved workspace, reuses op(A)'s maxima and residues instead of recomputing them; any
+// other call recomputes them.
 int ozaki_gemm_real(const GemmArgsR &g, hipStream_t s, uint64_t a_tag = 0);
-int ozaki_gemm_cplx(const GemmArgsZ &g, hipStream_t s);
+int ozaki_gemm_cplx(const GemmArgsZ &g, hipStream_t s, uint64_t a_tag = 0);
 
 // Test hook: U[j*ldu + i] = (sum_k A[i*kp + k] * B[j*kp + k]) mod m for one
 // modulus of the table, computed by the production INT8 MFMA kernel of the given tile

@@ -5,7 +5,7 @@ its computation, and its columns of C down, on three streams.
 Bars: bitwise equal to the same call through the device-pointer entry point and to the
 single-shot staging (MWGEMM_HOST_PANELS=0).  Cases cover real and complex, op(B) = 'N' and
 transposed, padded leading dimensions, beta = 0 and beta != 0, a ragged last panel, and
-the real exact-product path (whose panels share op(A)'s maxima and residues).  C's padding
+the real and complex exact-product paths (whose panels share op(A)'s maxima and residues).  C's padding
 rows must stay untouched."""
 import os
 
@@ -88,5 +88,13 @@ def test_exact_product_panels_share_a_and_match_one_call(gpu, ta, tb, beta_zero)
         mwblas.wgemm("N", "N", 40, 40, 40, ddgen.cplx_scalar(rng), A, 40, B, 40,
                      ddgen.cplx_scalar(rng), C, 40)
         _run(False, ta, tb, 600, 1100, 1700, 1430 + (ta == "T"), beta_zero, mwblas.PATH_OZAKI)
+    finally:
+        mwblas.set_path(prev)
+
+
+def test_complex_exact_product_panels_match_one_call(gpu):
+    prev = mwblas.set_path(mwblas.PATH_OZAKI)
+    try:
+        _run(True, "C", "N", 400, 700, 4000, 1440, False, mwblas.PATH_OZAKI)
     finally:
         mwblas.set_path(prev)
