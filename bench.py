@@ -78,8 +78,7 @@ def cpu_baseline(A, B, n, k, target_s=20.0):
     return {"value": round(gflops, 3), "unit": "DD-GFLOP/s", "cores": busy, "kind": "port",
             "sample": f"rows 0..{rows} of the {A.shape[1]}x{n}x{k} NN problem (same inputs), "
                       f"{t:.1f} s after a 64-row warm-up, OMP threads={threads} (jc split keeps "
-                      f"{busy} busy), OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND', 'unset')}, "
-                      f"AVX2={bool(L.port_has_avx2())}"}
+                      f"{busy} busy), AVX2={bool(L.port_has_avx2())}"}
 
 
 def batched_bench(dev, reps=5, count=2000, seed=2000):
@@ -616,8 +615,6 @@ def main():
 
 
 if __name__ == "__main__":
-    # The CPU baseline's OpenMP team runs pinned, one thread per core (the reference's
-    # benchmark protocol); OpenMP reads these when the oracle library loads.
-    os.environ.setdefault("OMP_PROC_BIND", "close")
-    os.environ.setdefault("OMP_PLACES", "cores")
+    # No OMP_PROC_BIND here: binding pins the main thread to one core, and the library's
+    # host copy threads, created from it, would inherit that one-core mask.
     main()

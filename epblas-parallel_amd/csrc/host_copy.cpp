@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <condition_variable>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <functional>
@@ -15,13 +16,85 @@
 #include <thread>
 #include <vector>
 
+#if defined(__linux__)
+#include <pthread.h>
+#include <sched.h>
+#include <strings.h>
+#endif
+
 namespace mw {
 namespace {
+
+#if defined(__linux__)
+// Whether an OpenMP runtime in the process was asked to bind its threads.  The binding
+// applies to the process's initial thread too, so a thread that thread creates inherits a
+// mask of one place (one core, say), while the CPUs the process may use are many more.
+bool omp_binding_requested()
+{
+    const char *b = std::getenv("OMP_PROC_BIND");
+    if (b && *b && strcasecmp(b, "false") != 0) return true;
+    return std::getenv("OMP_PLACES") || std::getenv("GOMP_CPU_AFFINITY") ||
+           std::getenv("KMP_AFFINITY");
+}
+
+// Parses a CPU list such as "0-15,32-47" into set; false if it names no CPU.
+bool parse_cpu_list(const char *text, cpu_set_t *set)
+{
+    CPU_ZERO(set);
+    const char *p = text;
+    while (*p) {
+        char *end = nullptr;
+        long a = std::strtol(p, &end, 10);
+        if (end == p) break;
+        long b = a;
+        p = end;
+        if (*p == '-') {
+            b = std::strtol(p + 1, &end, 10);
+            if (end == p + 1) break;
+            p = end;
+        }
+        for (long c = a; c <= b && c >= 0 && c < CPU_SETSIZE; ++c) CPU_SET((int)c, set);
+        if (*p == ',') ++p;
+        else break;
+    }
+    return CPU_COUNT(set) > 0;
+}
+
+// The CPUs of the process's cgroup cpuset (the allotment of a container or a batch job),
+// or every CPU when no cpuset file is readable (the kernel then keeps a thread to its
+// cpuset anyway).
+void allotted_cpus(cpu_set_t *set)
+{
+    for (const char *path : {"/sys/fs/cgroup/cpuset.cpus.effective",
+                             "/sys/fs/cgroup/cpuset/cpuset.effective_cpus"}) {
+        if (FILE *f = std::fopen(path, "r")) {
+            char buf[4096] = {0};
+            const size_t got = std::fread(buf, 1, sizeof buf - 1, f);
+            std::fclose(f);
+            buf[got] = 0;
+            if (parse_cpu_list(buf, set)) return;
+        }
+    }
+    CPU_ZERO(set);
+    for (int c = 0; c < CPU_SETSIZE; ++c) CPU_SET(c, set);
+}
+#endif
 
 class Pool {
 public:
     explicit Pool(int n) : n_(n)
     {
+#if defined(__linux__)
+        // When an OpenMP binding has narrowed the creating thread to fewer CPUs than the pool
+        // has threads, the workers would all share those; they run on the allotted CPUs
+        // instead.  An unbound caller's mask (taskset, a launcher) is inherited as is.
+        cpu_set_t mine;
+        if (omp_binding_requested() &&
+            sched_getaffinity(0, sizeof mine, &mine) == 0 && CPU_COUNT(&mine) < n_) {
+            widen_ = true;
+            allotted_cpus(&wide_);
+        }
+#endif
         for (int i = 1; i < n_; ++i) th_.emplace_back([this, i] { worker(i); });
     }
     ~Pool()
@@ -54,6 +127,9 @@ public:
 private:
     void worker(int i)
     {
+#if defined(__linux__)
+        if (widen_) (void)pthread_setaffinity_np(pthread_self(), sizeof wide_, &wide_);
+#endif
         unsigned seen = 0;
         for (;;) {
             const std::function<void(int)> *f;
@@ -72,6 +148,10 @@ private:
         }
     }
     int n_;
+#if defined(__linux__)
+    bool widen_ = false;
+    cpu_set_t wide_;
+#endif
     std::vector<std::thread> th_;
     std::mutex mu_, job_mu_;
     std::condition_variable cv_, done_cv_;
