@@ -476,8 +476,8 @@ def main():
                          "that scales hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on exactly one GPU (the GEMM shards along N with no "
-              "exchange beyond a broadcast of A; see the sharding note in the design doc). "
+        print("bench.py: the benchmark runs on exactly one GPU; several GPUs would each take a "
+              "column block of B and C, with no exchange beyond a broadcast of A. "
               "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
