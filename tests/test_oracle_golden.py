@@ -60,7 +60,7 @@ def test_golden_detects_a_planted_mistake():
 
 
 def test_ref_library_agrees_when_present():
-    """Where oracle/_ref exists (built from /root/reference), re-run one case live."""
+    """Where oracle/_ref exists (built from the reference sources), re-run one case live."""
     ref = oracle.ref_lib()
     if ref is None:
         pytest.skip("oracle/_ref/libqxref.so not built here")
