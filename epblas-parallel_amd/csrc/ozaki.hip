@@ -269,6 +269,30 @@ void k_oz_split(const T *__restrict__ X, int64_t ld, int64_t R, int64_t K, int64
                                mod_stride, r, kg);
 }
 
+// Complex residue split, three-product form: one unit (row r, 8-deep k group kq) per lane.
+// Row-contiguous input (KC): a wave covers 64 k groups of one row.  Column-contiguous input:
+// a wave covers 4 rows x 16 k groups, so the 8-byte residue words of one row written by a
+// wave form a 128-byte line, and each input load reads four 128-byte segments.
+template <bool KC>
+__global__ __launch_bounds__(256)
+void k_oz_split_k3(const zddv *__restrict__ X, int64_t ld, int64_t R, int64_t K, int64_t kq_n,
+                   const unsigned long long *__restrict__ rmax, int bits, int sim,
+                   const ModConst *__restrict__ mcs, int nmod, int8_t *__restrict__ out,
+                   int64_t kp, int64_t plane)
+{
+    int64_t r, kq;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (KC) {
+        kq = (int64_t)blockIdx.y * 64 + lane;
+        r = (int64_t)blockIdx.x * 4 + w;
+    } else {
+        r = (int64_t)blockIdx.x * 16 + w * 4 + (lane >> 4);
+        kq = (int64_t)blockIdx.y * 16 + (lane & 15);
+    }
+    if (r >= R || kq >= kq_n) return;
+    oz_split_unit_k3<KC>(X, ld, K, rmax, bits, sim, mcs, nmod, out, kp, plane, r, kq);
+}
+
 // Batched residue GEMM kernel: blockIdx.y = batch (modulus, or modulus x part for complex).
 template <int TM, int TN>
 __global__ __launch_bounds__(256, 2)
@@ -490,6 +514,25 @@ hipError_t launch_split_t(const T *X, int64_t ld, int64_t R, int64_t K, int64_t 
     return hipGetLastError();
 }
 
+// Three-product complex residues of one operand (kc: rows contiguous in k) into out.
+hipError_t launch_split_k3(const zddv *X, int64_t ld, int64_t R, int64_t K, int64_t kc_len,
+                           bool kc, const unsigned long long *rmax, int bits, int sim,
+                           const ModConst *mcs, int nmod, int8_t *out, int64_t kp, int64_t plane,
+                           hipStream_t s)
+{
+    const int64_t kq_n = kc_len / 8;
+    if (kc) {
+        dim3 grid((unsigned)((R + 3) / 4), (unsigned)((kq_n + 63) / 64));
+        hipLaunchKernelGGL(k_oz_split_k3<true>, grid, dim3(256), 0, s, X, ld, R, K, kq_n, rmax,
+                           bits, sim, mcs, nmod, out, kp, plane);
+    } else {
+        dim3 grid((unsigned)((R + 15) / 16), (unsigned)((kq_n + 15) / 16));
+        hipLaunchKernelGGL(k_oz_split_k3<false>, grid, dim3(256), 0, s, X, ld, R, K, kq_n, rmax,
+                           bits, sim, mcs, nmod, out, kp, plane);
+    }
+    return hipGetLastError();
+}
+
 // Residues of one operand (or one part of a complex operand) into out1 (signed by s1) and,
 // when out2 is given, also into out2 (signed by s2).
 template <typename T>
@@ -640,13 +683,15 @@ int ozaki_gemm_cplx(const GemmArgsZ &g, hipStream_t s, uint64_t a_tag)
     ModConst *mcs;
     if (int rc = oz_table(d, plan, &mcs)) return rc;
 
+    // Three products per modulus (the Karatsuba form of the complex product, exact modulo m):
+    // planes 3t+q of [rows][Kc] for q = 0 (re), 1 (im), 2 (re + im) of each operand, and
+    // products 3t+q of [Np][Mp].  The modulus count still covers |Re|, |Im| <= 2k 2^(2b);
+    // P2 itself is never reconstructed.  Layout as for real calls: [A maxima | A residues |
+    // B maxima | B residues | reduced products].
     const int64_t Mp = round_up(g.m, 256), Np = round_up(g.n, 256), Kc = round_up(g.k, 64);
-    const int64_t Kp = 2 * Kc;
     const int nmod = plan.nmod;
-    // batch b = 2t + part (part 0: real part of C, 1: imaginary part).  Layout as for real
-    // calls: [A maxima | A residues | B maxima | B residues | reduced products].
-    const size_t szA = (size_t)2 * nmod * Mp * Kp, szB = (size_t)2 * nmod * Np * Kp;
-    const size_t szU = (size_t)2 * nmod * Np * Mp;
+    const size_t szA = (size_t)3 * nmod * Mp * Kc, szB = (size_t)3 * nmod * Np * Kc;
+    const size_t szU = (size_t)3 * nmod * Np * Mp;
     const size_t oRA = 0, oA = round_up(Mp * 8, 256), oRB = oA + round_up(szA, 256);
     const size_t oB = oRB + round_up(Np * 8, 256), oU = oB + round_up(szB, 256);
     char *base;
@@ -657,7 +702,6 @@ int ozaki_gemm_cplx(const GemmArgsZ &g, hipStream_t s, uint64_t a_tag)
     int8_t *resA = reinterpret_cast<int8_t *>(base + oA);
     int8_t *resB = reinterpret_cast<int8_t *>(base + oB);
     uint8_t *Ub = reinterpret_cast<uint8_t *>(base + oU);
-    const int64_t bsA = Mp * Kp, bsB = Np * Kp;   // per-batch strides
     const bool reuse_a = a_tag != 0 && !moved && d->a_tag == a_tag && d->a_m == g.m &&
                          d->a_k == g.k && d->a_nmod == nmod && d->a_bits == plan.bits &&
                          d->a_cplx;
@@ -676,18 +720,11 @@ int ozaki_gemm_cplx(const GemmArgsZ &g, hipStream_t s, uint64_t a_tag)
     }
     {
         ScopedTimer tm(KC_OZ_SPLIT, s);
-        if (!reuse_a) {
-            // A: part 0 row = [Ar | -Ai], part 1 row = [Ar | Ai]
-            OZ_TRY(launch_split(g.A, g.lda, g.m, g.k, Kc, a_kc, rA, plan.bits, 0, mcs, nmod,
-                                resA, 1, resA + bsA, 1, Kp, 2 * bsA, s));
-            OZ_TRY(launch_split(g.A, g.lda, g.m, g.k, Kc, a_kc, rA, plan.bits, 1, mcs, nmod,
-                                resA + Kc, -sa, resA + bsA + Kc, sa, Kp, 2 * bsA, s));
-        }
-        // B: part 0 column = [Br ; Bi], part 1 column = [Bi ; Br]
-        OZ_TRY(launch_split(g.B, g.ldb, g.n, g.k, Kc, b_kc, rB, plan.bits, 0, mcs, nmod,
-                            resB, 1, resB + bsB + Kc, 1, Kp, 2 * bsB, s));
-        OZ_TRY(launch_split(g.B, g.ldb, g.n, g.k, Kc, b_kc, rB, plan.bits, 1, mcs, nmod,
-                            resB + Kc, sb, resB + bsB, sb, Kp, 2 * bsB, s));
+        if (!reuse_a)
+            OZ_TRY(launch_split_k3(g.A, g.lda, g.m, g.k, Kc, a_kc, rA, plan.bits, sa, mcs, nmod,
+                                   resA, Kc, Mp * Kc, s));
+        OZ_TRY(launch_split_k3(g.B, g.ldb, g.n, g.k, Kc, b_kc, rB, plan.bits, sb, mcs, nmod,
+                               resB, Kc, Np * Kc, s));
     }
     d->a_tag = a_tag;
     d->a_m = g.m;
@@ -697,7 +734,7 @@ int ozaki_gemm_cplx(const GemmArgsZ &g, hipStream_t s, uint64_t a_tag)
     d->a_cplx = true;
     {
         ScopedTimer tm(KC_OZ_GEMM, s);
-        OZ_TRY(launch_gemm(resA, resB, Ub, Mp, Np, Kp, bsA, bsB, Np * Mp, mcs, 2, 2 * nmod, g.variant, s));
+        OZ_TRY(launch_gemm(resA, resB, Ub, Mp, Np, Kc, Mp * Kc, Np * Kc, Np * Mp, mcs, 3, 3 * nmod, g.variant, s));
     }
     {
         ScopedTimer tm(KC_OZ_CRT, s);

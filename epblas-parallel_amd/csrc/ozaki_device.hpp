@@ -143,6 +143,59 @@ DDI void oz_split_unit(const T *__restrict__ X, int64_t ld, int64_t K,
     }
 }
 
+// Complex residue split for the three-product (Karatsuba) form: row r of a complex operand,
+// k in [8*kq, 8*kq+8) (8 entries per lane keep both parts' chunks in registers).  For each
+// modulus t it writes three int8 planes at out + (3t+q)*plane + r*kp + k:
+//   q = 0: re,  q = 1: sim*im,  q = 2: re + sim*im
+// (sim = -1 conjugates).  The q = 2 residue is r0 + r1 folded once into the symmetric range
+// (|r0 + r1| <= m - 1 for odd m; for m = 256 any representative's byte is the residue).
+// Products of the planes of two operands give, modulo m, P0 = Re.Re', P1 = Im.Im' and
+// P2 = (Re + Im)(Re' + Im'); the real and imaginary parts of the complex product are P0 - P1
+// and P2 - P0 - P1.  KC: entry (r,k) at X[r*ld + k]; otherwise at X[k*ld + r].
+template <bool KC>
+DDI void oz_split_unit_k3(const zddv *__restrict__ X, int64_t ld, int64_t K,
+                          const unsigned long long *__restrict__ rmax, int bits, int sim,
+                          const ModConst *__restrict__ mcs, int nmod, int8_t *__restrict__ out,
+                          int64_t kp, int64_t plane, int64_t r, int64_t kq)
+{
+    bool bad;
+    const int sigma = row_sigma(rmax[r], bits, bad);
+    double a0[8], a1[8], a2[8], b0[8], b1[8], b2[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const int64_t k = kq * 8 + s;
+        zddv v{{0.0, 0.0}, {0.0, 0.0}};
+        if (k < K && !bad) v = KC ? X[r * ld + k] : X[k * ld + r];
+        int_chunks(v.re.hi, v.re.lo, sigma, a0[s], a1[s], a2[s]);
+        int_chunks(v.im.hi, v.im.lo, sigma, b0[s], b1[s], b2[s]);
+        if (sim < 0) {
+            b0[s] = -b0[s];
+            b1[s] = -b1[s];
+            b2[s] = -b2[s];
+        }
+    }
+    const int64_t off = r * kp + kq * 8;
+    for (int t = 0; t < nmod; ++t) {
+        const ModConst mc = mcs[t];
+        const int mi = (int)mc.m, h = (mi - 1) >> 1;
+        int r0[8], r1[8], r2[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            r0[s] = sym_residue(a0[s], a1[s], a2[s], mc);
+            r1[s] = sym_residue(b0[s], b1[s], b2[s], mc);
+            const int x = r0[s] + r1[s];
+            r2[s] = x > h ? x - mi : (x < -h ? x + mi : x);
+        }
+        int8_t *o = out + (int64_t)(3 * t) * plane + off;
+        *reinterpret_cast<uint2 *>(o) =
+            make_uint2(pack_low_bytes(r0[0], r0[1], r0[2], r0[3]), pack_low_bytes(r0[4], r0[5], r0[6], r0[7]));
+        *reinterpret_cast<uint2 *>(o + plane) =
+            make_uint2(pack_low_bytes(r1[0], r1[1], r1[2], r1[3]), pack_low_bytes(r1[4], r1[5], r1[6], r1[7]));
+        *reinterpret_cast<uint2 *>(o + 2 * plane) =
+            make_uint2(pack_low_bytes(r2[0], r2[1], r2[2], r2[3]), pack_low_bytes(r2[4], r2[5], r2[6], r2[7]));
+    }
+}
+
 // u = v mod m for an exact INT32 dot product v (|v| <= 2^30): shifted by m*2^24 to a
 // non-negative integer below 2^33, the quotient is exact with inv_up.  (The CRT weight
 // y = (M/m)^-1 mod m lives in the reconstruction's fixed-point limbs.)
@@ -591,6 +644,9 @@ DDI void oz_crt_accumulate(const uint8_t *__restrict__ U, int64_t ldu, int64_t s
 {
     static_assert(RPL == 4, "rows per lane");
     constexpr int P = CPLX ? 2 : 1;
+    // Complex: three product planes per modulus (P0, P1, P2 of oz_split_unit_k3), from which
+    // the real part P0 - P1 and the imaginary part P2 - P0 - P1 are formed modulo m.
+    constexpr int NQ = CPLX ? 3 : 1;
 #pragma unroll
     for (int e = 0; e < RPL; ++e)
 #pragma unroll
@@ -598,38 +654,55 @@ DDI void oz_crt_accumulate(const uint8_t *__restrict__ U, int64_t ldu, int64_t s
 #pragma unroll
             for (int l = 0; l < 4; ++l) acc[e][p][l] = 0.0;
     // One modulus: the residue bytes of RPL rows times the four weight limbs.
-    auto step = [&](int t, const uint32_t (&word)[P]) {
+    auto step = [&](int t, const uint32_t (&word)[NQ]) {
         const double w0 = mcs[t].w[0], w1 = mcs[t].w[1], w2 = mcs[t].w[2], w3 = mcs[t].w[3];
+        const int mi = (int)mcs[t].m;
 #pragma unroll
-        for (int p = 0; p < P; ++p)
-#pragma unroll
-            for (int e = 0; e < RPL; ++e) {
-                const double u = (double)((word[p] >> (8 * e)) & 0xffu);
-                acc[e][p][0] = __builtin_fma(u, w0, acc[e][p][0]);
-                acc[e][p][1] = __builtin_fma(u, w1, acc[e][p][1]);
-                acc[e][p][2] = __builtin_fma(u, w2, acc[e][p][2]);
-                acc[e][p][3] = __builtin_fma(u, w3, acc[e][p][3]);
+        for (int e = 0; e < RPL; ++e) {
+            int u[P];
+            if constexpr (CPLX) {
+                const int p0 = (int)((word[0] >> (8 * e)) & 0xffu);
+                const int p1 = (int)((word[1] >> (8 * e)) & 0xffu);
+                const int p2 = (int)((word[2] >> (8 * e)) & 0xffu);
+                int re = p0 - p1;
+                re += re < 0 ? mi : 0;
+                int im = p2 - p0 - p1;
+                im += im < 0 ? mi : 0;
+                im += im < 0 ? mi : 0;
+                u[0] = re;
+                u[P - 1] = im;
+            } else {
+                u[0] = (int)((word[0] >> (8 * e)) & 0xffu);
             }
-    };
-    auto load = [&](int t, uint32_t (&word)[P]) {
 #pragma unroll
-        for (int p = 0; p < P; ++p)
-            word[p] = *reinterpret_cast<const uint32_t *>(U + (int64_t)(t * P + p) * sU + j * ldu + i0);
+            for (int p = 0; p < P; ++p) {
+                const double d = (double)u[p];
+                acc[e][p][0] = __builtin_fma(d, w0, acc[e][p][0]);
+                acc[e][p][1] = __builtin_fma(d, w1, acc[e][p][1]);
+                acc[e][p][2] = __builtin_fma(d, w2, acc[e][p][2]);
+                acc[e][p][3] = __builtin_fma(d, w3, acc[e][p][3]);
+            }
+        }
+    };
+    auto load = [&](int t, uint32_t (&word)[NQ]) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+            word[q] = *reinterpret_cast<const uint32_t *>(U + (int64_t)(t * NQ + q) * sU + j * ldu + i0);
     };
     // The residue words of TB moduli are loaded before any of their FMAs, so each lane
-    // keeps TB*P loads in flight.  The limb sums are exact integers, so the grouping does not
-    // change the result.
+    // keeps TB*NQ loads in flight.  The limb sums are exact integers, so the grouping does
+    // not change the result.
     constexpr int TB = 8;
     int t = 0;
     for (; t + TB <= nmod; t += TB) {
-        uint32_t words[TB][P];
+        uint32_t words[TB][NQ];
 #pragma unroll
         for (int q = 0; q < TB; ++q) load(t + q, words[q]);
 #pragma unroll
         for (int q = 0; q < TB; ++q) step(t + q, words[q]);
     }
     for (; t < nmod; ++t) {
-        uint32_t word[P];
+        uint32_t word[NQ];
         load(t, word);
         step(t, word);
     }
