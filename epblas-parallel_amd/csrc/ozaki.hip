@@ -18,8 +18,9 @@
 // step 1; b = 103 bounds it by 4e-31.  FP64 is used only for the split, the
 // reconstruction and the alpha/beta update; every product runs on the matrix cores.
 //
-// Complex: Re C' = [Ar | -Ai] [Br ; Bi], Im C' = [Ar | Ai] [Bi ; Br] -- two real products
-// of doubled depth per modulus (four real slice products), conjugation folded into signs.
+// Complex: three real products per modulus, P0 = Ar Br^T, P1 = Ai Bi^T and
+// P2 = (Ar+Ai)(Br+Bi)^T (Karatsuba, exact modulo m), from which the reconstruction forms
+// Re C' = P0 - P1 and Im C' = P2 - P0 - P1 modulo m; conjugation is folded into signs.
 #include "ozaki_device.hpp"
 #include "timer.hpp"
 
