@@ -131,3 +131,42 @@ def test_large_device_bitwise(gpu):
         Cp = c["C0"].copy()
         oracle.port_mgemmtr(*_args(c), Cp, c["ldc"])
         assert np.array_equal(dC.cpu().numpy(), Cp, equal_nan=True)
+
+
+def test_rectangle_64x64_tile_matches(gpu):
+    """MGEMMTR_TILE=64 selects the 64x64 rectangle tile instead of the default 32x64.  The
+    setting is read once per process, so the cases run in a child process: 16 fuzz cases
+    plus 700 x 900 in both triangles, bitwise against the restatement."""
+    import os
+    import subprocess
+    import sys
+    import textwrap
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = textwrap.dedent(f"""
+        import sys
+        sys.path[:0] = [{root!r}, {os.path.join(root, 'epblas-parallel_amd')!r},
+                        {os.path.join(root, 'oracle')!r}]
+        import numpy as np
+        import ddgen, mwblas, oracle
+        rng = np.random.default_rng(1500)
+        shapes = [(ddgen.rand_int_log(rng, 1, 256), ddgen.rand_int_log(rng, 1, 256)) for _ in range(16)]
+        shapes += [(700, 900), (700, 900)]
+        for i, (n, k) in enumerate(shapes):
+            uplo = "UL"[i % 2]
+            ta, tb = "NT"[int(rng.integers(0, 2))], "NT"[int(rng.integers(0, 2))]
+            ar, ac = (n, k) if ta == "N" else (k, n)
+            br, bc = (k, n) if tb == "N" else (n, k)
+            A, B = ddgen.real_matrix(rng, ar, ac, ar + 1), ddgen.real_matrix(rng, br, bc, br + 2)
+            C0 = ddgen.real_matrix(rng, n, n, n + 3)
+            al, be = ddgen.real_scalar(rng, scale=2), ddgen.real_scalar(rng, scale=2)
+            args = (uplo, ta, tb, n, k, al, A, ar + 1, B, br + 2, be)
+            Cg, Cp = C0.copy(), C0.copy()
+            mwblas.mgemmtr(*args, Cg, n + 3)
+            oracle.port_mgemmtr(*args, Cp, n + 3)
+            assert np.array_equal(Cg, Cp, equal_nan=True), (i, uplo, ta, tb, n, k)
+        print("rectangle 64x64 ok", flush=True)
+    """)
+    env = dict(os.environ, MGEMMTR_TILE="64")
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                       timeout=240)
+    assert r.returncode == 0 and "rectangle 64x64 ok" in r.stdout, r.stdout + r.stderr
