@@ -98,3 +98,38 @@ def test_complex_exact_product_panels_match_one_call(gpu):
         _run(True, "C", "N", 400, 700, 4000, 1440, False, mwblas.PATH_OZAKI)
     finally:
         mwblas.set_path(prev)
+
+
+def test_copy_threads_under_an_openmp_binding(gpu):
+    """With OMP_PROC_BIND/OMP_PLACES set, an OpenMP runtime binds the process's initial
+    thread to one place, and threads it creates inherit that mask; the host copy pool then
+    runs its threads on the process's allotted CPUs instead.  A child process with the
+    binding set (and the OpenMP oracle loaded first, so the binding is in force before the
+    pool starts) runs a panelled drop-in call, bitwise against the restatement."""
+    import subprocess
+    import sys
+    import textwrap
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = textwrap.dedent(f"""
+        import os, sys
+        sys.path[:0] = [{root!r}, {os.path.join(root, 'epblas-parallel_amd')!r},
+                        {os.path.join(root, 'oracle')!r}]
+        import numpy as np
+        import ddgen, oracle
+        oracle.lib()
+        import mwblas
+        mwblas.set_path(mwblas.PATH_PLAIN)
+        rng = np.random.default_rng(1450)
+        m, n, k = 600, 1100, 1700
+        A, B, C0 = ddgen.real_matrix(rng, m, k), ddgen.real_matrix(rng, k, n), ddgen.real_matrix(rng, m, n)
+        al, be = ddgen.real_scalar(rng, scale=2), ddgen.real_scalar(rng, scale=2)
+        Cg, Cp = C0.copy(), C0.copy()
+        mwblas.mgemm("N", "N", m, n, k, al, A, m, B, k, be, Cg, m)
+        oracle.port_mgemm("N", "N", m, n, k, al, A, m, B, k, be, Cp, m)
+        assert np.array_equal(Cg, Cp)
+        print("bound ok", os.sched_getaffinity(0).__len__(), flush=True)
+    """)
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores")
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                       timeout=240)
+    assert r.returncode == 0 and "bound ok" in r.stdout, r.stdout + r.stderr
