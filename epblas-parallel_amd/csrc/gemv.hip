@@ -7,8 +7,9 @@
 //   order decides the bits).  A workgroup (one wave) stages t_j = alpha*x_j (faithful),
 //   the x_j == 0 skip flags and a per-batch "any skip" flag for 1024 columns at a time in
 //   LDS; A is read column by column with consecutive lanes on consecutive rows.  With m
-//   rows there are only m/64 waves, so op 'N' is bound by the FP64 issue rate of the SIMDs
-//   it occupies rather than by HBM once m is below ~64 rows per SIMD.  Rows in whole
+//   rows there are only m/64 waves, so once m is below ~64 rows per SIMD op 'N' is bound
+//   by the latency of each row's chain (7 dependent FP64 adds per column step through the
+//   sum's high limb), not by HBM.  Rows in whole
 //   4-blocks of their band take the fast operations, the band tails the faithful ones (the
 //   band layout is the reference's OpenMP row split for mgemv, and "whole 4-blocks, last
 //   m % 4 rows" for wgemv).
