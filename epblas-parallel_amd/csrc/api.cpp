@@ -9,12 +9,19 @@
 //   beta pre-pass: beta == 0 -> exact zeros (NaN cleared), beta == 1 -> skip,
 //                  else C*beta                              (:68-77)
 //   alpha == 0 or k == 0: only the pre-pass                 (:78)
-// Deviations, all toward failing loudly instead of silently: invalid trans letters or
-// leading dimensions are rejected (the reference does not check them), and a device
-// failure is reported and aborts (or, with MWGEMM_ON_ERROR=return, leaves C unmodified and
-// is readable through mwgemm_last_error) where the reference's serial path would silently
-// leave C = beta*C (mgemm_serial.cpp:353-365).
+//   trans / uplo letters outside N/T/C and U/L are read as the reference's packers read
+//   them, not rejected (see the ref_* letter functions below)
+// Deviations, all toward failing loudly instead of silently: leading dimensions shorter
+// than the rows they hold are rejected (the reference reads overlapping columns), and a
+// device failure is reported and aborts (or, with MWGEMM_ON_ERROR=return, leaves C
+// unmodified and is readable through mwgemm_last_error) where the reference's serial path
+// would silently leave C = beta*C (mgemm_serial.cpp:353-365).  An exact-product call whose
+// workspace cannot be allocated runs the plain path instead, as the reference's parallel
+// driver recomputes serially when its pack buffers cannot be had (mgemm_parallel.cpp:
+// 180-186).  The device-pointer entry points (no counterpart in the reference) reject
+// letters outside N/T/C and U/L.
 #include <hip/hip_runtime.h>
+#include <sched.h>
 
 #include <algorithm>
 #include <atomic>
@@ -87,6 +94,26 @@ int record_error(int code, const char *fmt, ...)
     } while (0)
 
 // ----------------------------------------------------------------------------------
+// The library's own device buffers.
+// ----------------------------------------------------------------------------------
+namespace {
+std::atomic<int64_t> g_alloc_limit{0};   // bytes; 0: no limit (test hook)
+}
+
+bool device_alloc_allowed(size_t bytes)
+{
+    const int64_t lim = g_alloc_limit.load();
+    return lim <= 0 || bytes <= (size_t)lim;
+}
+
+hipError_t device_alloc(void **p, size_t bytes)
+{
+    *p = nullptr;
+    if (!device_alloc_allowed(bytes)) return hipErrorOutOfMemory;
+    return hipMalloc(p, bytes);
+}
+
+// ----------------------------------------------------------------------------------
 // Per-device state: one stream and a grow-only staging/workspace arena.
 // ----------------------------------------------------------------------------------
 namespace {
@@ -141,16 +168,23 @@ int ensure_stream(DevState *st)
 // whose shapes grow call by call (a blocked solver's trailing updates) reallocates rarely.
 int ensure_stage(DevState *st, size_t bytes)
 {
-    if (st->stage_bytes >= bytes) return 0;
+    if (st->stage_bytes >= bytes && device_alloc_allowed(bytes)) return 0;
+    const size_t need = bytes;
     bytes = std::max(bytes, st->stage_bytes + st->stage_bytes / 2);
     bytes = (bytes + (size_t(2) << 20) - 1) & ~((size_t(2) << 20) - 1);
+    if (!device_alloc_allowed(bytes)) bytes = need;   // the growth margin is optional
     if (st->stage) {
         MW_TRY(hipStreamSynchronize(st->stream));
         MW_TRY(hipFree(st->stage));
         st->stage = nullptr;
         st->stage_bytes = 0;
     }
-    MW_TRY(hipMalloc(&st->stage, bytes));
+    if (device_alloc(&st->stage, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        st->stage = nullptr;
+        return record_error((int)hipErrorOutOfMemory,
+                            "host staging: cannot allocate %zu bytes of device memory", bytes);
+    }
     st->stage_bytes = bytes;
     return 0;
 }
@@ -337,7 +371,7 @@ int plain_order()
 }
 
 namespace {
-// Letters: returns OP_N/OP_T/OP_C or -1.
+// Letters of the library's own device entry points: OP_N/OP_T/OP_C, or -1 for any other.
 int op_real(char c)
 {
     c = (char)toupper((unsigned char)c);
@@ -352,6 +386,48 @@ int op_cplx(char c)
     if (c == 'T') return OP_T;
     if (c == 'C') return OP_C;
     return -1;
+}
+
+// Letters of the drop-in entry points, read exactly as the reference reads them (it never
+// rejects a letter; each packer tests for the letters it knows and treats the rest alike).
+//   real GEMM, gemmtr, gemv: anything but 'N' is transposed (blas_char.h:36 folds 'C';
+//     the packers test == 'N', mgemm_serial.cpp:52, 71; mgemv.cpp:248-250);
+//   complex op(A): 'N', 'T', anything else conjugate-transposed (wgemm_pack_A's final
+//     branch, wgemm_serial.cpp:56-73);
+//   complex op(B): 'N', 'T', 'C'; any other letter is transposed without conjugation by the
+//     AVX2 packer (it conjugates only for 'C', wgemm_serial.cpp:145, 174-185) and
+//     conjugate-transposed by the scalar packer (:81-99), so it follows the plain order;
+//   complex gemv: 'N', 'C', anything else transposed (wgemv.cpp:308-319);
+//   gemmtr uplo: 'U' is the upper triangle, anything else the lower (mgemmtr_serial.cpp:178).
+int ref_op_real(char c) { return toupper((unsigned char)c) == 'N' ? OP_N : OP_T; }
+int ref_op_cplx_a(char c)
+{
+    c = (char)toupper((unsigned char)c);
+    return c == 'N' ? OP_N : c == 'T' ? OP_T : OP_C;
+}
+int ref_op_cplx_b(char c, int order)
+{
+    c = (char)toupper((unsigned char)c);
+    if (c == 'N') return OP_N;
+    if (c == 'T') return OP_T;
+    if (c == 'C') return OP_C;
+    return order == ORDER_SCALAR ? OP_C : OP_T;
+}
+int ref_op_gemv_cplx(char c)
+{
+    c = (char)toupper((unsigned char)c);
+    return c == 'N' ? OP_N : c == 'C' ? OP_C : OP_T;
+}
+int ref_upper(char c) { return toupper((unsigned char)c) == 'U' ? 1 : 0; }
+// The complex gemmtr's diagonal triangles read a letter outside N/T/C as 'T' (transposed
+// unless 'N', conjugated only for 'C', wgemmtr_serial.cpp:54-57), while its rectangle, which
+// goes through the GEMM's packers, may read it as 'C': bit 0 / bit 1 mark op(A) / op(B)
+// taken as 'T' on the diagonal where the rectangle conjugates.
+int ref_gemmtr_diag_alt(char ta, char tb, int opa, int opb)
+{
+    ta = (char)toupper((unsigned char)ta);
+    tb = (char)toupper((unsigned char)tb);
+    return (opa == OP_C && ta != 'C' ? 1 : 0) | (opb == OP_C && tb != 'C' ? 2 : 0);
 }
 
 bool dd_is0(const ddv &x) { return x.hi == 0.0 && x.lo == 0.0; }
@@ -373,15 +449,14 @@ bool is_device_ptr(const void *p)
 }
 
 // AUTO takes the exact-product path once the product is large enough to amortise the
-// residue split; OZAKI falls back to PLAIN (still on the GPU) beyond the depth the INT32
-// accumulation keeps exact.
+// residue split and its depth is one the plans cover.  An explicit OZAKI request beyond
+// that depth fails loudly (ozaki_gemm_* report it) instead of silently running PLAIN.
 int choose_path(int path, int64_t m, int64_t n, int64_t k, bool cplx)
 {
     const bool ok = cplx ? ozaki_supported_cplx(k) : ozaki_supported(k);
     if (path == MWGEMM_PATH_AUTO)
         return ok && (double)m * (double)n * (double)k * (cplx ? 4.0 : 1.0) >= 1.0e9
                    ? MWGEMM_PATH_OZAKI : MWGEMM_PATH_PLAIN;
-    if (path == MWGEMM_PATH_OZAKI && !ok) return MWGEMM_PATH_PLAIN;
     return path;
 }
 }  // namespace
@@ -405,7 +480,13 @@ int gemm_real_device(GemmArgsR g, int path, hipStream_t s, uint64_t a_tag)
         MW_TRY(plain_beta_only_real(g, s));
         return 0;
     }
-    if (choose_path(path, g.m, g.n, g.k, false) == MWGEMM_PATH_OZAKI) return ozaki_gemm_real(g, s, a_tag);
+    if (choose_path(path, g.m, g.n, g.k, false) == MWGEMM_PATH_OZAKI) {
+        const int rc = ozaki_gemm_real(g, s, a_tag);
+        // Without its workspace the exact product runs on the plain path, which needs none
+        // (the reference's driver likewise recomputes serially when its pack buffers cannot
+        // be had, mgemm_parallel.cpp:180-186).
+        if (rc != OZ_NO_WORKSPACE) return rc;
+    }
     MW_TRY(plain_gemm_real(g, s));
     return 0;
 }
@@ -437,6 +518,7 @@ int gemmtr_tile()
     return v;
 }
 
+// Uplo of the device entry points: 1 upper, 0 lower, -1 for a letter other than U/L.
 int uplo_upper(char uplo)
 {
     const char u = (char)toupper((unsigned char)uplo);
@@ -444,11 +526,10 @@ int uplo_upper(char uplo)
 }
 
 // Triangular-output GEMM on device memory (g.m == g.n); the plain path always (its result
-// is bit-for-bit the reference's).
-int gemmtr_real_device(GemmArgsR g, char uplo, hipStream_t s)
+// is bit-for-bit the reference's).  upper: 1 / 0, or -1 for an invalid uplo letter.
+int gemmtr_real_device(GemmArgsR g, int upper, hipStream_t s)
 {
     g.order = plain_order();
-    const int upper = uplo_upper(uplo);
     if (upper < 0) return record_error(-2, "mgemmtr: invalid uplo letter");
     if (g.opa < 0 || g.opb < 0) return record_error(-2, "mgemmtr: invalid trans letter");
     if (g.n <= 0) return 0;
@@ -464,10 +545,9 @@ int gemmtr_real_device(GemmArgsR g, char uplo, hipStream_t s)
     return 0;
 }
 
-int gemmtr_cplx_device(GemmArgsZ g, char uplo, hipStream_t s)
+int gemmtr_cplx_device(GemmArgsZ g, int upper, hipStream_t s)
 {
     g.order = plain_order();
-    const int upper = uplo_upper(uplo);
     if (upper < 0) return record_error(-2, "wgemmtr: invalid uplo letter");
     if (g.opa < 0 || g.opb < 0) return record_error(-2, "wgemmtr: invalid trans letter");
     if (g.n <= 0) return 0;
@@ -498,7 +578,10 @@ int gemm_cplx_device(GemmArgsZ g, int path, hipStream_t s, uint64_t a_tag)
         MW_TRY(plain_beta_only_cplx(g, s));
         return 0;
     }
-    if (choose_path(path, g.m, g.n, g.k, true) == MWGEMM_PATH_OZAKI) return ozaki_gemm_cplx(g, s, a_tag);
+    if (choose_path(path, g.m, g.n, g.k, true) == MWGEMM_PATH_OZAKI) {
+        const int rc = ozaki_gemm_cplx(g, s, a_tag);
+        if (rc != OZ_NO_WORKSPACE) return rc;   // as for the real GEMM
+    }
     MW_TRY(plain_gemm_cplx(g, s));
     return 0;
 }
@@ -726,13 +809,12 @@ int mgemm_host(char transa, char transb, int64_t m, int64_t n, int64_t k,
     if (m <= 0 || n <= 0) return 0;
     GemmArgsR g{};
     g.m = m; g.n = n; g.k = k;
-    g.opa = op_real(transa); g.opb = op_real(transb);
+    g.opa = ref_op_real(transa); g.opb = ref_op_real(transb);
     g.alpha = ddv{alpha->limbs[0], alpha->limbs[1]};
     g.beta = ddv{beta->limbs[0], beta->limbs[1]};
     g.A = reinterpret_cast<const ddv *>(a); g.lda = lda;
     g.B = reinterpret_cast<const ddv *>(b); g.ldb = ldb;
     g.C = reinterpret_cast<ddv *>(c); g.ldc = ldc;
-    if (g.opa < 0 || g.opb < 0) return record_error(-2, "mgemm: invalid trans letter");
     const bool need_ab = k > 0 && !dd_is0(g.alpha);
     const int64_t ar = need_ab ? (g.opa == OP_N ? m : k) : 0, ac = g.opa == OP_N ? k : m;
     const int64_t br = need_ab ? (g.opb == OP_N ? k : n) : 0, bc = g.opb == OP_N ? n : k;
@@ -760,13 +842,12 @@ int wgemm_host(char transa, char transb, int64_t m, int64_t n, int64_t k,
     if (m <= 0 || n <= 0) return 0;
     GemmArgsZ g{};
     g.m = m; g.n = n; g.k = k;
-    g.opa = op_cplx(transa); g.opb = op_cplx(transb);
+    g.opa = ref_op_cplx_a(transa); g.opb = ref_op_cplx_b(transb, plain_order());
     g.alpha = zddv{{alpha->re.limbs[0], alpha->re.limbs[1]}, {alpha->im.limbs[0], alpha->im.limbs[1]}};
     g.beta = zddv{{beta->re.limbs[0], beta->re.limbs[1]}, {beta->im.limbs[0], beta->im.limbs[1]}};
     g.A = reinterpret_cast<const zddv *>(a); g.lda = lda;
     g.B = reinterpret_cast<const zddv *>(b); g.ldb = ldb;
     g.C = reinterpret_cast<zddv *>(c); g.ldc = ldc;
-    if (g.opa < 0 || g.opb < 0) return record_error(-2, "wgemm: invalid trans letter");
     const bool need_ab = k > 0 && !zdd_is0(g.alpha);
     const int64_t ar = need_ab ? (g.opa == OP_N ? m : k) : 0, ac = g.opa == OP_N ? k : m;
     const int64_t br = need_ab ? (g.opb == OP_N ? k : n) : 0, bc = g.opb == OP_N ? n : k;
@@ -784,14 +865,17 @@ int wgemm_host(char transa, char transb, int64_t m, int64_t n, int64_t k,
                              [path](GemmArgsZ gg, hipStream_t s) { return gemm_cplx_device(gg, path, s); });
 }
 
+// Arguments of a triangular-output GEMM; ref: read the letters as the reference does (the
+// drop-in entry points), else strictly (the device entry points).
 GemmArgsR real_args(char transa, char transb, int64_t m, int64_t n, int64_t k,
                     const mw_float64x2 *alpha, const mw_float64x2 *a, int64_t lda,
                     const mw_float64x2 *b, int64_t ldb, const mw_float64x2 *beta,
-                    mw_float64x2 *c, int64_t ldc)
+                    mw_float64x2 *c, int64_t ldc, bool ref)
 {
     GemmArgsR g{};
     g.m = m; g.n = n; g.k = k;
-    g.opa = op_real(transa); g.opb = op_real(transb);
+    g.opa = ref ? ref_op_real(transa) : op_real(transa);
+    g.opb = ref ? ref_op_real(transb) : op_real(transb);
     g.alpha = ddv{alpha->limbs[0], alpha->limbs[1]};
     g.beta = ddv{beta->limbs[0], beta->limbs[1]};
     g.A = reinterpret_cast<const ddv *>(a); g.lda = lda;
@@ -803,11 +887,18 @@ GemmArgsR real_args(char transa, char transb, int64_t m, int64_t n, int64_t k,
 GemmArgsZ cplx_args(char transa, char transb, int64_t m, int64_t n, int64_t k,
                     const mw_complex64x2 *alpha, const mw_complex64x2 *a, int64_t lda,
                     const mw_complex64x2 *b, int64_t ldb, const mw_complex64x2 *beta,
-                    mw_complex64x2 *c, int64_t ldc)
+                    mw_complex64x2 *c, int64_t ldc, bool ref)
 {
     GemmArgsZ g{};
     g.m = m; g.n = n; g.k = k;
-    g.opa = op_cplx(transa); g.opb = op_cplx(transb);
+    if (ref) {
+        g.opa = ref_op_cplx_a(transa);
+        g.opb = ref_op_cplx_b(transb, plain_order());
+        g.diag_alt = ref_gemmtr_diag_alt(transa, transb, g.opa, g.opb);
+    } else {
+        g.opa = op_cplx(transa);
+        g.opb = op_cplx(transb);
+    }
     g.alpha = zddv{{alpha->re.limbs[0], alpha->re.limbs[1]}, {alpha->im.limbs[0], alpha->im.limbs[1]}};
     g.beta = zddv{{beta->re.limbs[0], beta->re.limbs[1]}, {beta->im.limbs[0], beta->im.limbs[1]}};
     g.A = reinterpret_cast<const zddv *>(a); g.lda = lda;
@@ -824,9 +915,8 @@ int mgemmtr_host(char uplo, char transa, char transb, int64_t n, int64_t k,
                  int64_t ldc)
 {
     if (n <= 0) return 0;
-    GemmArgsR g = real_args(transa, transb, n, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
-    if (uplo_upper(uplo) < 0) return record_error(-2, "mgemmtr: invalid uplo letter");
-    if (g.opa < 0 || g.opb < 0) return record_error(-2, "mgemmtr: invalid trans letter");
+    GemmArgsR g = real_args(transa, transb, n, n, k, alpha, a, lda, b, ldb, beta, c, ldc, true);
+    const int upper = ref_upper(uplo);
     const bool need_ab = k > 0 && !dd_is0(g.alpha);
     const int64_t ar = need_ab ? (g.opa == OP_N ? n : k) : 0, ac = g.opa == OP_N ? k : n;
     const int64_t br = need_ab ? (g.opb == OP_N ? k : n) : 0, bc = g.opb == OP_N ? n : k;
@@ -834,7 +924,7 @@ int mgemmtr_host(char uplo, char transa, char transb, int64_t n, int64_t k,
         return record_error(-3, "mgemmtr: leading dimension too small (lda=%lld ldb=%lld ldc=%lld)",
                             (long long)lda, (long long)ldb, (long long)ldc);
     return staged_call<ddv>(g, ar, ac, br, bc, false,
-                            [uplo](GemmArgsR gg, hipStream_t s) { return gemmtr_real_device(gg, uplo, s); });
+                            [upper](GemmArgsR gg, hipStream_t s) { return gemmtr_real_device(gg, upper, s); });
 }
 
 int wgemmtr_host(char uplo, char transa, char transb, int64_t n, int64_t k,
@@ -843,9 +933,8 @@ int wgemmtr_host(char uplo, char transa, char transb, int64_t n, int64_t k,
                  mw_complex64x2 *c, int64_t ldc)
 {
     if (n <= 0) return 0;
-    GemmArgsZ g = cplx_args(transa, transb, n, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
-    if (uplo_upper(uplo) < 0) return record_error(-2, "wgemmtr: invalid uplo letter");
-    if (g.opa < 0 || g.opb < 0) return record_error(-2, "wgemmtr: invalid trans letter");
+    GemmArgsZ g = cplx_args(transa, transb, n, n, k, alpha, a, lda, b, ldb, beta, c, ldc, true);
+    const int upper = ref_upper(uplo);
     const bool need_ab = k > 0 && !zdd_is0(g.alpha);
     const int64_t ar = need_ab ? (g.opa == OP_N ? n : k) : 0, ac = g.opa == OP_N ? k : n;
     const int64_t br = need_ab ? (g.opb == OP_N ? k : n) : 0, bc = g.opb == OP_N ? n : k;
@@ -853,7 +942,7 @@ int wgemmtr_host(char uplo, char transa, char transb, int64_t n, int64_t k,
         return record_error(-3, "wgemmtr: leading dimension too small (lda=%lld ldb=%lld ldc=%lld)",
                             (long long)lda, (long long)ldb, (long long)ldc);
     return staged_call<zddv>(g, ar, ac, br, bc, false,
-                             [uplo](GemmArgsZ gg, hipStream_t s) { return gemmtr_cplx_device(gg, uplo, s); });
+                             [upper](GemmArgsZ gg, hipStream_t s) { return gemmtr_cplx_device(gg, upper, s); });
 }
 
 // ----------------------------------------------------------------------------------
@@ -862,19 +951,37 @@ int wgemmtr_host(char uplo, char transa, char transb, int64_t n, int64_t k,
 // Row bands of the real op-'N' product: the reference splits the rows over its OpenMP team
 // when m >= 64 (mgemv.cpp:95-100, at most 256 threads), and the rows after the last whole
 // 4-block of each band take the faithful operations, so the team size is part of the bit
-// pattern.  MWGEMV_THREADS names it; else OMP_NUM_THREADS (its first entry); else the
-// host's hardware thread count, which is what the reference's OpenMP default would use.
+// pattern.  MWGEMV_THREADS names it; else the team an OpenMP runtime forms by default
+// (omp_get_max_threads): the first entry of OMP_NUM_THREADS, else the number of CPUs in the
+// process's affinity mask (what taskset and cgroup cpusets restrict, not the CPUs online),
+// either capped by OMP_THREAD_LIMIT.
+int64_t omp_default_team()
+{
+    auto env_int = [](const char *name) -> long {
+        const char *e = getenv(name);
+        return e ? atol(e) : 0;   // atol reads the first entry of a list such as "8,4"
+    };
+    long team = env_int("OMP_NUM_THREADS");
+    if (team <= 0) {
+        cpu_set_t set;
+        CPU_ZERO(&set);
+        if (sched_getaffinity(0, sizeof set, &set) == 0) team = CPU_COUNT(&set);
+        if (team <= 0) {
+            const unsigned hw = std::thread::hardware_concurrency();
+            team = hw ? (long)hw : 1;
+        }
+    }
+    const long limit = env_int("OMP_THREAD_LIMIT");
+    if (limit > 0 && team > limit) team = limit;
+    return (int64_t)team;
+}
+
 int64_t gemv_bands(int64_t m)
 {
     static const int64_t team = [] {
-        for (const char *name : {"MWGEMV_THREADS", "OMP_NUM_THREADS"}) {
-            if (const char *e = getenv(name)) {
-                const long v = atol(e);
-                if (v > 0) return (int64_t)v;
-            }
-        }
-        const unsigned hw = std::thread::hardware_concurrency();
-        return (int64_t)(hw ? hw : 1);
+        const char *e = getenv("MWGEMV_THREADS");
+        const long v = e ? atol(e) : 0;
+        return v > 0 ? (int64_t)v : omp_default_team();
     }();
     if (m < 64) return 1;
     return team < 256 ? team : 256;
@@ -955,7 +1062,7 @@ int mgemv_host(char trans, int64_t m, int64_t n, const mw_float64x2 *alpha,
                const mw_float64x2 *a, int64_t lda, const mw_float64x2 *x, int64_t incx,
                const mw_float64x2 *beta, mw_float64x2 *y, int64_t incy)
 {
-    const int op = op_real(trans);
+    const int op = ref_op_real(trans);
     if (int rc = gemv_check("mgemv", op, m, n, lda, incx, incy)) return rc;
     if (m == 0 || n == 0) return 0;
     const bool notrans = op == OP_N;
@@ -974,7 +1081,7 @@ int wgemv_host(char trans, int64_t m, int64_t n, const mw_complex64x2 *alpha,
                const mw_complex64x2 *a, int64_t lda, const mw_complex64x2 *x, int64_t incx,
                const mw_complex64x2 *beta, mw_complex64x2 *y, int64_t incy)
 {
-    const int op = op_cplx(trans);
+    const int op = ref_op_gemv_cplx(trans);
     if (int rc = gemv_check("wgemv", op, m, n, lda, incx, incy)) return rc;
     if (m == 0 || n == 0) return 0;
     const bool notrans = op == OP_N;
@@ -1163,7 +1270,7 @@ int mwgemm_mgemmtr_device(char uplo, char transa, char transb, int64_t n, int64_
 {
     hipStream_t s = static_cast<hipStream_t>(stream);   // NULL: HIP's default stream
     return gemmtr_real_device(real_args(transa, transb, n, n, k, alpha, dA, lda, dB, ldb, beta,
-                                        dC, ldc), uplo, s);
+                                        dC, ldc, false), uplo_upper(uplo), s);
 }
 
 int mwgemm_wgemmtr_device(char uplo, char transa, char transb, int64_t n, int64_t k,
@@ -1173,7 +1280,7 @@ int mwgemm_wgemmtr_device(char uplo, char transa, char transb, int64_t n, int64_
 {
     hipStream_t s = static_cast<hipStream_t>(stream);   // NULL: HIP's default stream
     return gemmtr_cplx_device(cplx_args(transa, transb, n, n, k, alpha, dA, lda, dB, ldb, beta,
-                                        dC, ldc), uplo, s);
+                                        dC, ldc, false), uplo_upper(uplo), s);
 }
 
 void mgemv_(const char *trans, const int *m, const int *n, const mw_float64x2 *alpha,
@@ -1340,6 +1447,13 @@ int mwgemm_ozaki_moduli(int nmod, int *m, int *y)
         if (m) m[t] = (int)p.mc[t].m;
         if (y) y[t] = (int)p.mc[t].y;
     }
+    return 0;
+}
+
+int mwgemm_testing_set_alloc_limit(int64_t bytes)
+{
+    if (bytes < 0) return -1;
+    g_alloc_limit.store(bytes);
     return 0;
 }
 

@@ -474,6 +474,7 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
     // reconstruction) to pay off -- at least 1e9 multiply-adds over problems of mean size
     // 96 or more (the 2000-problem batch with sizes in [16,512] runs 1.2x faster there).
     const int64_t vol = count > 0 ? (int64_t)std::cbrt(flops / 2.0 / (double)count) : 0;
+    const bool auto_path = path == MWGEMM_PATH_AUTO;
     if (path == MWGEMM_PATH_AUTO) {
         const int tv = tune_lookup(TK_BATCH, count, vol, kmax);
         if (tv >= 0) {
@@ -483,7 +484,11 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
             path = flops / 2.0 >= 1.0e9 && vol >= 96 ? MWGEMM_PATH_OZAKI : MWGEMM_PATH_PLAIN;
         }
     }
-    if (path == MWGEMM_PATH_OZAKI && kmax > 0 && !ozaki_supported(kmax)) path = MWGEMM_PATH_PLAIN;
+    if (path == MWGEMM_PATH_OZAKI && kmax > 0 && !ozaki_supported(kmax)) {
+        if (!auto_path)
+            return record_error(-5, "batched ozaki: contraction depth %lld out of range", (long long)kmax);
+        path = MWGEMM_PATH_PLAIN;
+    }
 
     BatchPlan *P = new BatchPlan();
     P->path = path;
@@ -605,10 +610,17 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
     const size_t o_ra = take(pre), o_rb = take(pre), o_sa = take(pre), o_sb = take(pre), o_cr = take(pre);
     const size_t o_rmax = take(sizeof(unsigned long long) * std::max<int64_t>(rmax_n, 1));
     const size_t o_ws = take(std::max<size_t>(ws, 1));
-    if (hipMalloc(&P->dmem, off) != hipSuccess) {
+    if (device_alloc(&P->dmem, off) != hipSuccess) {
         (void)hipGetLastError();
+        P->dmem = nullptr;
         size_t bytes = off;
         plan_free(P);
+        // Without its workspace the exact product runs on the plain path, whose plan holds
+        // only the problem and tile tables (the reference's driver likewise recomputes
+        // when its pack buffers cannot be had, mgemm_parallel.cpp:180-186).
+        if (oz)
+            return plan_build(out, count, transa, transb, m, n, k, alpha, dA, lda, dB, ldb, beta,
+                              dC, ldc, MWGEMM_PATH_PLAIN);
         return record_error(-6, "batched mgemm: cannot allocate %zu bytes of plan memory", bytes);
     }
     char *base = static_cast<char *>(P->dmem);
@@ -644,7 +656,7 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
         rc = record_error(-1, "batched mgemm: plan upload failed");
     }
     if (!rc && oz && nmod > 0) {
-        if (hipMalloc(&P->mcs, sizeof(ModConst) * OZ_MAX_MOD) != hipSuccess ||
+        if (device_alloc(reinterpret_cast<void **>(&P->mcs), sizeof(ModConst) * OZ_MAX_MOD) != hipSuccess ||
             hipMemcpy(P->mcs, P->oz.mc, sizeof(ModConst) * nmod, hipMemcpyHostToDevice) != hipSuccess)
         {
             (void)hipGetLastError();
@@ -742,7 +754,9 @@ int mwgemm_batched_plan_run(mwgemm_batched_plan plan, void *stream)
     hipStream_t s = static_cast<hipStream_t>(stream);
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     const bool capturing = hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
-    if (!capturing && P->done_recorded && P->done_stream != s) {
+    // Always wait when an earlier run was recorded: a stream handle may be reused for another
+    // stream, so an equal handle proves nothing (the wait is free on the same stream).
+    if (!capturing && P->done_recorded) {
         hipError_t w = hipStreamWaitEvent(s, P->done, 0);
         if (w != hipSuccess) return record_error((int)w, "hipStreamWaitEvent: %s", hipGetErrorString(w));
     }

@@ -32,12 +32,22 @@ struct GemmArgsZ {
     zddv *C; int64_t ldc;
     int variant = -1;
     int order = 0;
+    // Triangular-output GEMM only: bit 0 / bit 1 = the diagonal triangles take op(A) /
+    // op(B) as OP_T where the rectangle has OP_C (a letter outside N/T/C, which the
+    // reference's diagonal step and GEMM packers read differently).
+    int diag_alt = 0;
 };
 
 // Operation orders of the plain path: the reference's AVX2 kernel, or its scalar kernel
 // (hosts without AVX2/FMA).  Chosen per call from the library setting (api.cpp).
 enum PlainOrder { ORDER_SIMD = 0, ORDER_SCALAR = 1 };
 int plain_order();   // the current setting
+
+// Device buffers the library allocates for itself (exact-product workspace, host staging,
+// batched plans) go through these (api.cpp).  mwgemm_testing_set_alloc_limit makes every
+// such buffer above a size unobtainable, as on a device without that much free memory.
+hipError_t device_alloc(void **p, size_t bytes);
+bool device_alloc_allowed(size_t bytes);
 
 // Device-pointer GEMM cores (api.cpp).  a_tag: see ozaki_gemm_real.
 int gemm_real_device(GemmArgsR g, int path, hipStream_t s, uint64_t a_tag = 0);

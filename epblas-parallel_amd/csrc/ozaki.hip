@@ -295,7 +295,8 @@ void k_oz_split_k3(const zddv *__restrict__ X, int64_t ld, int64_t R, int64_t K,
 }
 
 // Batched residue GEMM kernel: blockIdx.y = batch (modulus, or modulus x part for complex).
-template <int TM, int TN>
+// FOLD: depth beyond OZ_FOLD_DEPTH (the accumulators are reduced on the way).
+template <int TM, int TN, bool FOLD>
 __global__ __launch_bounds__(256, 2)
 void k_oz_gemm(const int8_t *__restrict__ A, const int8_t *__restrict__ B,
                uint8_t *__restrict__ U, int64_t kp, int64_t ldu, int64_t sA, int64_t sB,
@@ -309,11 +310,13 @@ void k_oz_gemm(const int8_t *__restrict__ A, const int8_t *__restrict__ B,
     tile_order(blockIdx.x, tiles_m, (int)gridDim.x / tiles_m, tm, tn);
     const int64_t i0 = (int64_t)tm * BM;
     const int64_t j0 = (int64_t)tn * BN;
-    oz_gemm_tile<TM, TN, BK>(A + batch * sA + i0 * kp, B + batch * sB + j0 * kp, U + batch * sU,
-                             kp, ldu, mcs + batch / per, 1, 0, 0, 0, i0, j0, sa, sb);
+    oz_gemm_tile<TM, TN, BK, FOLD>(A + batch * sA + i0 * kp, B + batch * sB + j0 * kp,
+                                   U + batch * sU, kp, ldu, mcs + batch / per, 1, 0, 0, 0, i0, j0,
+                                   sa, sb);
 }
 
 // 256 x 256 residue tiles through the DMA-fed LDS ring (eight waves, one workgroup per CU).
+template <bool FOLD>
 __global__ __launch_bounds__(RING_THREADS, 1)
 void k_oz_gemm_ring(const int8_t *__restrict__ A, const int8_t *__restrict__ B,
                     uint8_t *__restrict__ U, int64_t kp, int64_t ldu, int64_t sA, int64_t sB,
@@ -325,9 +328,10 @@ void k_oz_gemm_ring(const int8_t *__restrict__ A, const int8_t *__restrict__ B,
     tile_order(blockIdx.x, tiles_m, (int)gridDim.x / tiles_m, tm, tn);
     const int64_t i0 = (int64_t)tm * 256;
     const int64_t j0 = (int64_t)tn * 256;
-    oz_gemm_tile_ring<4, 2, 2, 4, false>(A + batch * sA + i0 * kp, B + batch * sB + j0 * kp,
-                                         U + batch * sU, kp, ldu, mcs + batch / per, 1, 0, 0, 0,
-                                         i0, j0, smem);
+    oz_gemm_tile_ring<4, 2, 2, 4, false, false, FOLD>(A + batch * sA + i0 * kp,
+                                                      B + batch * sB + j0 * kp, U + batch * sU,
+                                                      kp, ldu, mcs + batch / per, 1, 0, 0, 0, i0,
+                                                      j0, smem);
 }
 
 // Reconstruction kernel: a workgroup owns 1024 consecutive rows of one column.  Each lane
@@ -390,11 +394,16 @@ namespace {
 struct OzDevice {
     void *ws = nullptr;
     size_t ws_bytes = 0;
-    // Completion of the last call that used ws, and its stream: a call on another stream
-    // waits for it (on the device) before its kernels overwrite the workspace.
+    // Completion of the last eager call that used ws: a later eager call waits for it (on
+    // the device) before its kernels overwrite the workspace.
     hipEvent_t ws_done = nullptr;
-    hipStream_t ws_stream = nullptr;
     bool ws_recorded = false;
+    // A call captured into a graph bakes ws's address into the graph, which may be replayed
+    // at any time on any stream.  Such a buffer is never freed or reused by eager calls:
+    // the next eager call moves to a buffer of its own and the captured one is kept here
+    // until the process ends.
+    bool ws_captured = false;
+    std::vector<void *> retired;
     std::map<int, ModConst *> tables;   // keyed by modulus count
     // The op(A) part of the workspace (maxima, then residues, at its start) as the last real
     // call left it: its tag (0: no reuse), shape and modulus count.
@@ -423,22 +432,37 @@ bool stream_capturing(hipStream_t s)
     return hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
 }
 
-// The per-device workspace for a call on stream s, ordered after the previous call that
-// used it: on the same stream by stream order, on another stream by waiting (on the
-// device) for that call's completion event.  Under stream capture the workspace must
-// already be large enough (capture cannot free or synchronise), and captured work is
-// ordered only by how the graph is launched.
+// The per-device workspace for a call on stream s.  Eager calls are ordered after the
+// previous eager call that used it: on the same stream by stream order, on another by
+// waiting (on the device) for its completion event, which is always done when one was
+// recorded (a stream handle may be reused for another stream, so equal handles prove
+// nothing).  Under stream capture the workspace must already be large enough (capture can
+// neither allocate nor synchronise); the captured buffer then belongs to the graph (see
+// OzDevice), and replays of captured calls are ordered only by how the graphs are
+// launched.  Returns OZ_NO_WORKSPACE, with nothing enqueued, when the buffer cannot be
+// allocated.
 int oz_workspace(OzDevice *d, size_t bytes, hipStream_t s, char **base, bool *moved = nullptr)
 {
     if (moved) *moved = false;
-    const bool capturing = stream_capturing(s);
-    if (d->ws_recorded && d->ws_stream != s && !capturing)
-        OZ_TRY(hipStreamWaitEvent(s, d->ws_done, 0));
-    if (d->ws_bytes < bytes) {
-        if (capturing)
+    if (!device_alloc_allowed(bytes)) return OZ_NO_WORKSPACE;
+    if (stream_capturing(s)) {
+        if (d->ws_bytes < bytes)
             return record_error(-4, "ozaki: the workspace would have to grow to %zu bytes during "
                                     "stream capture; run the call once outside capture first",
                                 bytes);
+        d->ws_captured = true;
+        *base = static_cast<char *>(d->ws);
+        return 0;
+    }
+    if (d->ws_recorded) OZ_TRY(hipStreamWaitEvent(s, d->ws_done, 0));
+    if (d->ws_captured) {
+        d->retired.push_back(d->ws);
+        d->ws = nullptr;
+        d->ws_bytes = 0;
+        d->ws_captured = false;
+        if (moved) *moved = true;
+    }
+    if (d->ws_bytes < bytes) {
         if (d->ws) {
             if (d->ws_recorded) OZ_TRY(hipEventSynchronize(d->ws_done));
             OZ_TRY(hipStreamSynchronize(s));
@@ -446,7 +470,11 @@ int oz_workspace(OzDevice *d, size_t bytes, hipStream_t s, char **base, bool *mo
             d->ws = nullptr;
             d->ws_bytes = 0;
         }
-        OZ_TRY(hipMalloc(&d->ws, bytes));
+        if (device_alloc(&d->ws, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            d->ws = nullptr;
+            return OZ_NO_WORKSPACE;
+        }
         d->ws_bytes = bytes;
         if (moved) *moved = true;
     }
@@ -454,13 +482,12 @@ int oz_workspace(OzDevice *d, size_t bytes, hipStream_t s, char **base, bool *mo
     return 0;
 }
 
-// Records the completion of the call just enqueued on s (after its last kernel).
+// Records the completion of the eager call just enqueued on s (after its last kernel).
 int oz_workspace_done(OzDevice *d, hipStream_t s)
 {
     if (stream_capturing(s)) return 0;
     if (!d->ws_done) OZ_TRY(hipEventCreateWithFlags(&d->ws_done, hipEventDisableTiming));
     OZ_TRY(hipEventRecord(d->ws_done, s));
-    d->ws_stream = s;
     d->ws_recorded = true;
     return 0;
 }
@@ -473,8 +500,15 @@ int oz_table(OzDevice *d, const OzPlan &p, ModConst **out)
         return 0;
     }
     ModConst *t = nullptr;
-    OZ_TRY(hipMalloc(&t, sizeof(ModConst) * OZ_MAX_MOD));
-    OZ_TRY(hipMemcpy(t, p.mc, sizeof(ModConst) * p.nmod, hipMemcpyHostToDevice));
+    if (device_alloc(reinterpret_cast<void **>(&t), sizeof(ModConst) * OZ_MAX_MOD) != hipSuccess) {
+        (void)hipGetLastError();
+        return OZ_NO_WORKSPACE;
+    }
+    if (hipMemcpy(t, p.mc, sizeof(ModConst) * p.nmod, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(t);
+        return record_error(-1, "ozaki: modulus table upload failed");
+    }
     d->tables[p.nmod] = t;
     *out = t;
     return 0;
@@ -556,8 +590,12 @@ hipError_t launch_gemm_tile(const int8_t *A, const int8_t *B, uint8_t *U, int64_
 {
     const int tiles_m = (int)(mp / (64 * TM));
     dim3 grid((unsigned)(tiles_m * (np / (64 * TN))), (unsigned)batches);
-    hipLaunchKernelGGL((k_oz_gemm<TM, TN>), grid, dim3(256), 0, s, A, B, U, kp, mp, sA, sB, sU,
-                       mcs, per, tiles_m);
+    if (kp > OZ_FOLD_DEPTH)
+        hipLaunchKernelGGL((k_oz_gemm<TM, TN, true>), grid, dim3(256), 0, s, A, B, U, kp, mp, sA,
+                           sB, sU, mcs, per, tiles_m);
+    else
+        hipLaunchKernelGGL((k_oz_gemm<TM, TN, false>), grid, dim3(256), 0, s, A, B, U, kp, mp, sA,
+                           sB, sU, mcs, per, tiles_m);
     return hipGetLastError();
 }
 
@@ -567,8 +605,12 @@ hipError_t launch_gemm_ring(const int8_t *A, const int8_t *B, uint8_t *U, int64_
 {
     const int tiles_m = (int)(mp / 256);
     dim3 grid((unsigned)(tiles_m * (np / 256)), (unsigned)batches);
-    hipLaunchKernelGGL(k_oz_gemm_ring, grid, dim3(RING_THREADS), 0, s, A, B, U, kp, mp, sA, sB,
-                       sU, mcs, per, tiles_m);
+    if (kp > OZ_FOLD_DEPTH)
+        hipLaunchKernelGGL(k_oz_gemm_ring<true>, grid, dim3(RING_THREADS), 0, s, A, B, U, kp, mp,
+                           sA, sB, sU, mcs, per, tiles_m);
+    else
+        hipLaunchKernelGGL(k_oz_gemm_ring<false>, grid, dim3(RING_THREADS), 0, s, A, B, U, kp, mp,
+                           sA, sB, sU, mcs, per, tiles_m);
     return hipGetLastError();
 }
 

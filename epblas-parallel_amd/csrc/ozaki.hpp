@@ -34,11 +34,22 @@ struct OzPlan {
 // Returns false when k is out of range.
 bool ozaki_make_plan(int64_t k_eff, int target_bits, int force_nmod, OzPlan *plan);
 
-// Largest contraction depth the INT32 accumulation keeps exact (k * 128^2 < 2^31).
-constexpr int64_t OZ_MAX_KEFF = 65536;
+// Depth after which the residue GEMMs reduce their INT32 accumulators modulo m: a sum of
+// OZ_FOLD_DEPTH products of symmetric residues (|r| <= 128) stays below 2^29, so a running
+// residue in [0, m) plus the next partial sum stays inside the 2^30 range the epilogue's
+// exact quotient handles.  Any depth is thus exact.
+constexpr int64_t OZ_FOLD_DEPTH = 32768;
+// Largest contraction depth (k for real, 2k for complex) the plans cover: the modulus count
+// grows by one bit per doubling of k, and 2^24 still fits the 40-modulus table at b = 103.
+constexpr int64_t OZ_MAX_KEFF = int64_t(1) << 24;
 
 bool ozaki_supported(int64_t k);
 bool ozaki_supported_cplx(int64_t k);
+
+// Return value of ozaki_gemm_real / ozaki_gemm_cplx when the call's workspace (or modulus
+// table) could not be allocated: nothing of the call was enqueued, and the caller runs the
+// plain path instead.
+constexpr int OZ_NO_WORKSPACE = 1;
 // a_tag != 0: the caller runs a sequence of calls that share op(A) (the column panels of one
 // host-pointer GEMM) and names it by a_tag.  A call whose tag matches the last call's on this
 // device (real or complex alike), with the same m, k, modulus count and integer width and an

@@ -205,6 +205,18 @@ DDI uint32_t crt_residue(int v, const ModConst &mc)
     return (uint32_t)__builtin_fma(-floor(d * mc.inv_up), mc.m, d);
 }
 
+// A representative of v modulo m with |r| < 4m, from 32-bit arithmetic only (light on
+// registers, for use inside the residue GEMMs' main loops).  For |v| <= 2^30 and m >= 71,
+// v/m < 2^24; the three roundings of fl(fl(v) * fl(1/m)) (relative 2^-24 each) move it by
+// less than 3 and the truncation by less than 1, so q is within 4 of v/m, and v - q*m is
+// exact in INT32.  The class modulo m is unchanged, so the epilogue's crt_residue still
+// returns the canonical residue of the whole sum.
+DDI int fold_residue(int v, float inv_m, int m)
+{
+    const int q = (int)((float)v * inv_m);
+    return v - q * m;
+}
+
 // Workgroup -> output tile.  Workgroups are dispatched round-robin over the 8 XCDs (each
 // with its own L2), so ids are first regrouped so that one XCD receives a contiguous run;
 // runs are then walked in bands of GROUP tile-rows, column by column, so the workgroups
@@ -231,7 +243,7 @@ DDI void tile_order(int bid, int tiles_m, int tiles_n, int &tm, int &tn)
 // boundaries: the next modulus's first slab is in flight while the previous modulus's
 // accumulators are reduced and stored (short K, as in batches of small problems, then
 // costs no pipeline refill per modulus).
-template <int TM, int TN, int BK = 64>
+template <int TM, int TN, int BK = 64, bool FOLD = true>
 DDI void oz_gemm_tile(const int8_t *__restrict__ Ab, const int8_t *__restrict__ Bb,
                       uint8_t *__restrict__ Ub, int64_t kp, int64_t ldu,
                       const ModConst *__restrict__ mcs, int nmod, int64_t sA, int64_t sB,
@@ -297,6 +309,19 @@ DDI void oz_gemm_tile(const int8_t *__restrict__ Ab, const int8_t *__restrict__ 
                 }
             }
     };
+    // Deep products: every OZ_FOLD_DEPTH of depth the accumulators are replaced by a small
+    // representative of their class modulo m_t, which keeps the INT32 sums exact at any depth.
+    auto fold = [&](int t) {
+        const float rm = (float)mcs[t].inv_m;
+        const int mi = (int)mcs[t].m;
+#pragma unroll
+        for (int a = 0; a < TM; ++a)
+#pragma unroll
+            for (int b = 0; b < TN; ++b)
+#pragma unroll
+                for (int e = 0; e < 16; ++e) acc[a][b][e] = fold_residue(acc[a][b][e], rm, mi);
+    };
+    constexpr int64_t FOLD_SLABS = OZ_FOLD_DEPTH / BK;
     clear();
 
     const int64_t nk = kp / BK;
@@ -371,6 +396,8 @@ DDI void oz_gemm_tile(const int8_t *__restrict__ Ab, const int8_t *__restrict__ 
         if (++ckt == nk) {
             ckt = 0;
             ++ct;
+        } else if (FOLD && ckt % FOLD_SLABS == 0) {
+            fold(ct);
         }
     }
 
@@ -431,8 +458,9 @@ DDI void wait_lds() { __builtin_amdgcn_s_waitcnt(15 | (7 << 4) | (0 << 8) | (3 <
 // are reduced and stored while later stages' DMA is in flight.  With CLAMP, only the first
 // rows_a rows of the A panel and rows_b rows of the B panel exist (multiples of 32): rows
 // beyond them are fetched as copies of the last row (valid, cached memory) and their
-// results are not stored.
-template <int WM, int WN, int TM, int TN, bool MULTI, bool CLAMP = false>
+// results are not stored.  With FOLD, the accumulators are reduced every OZ_FOLD_DEPTH of
+// depth (needed only when kp exceeds it).
+template <int WM, int WN, int TM, int TN, bool MULTI, bool CLAMP = false, bool FOLD = true>
 DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restrict__ Bb,
                            uint8_t *__restrict__ Ub, int64_t kp, int64_t ldu,
                            const ModConst *__restrict__ mcs, int nmod, int64_t sA, int64_t sB,
@@ -550,16 +578,35 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
                 }
             }
     };
-    // Compute cursor (MULTI): modulus and K slab of the stage being finished.
+    // Deep products: every OZ_FOLD_DEPTH of depth the accumulators are replaced by a small
+    // representative of their class modulo m_t, which keeps the INT32 sums exact at any depth.
+    auto fold = [&](int t) {
+        const float rm = (float)mcs[t].inv_m;
+        const int mi = (int)mcs[t].m;
+#pragma unroll
+        for (int a = 0; a < TM; ++a)
+#pragma unroll
+            for (int b = 0; b < TN; ++b)
+#pragma unroll
+                for (int e = 0; e < 16; ++e) acc[a][b][e] = fold_residue(acc[a][b][e], rm, mi);
+    };
+    constexpr int64_t FOLD_STAGES = OZ_FOLD_DEPTH / BK;
+    // Compute cursor: modulus and K slab of the stage being finished.  The last stage of the
+    // sequence is finished after the loops (flush of the last modulus).
     int ct = 0;
     int64_t ckt = 0;
     auto stage_done = [&]() {
-        if (MULTI && ++ckt == nk) {
-            flush(ct);
-            clear();
-            ckt = 0;
-            ++ct;
+        ++ckt;
+        if constexpr (MULTI) {
+            if (ckt == nk) {   // the last stage of a modulus before the last one
+                flush(ct);
+                clear();
+                ckt = 0;
+                ++ct;
+                return;
+            }
         }
+        if (FOLD && ckt % FOLD_STAGES == 0) fold(ct);
     };
     clear();
 

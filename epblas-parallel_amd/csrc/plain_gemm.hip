@@ -180,7 +180,7 @@ __global__ __launch_bounds__(256)
 void k_wgemm_plain(int64_t m, int64_t n, int64_t k, zddv alpha, zddv beta, int beta_mode,
                    const zddv *__restrict__ A, int64_t lda, const zddv *__restrict__ B,
                    int64_t ldb, zddv *__restrict__ C, int64_t ldc, int tiles_m, int rect_tiles,
-                   int64_t tri_nb)
+                   int64_t tri_nb, int diag_alt)
 {
     constexpr int BM = 16 * TM, BN = 16 * TN;
     static_assert(KCHUNK % BK == 0, "BK must divide the flush period");
@@ -188,9 +188,29 @@ void k_wgemm_plain(int64_t m, int64_t n, int64_t k, zddv alpha, zddv beta, int b
     __shared__ __attribute__((aligned(32))) zddv Bs[BK][BN];
     if constexpr (TRI != TRI_NONE) {
         if ((int)blockIdx.x >= rect_tiles) {
-            wgemmtr_diag_lane<OPA, OPB>((int64_t)(blockIdx.x - rect_tiles) * 256 + threadIdx.x,
-                                        n, k, tri_nb, TRI == TRI_UPPER, alpha, beta, beta_mode,
-                                        A, lda, B, ldb, C, ldc);
+            // A letter outside N/T/C: the reference's diagonal step reads it as 'T' where the
+            // GEMM packers of the rectangle may read it as 'C' (wgemmtr_serial.cpp:54-57 vs
+            // wgemm_serial.cpp:56-73, 81-99); diag_alt marks those operands.
+            constexpr int DA = OPA == OP_C ? OP_T : OPA, DB = OPB == OP_C ? OP_T : OPB;
+            const int64_t t = (int64_t)(blockIdx.x - rect_tiles) * 256 + threadIdx.x;
+            const bool up = TRI == TRI_UPPER;
+            switch (diag_alt) {
+            case 1:
+                wgemmtr_diag_lane<DA, OPB>(t, n, k, tri_nb, up, alpha, beta, beta_mode, A, lda, B,
+                                           ldb, C, ldc);
+                break;
+            case 2:
+                wgemmtr_diag_lane<OPA, DB>(t, n, k, tri_nb, up, alpha, beta, beta_mode, A, lda, B,
+                                           ldb, C, ldc);
+                break;
+            case 3:
+                wgemmtr_diag_lane<DA, DB>(t, n, k, tri_nb, up, alpha, beta, beta_mode, A, lda, B,
+                                          ldb, C, ldc);
+                break;
+            default:
+                wgemmtr_diag_lane<OPA, OPB>(t, n, k, tri_nb, up, alpha, beta, beta_mode, A, lda, B,
+                                            ldb, C, ldc);
+            }
             return;
         }
     }
@@ -368,7 +388,7 @@ hipError_t launch_cplx_tile(const GemmArgsZ &g, hipStream_t s, int64_t tri_nb)
     hipLaunchKernelGGL((k_wgemm_plain<CTM, CTN, CBK, OA, OB, TRI, SCALAR>),
                        dim3((unsigned)(tiles + diag_blocks)), dim3(256), 0, s, g.m, g.n, g.k,
                        g.alpha, g.beta, g.beta_mode, g.A, g.lda, g.B, g.ldb, g.C, g.ldc, tiles_m,
-                       (int)tiles, tri_nb);
+                       (int)tiles, tri_nb, TRI != TRI_NONE ? g.diag_alt : 0);
     return hipGetLastError();
 }
 

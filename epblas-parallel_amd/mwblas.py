@@ -400,6 +400,17 @@ def testing_residue_gemm(dA, dB, dU, mp, np_, kp, mod_index, variant=1, stream=N
         _check(rc)
 
 
+def testing_set_alloc_limit(nbytes):
+    """Test hook: every device buffer the library would allocate for itself above nbytes
+    counts as unobtainable (0 lifts the limit).  Returns the limit that was set."""
+    f = lib().mwgemm_testing_set_alloc_limit
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int64]
+    if f(int(nbytes)) != 0:
+        raise ValueError("bad allocation limit")
+    return int(nbytes)
+
+
 def _batched_args(ms, ns, ks, alphas, As, ldas, Bs, ldbs, betas, Cs, ldcs):
     cnt = len(ms)
     i64 = lambda v: np.ascontiguousarray(np.asarray(v, dtype=np.int64).reshape(cnt))

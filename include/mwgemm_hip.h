@@ -36,6 +36,14 @@ typedef struct { mw_float64x2 re, im; } mw_complex64x2;
 
 /* ---------------------------------------------------------------------------------
  * Drop-in boundary.
+ *
+ * Letters are read as the reference reads them; none is rejected.  Real routines: any
+ * letter but 'N' (either case) transposes (common/blas_char.h:33-36, mgemm_serial.cpp:52).
+ * Complex op(A): 'N', 'T', anything else conjugate-transposes (wgemm_serial.cpp:56-73).
+ * Complex op(B): 'N', 'T', 'C'; another letter transposes in the AVX2 operation order and
+ * conjugate-transposes in the scalar order, as the reference's two packers do
+ * (wgemm_serial.cpp:145, 174-185 and :81-99).  wgemv: 'N', 'C', anything else transposes.
+ * A call whose exact-product workspace cannot be allocated computes on the plain path.
  * ------------------------------------------------------------------------------- */
 
 /* C := alpha*op(A)*op(B) + beta*C over double-double, LP64 integers.
@@ -86,7 +94,8 @@ MWGEMM_API void wgemm_serial(char transa, char transb, ptrdiff_t m, ptrdiff_t n,
  * (macro common/epblas_facade.h:242-248), whose numerics live in mgemmtr_serial.cpp:117-200.
  * Bit-for-bit the reference's result: columns are walked in blocks of nb (MGEMMTR_NB,
  * default 8), the nb x nb diagonal triangles take the scalar faithful sequence, the rest
- * the GEMM's per-element sequence.  Uplo letters other than U/L are rejected. */
+ * the GEMM's per-element sequence.  As in the reference, uplo 'U' (either case) selects the
+ * upper triangle and any other letter the lower (mgemmtr_serial.cpp:178). */
 MWGEMM_API void mgemmtr_(const char *uplo, const char *transa, const char *transb, const int *n,
               const int *k, const mw_float64x2 *alpha, const mw_float64x2 *a, const int *lda,
               const mw_float64x2 *b, const int *ldb, const mw_float64x2 *beta, mw_float64x2 *c,
@@ -189,7 +198,12 @@ MWGEMM_API int mwgemm_set_abort(int on);                  /* returns the previou
  * every HIP launch, so the call is ordered after work queued there.  alpha/beta are host
  * values.  Return 0 or a hipError_t code.  The call is asynchronous w.r.t. the host; path
  * is one of MWGEMM_PATH_*.  The device-pointer gemmtr, gemv and batched entry points below
- * take their stream the same way. */
+ * take their stream the same way, and accept only the letters N/T/C and U/L.
+ * Exact-product calls share one workspace per device.  Eager calls on different streams are
+ * ordered on the device.  A call captured into a HIP graph (the workspace must already be
+ * large enough: make the same call once outside capture first) keeps its buffer for the
+ * graph, and later eager calls move to a new one, so graph replays and eager calls never
+ * share memory; replays of graphs captured on one device must be ordered by the caller. */
 MWGEMM_API int mwgemm_mgemm_device(char transa, char transb, int64_t m, int64_t n, int64_t k,
                         const mw_float64x2 *alpha, const mw_float64x2 *dA, int64_t lda,
                         const mw_float64x2 *dB, int64_t ldb, const mw_float64x2 *beta,
@@ -279,6 +293,13 @@ MWGEMM_API int mwgemm_autotune_clear(void);
 MWGEMM_API int mwgemm_ozaki_config(int64_t k, int cplx, int *nmod, int *bits);
 MWGEMM_API int mwgemm_ozaki_set(int bits, int nmod);
 MWGEMM_API int mwgemm_ozaki_moduli(int nmod, int *m, int *y);
+
+/* Test hook: while bytes > 0, every device buffer the library would allocate for itself
+ * (exact-product workspace and modulus table, host-operand staging, batched plan memory)
+ * larger than bytes is treated as unobtainable, as on a device with no more free memory, so
+ * the allocation-failure behaviour can be tested with ordinary shapes; 0 lifts the limit.
+ * Returns 0, or -1 for a negative size. */
+MWGEMM_API int mwgemm_testing_set_alloc_limit(int64_t bytes);
 
 /* Test hook: runs the production INT8 MFMA residue kernel once on device buffers,
  * U[j*mp + i] = (sum_k A[i*kp + k] * B[j*kp + k]) mod m with m the mod_index-th

@@ -163,9 +163,17 @@ static inline int dd_is1(dd x) { return x.hi == 1.0 && x.lo == 0.0; }
 static inline int zdd_is0(zdd x) { return dd_is0(x.re) && dd_is0(x.im); }
 static inline int zdd_is1(zdd x) { return dd_is1(x.re) && dd_is0(x.im); }
 
-/* Trans letters (common/blas_char.h:33-39): upper-case; real folds 'C' to 'T'. */
+/* Trans letters (common/blas_char.h:33-39): upper-case; real folds 'C' to 'T'.  Letters
+ * outside N/T/C are not rejected by the reference; each packer reads them its own way:
+ *   real: anything but 'N' is transposed (mgemm_serial.cpp:52, 71);
+ *   complex op(A): 'N', 'T', anything else conjugate-transposed (wgemm_serial.cpp:56-73);
+ *   complex op(B), SIMD packer: transposed, conjugated only for 'C' (wgemm_serial.cpp:145,
+ *     174-185); scalar packer: 'N', 'T', anything else conjugate-transposed (:81-99);
+ *   triangular-output diagonal: transposed unless 'N', conjugated only for 'C'
+ *     (wgemmtr_serial.cpp:54-57). */
 static inline char trans_real(char c) { c = (char)toupper((unsigned char)c); return c == 'C' ? 'T' : c; }
 static inline char trans_cplx(char c) { return (char)toupper((unsigned char)c); }
+static inline int conj_by_gemm_packer(char t) { return t != 'N' && t != 'T'; }
 
 /* Exported primitive wrappers so tests can check the primitives one by one. */
 void port_dd_mul_fast(const dd *a, const dd *b, dd *r) { *r = dd_mul_fast(*a, *b); }
@@ -428,7 +436,7 @@ static void zpack_a(const zdd *A, ptrdiff_t lda, char ta, ptrdiff_t ic, ptrdiff_
         for (ptrdiff_t i = 0; i < ib; ++i) {
             zdd v = (ta == 'N') ? A[(size_t)(pc + p) * lda + ic + i]
                                 : A[(size_t)(ic + i) * lda + pc + p];
-            Ap[p * ib + i] = (ta == 'C') ? zdd_conj(v) : v;
+            Ap[p * ib + i] = conj_by_gemm_packer(ta) ? zdd_conj(v) : v;
         }
 }
 
@@ -487,11 +495,11 @@ int port_wgemm(char transa, char transb, ptrdiff_t m, ptrdiff_t n, ptrdiff_t k,
             zdd *cj = c + (size_t)j * ldc;
             for (ptrdiff_t p = 0; p < k; ++p) {
                 zdd bpj = (tb == 'N') ? b[(size_t)j * ldb + p] : b[(size_t)p * ldb + j];
-                if (tb == 'C') bpj = zdd_conj(bpj);
+                if (conj_by_gemm_packer(tb)) bpj = zdd_conj(bpj);
                 const zdd t = zdd_mul_exact(alpha, bpj);
                 for (ptrdiff_t i = 0; i < m; ++i) {
                     zdd aip = (ta == 'N') ? a[(size_t)p * lda + i] : a[(size_t)i * lda + p];
-                    if (ta == 'C') aip = zdd_conj(aip);
+                    if (conj_by_gemm_packer(ta)) aip = zdd_conj(aip);
                     cj[i] = zdd_add_exact(cj[i], zdd_mul_exact(t, aip));
                 }
             }

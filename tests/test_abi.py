@@ -52,25 +52,17 @@ def test_empty_problem_returns_without_touching_anything():
 def test_invalid_arguments_fail_loudly():
     A = np.zeros((4, 4, 2))
     C = np.zeros((4, 4, 2))
-    with pytest.raises(mwblas.MwgemmError, match="trans"):
-        mwblas.mgemm("X", "N", 4, 4, 4, [1, 0], A, 4, A, 4, [0, 0], C, 4)
+    # drop-in entry points: a leading dimension shorter than its rows is rejected before
+    # anything is staged (the reference would read overlapping columns); letters are read
+    # as the reference reads them (tests/test_gpu_letters.py), never rejected
     with pytest.raises(mwblas.MwgemmError, match="leading dimension"):
         mwblas.mgemm("N", "N", 4, 4, 4, [1, 0], A, 3, A, 4, [0, 0], C, 4)
-    with pytest.raises(mwblas.MwgemmError, match="trans"):
-        mwblas.wgemm("N", "Q", 4, 4, 4, [1, 0, 0, 0], A, 4, A, 4, [0, 0, 0, 0], C, 4)
-    # triangular-output GEMM: uplo must be U or L (the reference treats any other letter
-    # as L); trans letters and leading dimensions as for the GEMM
-    with pytest.raises(mwblas.MwgemmError, match="uplo"):
-        mwblas.mgemmtr("X", "N", "N", 4, 4, [1, 0], A, 4, A, 4, [0, 0], C, 4)
-    with pytest.raises(mwblas.MwgemmError, match="trans"):
-        mwblas.mgemmtr("U", "N", "Z", 4, 4, [1, 0], A, 4, A, 4, [0, 0], C, 4)
+    with pytest.raises(mwblas.MwgemmError, match="leading dimension"):
+        mwblas.mgemm("X", "N", 4, 4, 4, [1, 0], A, 3, A, 4, [0, 0], C, 4)   # 'X' reads as 'T'
     with pytest.raises(mwblas.MwgemmError, match="leading dimension"):
         mwblas.wgemmtr("L", "N", "N", 4, 4, [1, 0, 0, 0], A, 4, A, 4, [0, 0, 0, 0], C, 3)
     mwblas.mgemmtr("U", "N", "N", 0, 4, [1, 0], A, 4, A, 4, [0, 0], C, 4)   # n == 0: no-op
-    # matrix-vector products: trans letter, lda, zero increments
     v = np.zeros((8, 2))
-    with pytest.raises(mwblas.MwgemmError, match="trans"):
-        mwblas.mgemv("Q", 4, 4, [1, 0], A, 4, v, 1, [0, 0], v, 1)
     with pytest.raises(mwblas.MwgemmError, match="leading dimension"):
         mwblas.mgemv("N", 4, 4, [1, 0], A, 3, v, 1, [0, 0], v, 1)
     with pytest.raises(mwblas.MwgemmError, match="increment"):
@@ -79,6 +71,21 @@ def test_invalid_arguments_fail_loudly():
     mwblas.mgemv("N", 0, 4, [1, 0], A, 1, v, 1, [2, 0], y, 1)     # m == 0: y untouched
     assert np.all(y == 3.0)
     assert np.all(C == 0.0)
+    # the library's own device entry points (no reference counterpart) take only N/T/C and
+    # U/L; the letter is checked before any pointer is used
+    with pytest.raises(mwblas.MwgemmError, match="trans"):
+        mwblas.gemm_device(False, "X", "N", 4, 4, 4, [1, 0], None, 4, None, 4, [0, 0], None, 4)
+    with pytest.raises(mwblas.MwgemmError, match="trans"):
+        mwblas.gemm_device(True, "N", "Q", 4, 4, 4, [1, 0, 0, 0], None, 4, None, 4,
+                           [0, 0, 0, 0], None, 4)
+    with pytest.raises(mwblas.MwgemmError, match="uplo"):
+        mwblas.gemmtr_device(False, "X", "N", "N", 4, 4, [1, 0], None, 4, None, 4, [0, 0],
+                             None, 4)
+    with pytest.raises(mwblas.MwgemmError, match="trans"):
+        mwblas.gemmtr_device(True, "U", "N", "Z", 4, 4, [1, 0, 0, 0], None, 4, None, 4,
+                             [0, 0, 0, 0], None, 4)
+    with pytest.raises(mwblas.MwgemmError, match="trans"):
+        mwblas.gemv_device(False, "Q", 4, 4, [1, 0], None, 4, None, 1, [0, 0], None, 1)
 
 
 def test_path_switch():

@@ -19,21 +19,29 @@ import oracle
 pytestmark = pytest.mark.gpu
 
 
+def _omp_default_team():
+    """The team an OpenMP runtime forms by default (omp_get_max_threads): the first entry of
+    OMP_NUM_THREADS, else the CPUs of the process's affinity mask, capped by
+    OMP_THREAD_LIMIT."""
+    team = 0
+    v = os.environ.get("OMP_NUM_THREADS")
+    if v:
+        try:
+            team = int(v.split(",")[0])
+        except ValueError:
+            team = 0
+    if team <= 0:
+        team = len(os.sched_getaffinity(0))
+    lim = os.environ.get("OMP_THREAD_LIMIT")
+    if lim and lim.isdigit() and 0 < int(lim) < team:
+        team = int(lim)
+    return team
+
+
 def _dropin_bands(m):
-    """The band count the drop-in uses (same rule as the library)."""
-    team = None
-    for name in ("MWGEMV_THREADS", "OMP_NUM_THREADS"):
-        v = os.environ.get(name)
-        if v:
-            try:
-                team = int(v.split(",")[0])
-            except ValueError:
-                team = None
-            if team and team > 0:
-                break
-            team = None
-    if team is None:
-        team = os.cpu_count() or 1
+    """The band count the drop-in uses: MWGEMV_THREADS, else the OpenMP default team."""
+    v = os.environ.get("MWGEMV_THREADS")
+    team = int(v) if v and v.isdigit() and int(v) > 0 else _omp_default_team()
     return 1 if m < 64 else min(team, 256)
 
 
@@ -156,3 +164,47 @@ def test_twins_agree(gpu):
     outs.append(dy.cpu().numpy())
     for o in outs:
         assert np.array_equal(o, _port(c, True, 1), equal_nan=True)
+
+
+_CHILD = r"""
+import os, sys
+import numpy as np
+os.sched_setaffinity(0, set(sorted(os.sched_getaffinity(0))[:3]))
+sys.path[:0] = sys.argv[1:3]
+import ddgen, mwblas, oracle
+rng = np.random.default_rng(777)
+m, n = 203, 150
+A = ddgen.real_matrix(rng, m, n)
+x = ddgen.real_matrix(rng, n, 1)[0]
+y0 = ddgen.real_matrix(rng, m, 1)[0]
+al, be = ddgen.real_scalar(rng, scale=2), ddgen.real_scalar(rng, scale=2)
+y = y0.copy()
+mwblas.mgemv("N", m, n, al, A, m, x, 1, be, y, 1)
+want = y0.copy()
+oracle.port_mgemv("N", m, n, al, A, m, x, 1, be, want, 1, nbands=3)
+other = y0.copy()
+oracle.port_mgemv("N", m, n, al, A, m, x, 1, be, other, 1, nbands=4)
+assert not np.array_equal(want, other), "band count 3 and 4 give the same bits"
+assert np.array_equal(y, want), "drop-in band count differs from the 3-CPU affinity mask"
+print("affinity child ok")
+"""
+
+
+def test_dropin_bands_follow_a_narrowed_affinity_mask(gpu):
+    """A process restricted to three CPUs (taskset, a cgroup cpuset) gets a three-thread
+    OpenMP team from the reference, so the drop-in must split op 'N' rows into three bands.
+    Runs in a child whose mask is narrowed before the library or the GPU is touched."""
+    import subprocess
+    import sys
+    if len(os.sched_getaffinity(0)) < 4:
+        pytest.skip("needs at least four CPUs to narrow the mask to three")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ)
+    for name in ("MWGEMV_THREADS", "OMP_NUM_THREADS", "OMP_THREAD_LIMIT"):
+        env.pop(name, None)
+    print("child: mgemv_ 'N' 203 x 150 under a 3-CPU affinity mask", flush=True)
+    r = subprocess.run([sys.executable, "-c", _CHILD, os.path.join(root, "epblas-parallel_amd"),
+                        os.path.join(root, "oracle")], capture_output=True, text=True, env=env,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "affinity child ok" in r.stdout

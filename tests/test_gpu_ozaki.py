@@ -200,3 +200,81 @@ def test_4096_complex_conj_vs_binary128(gpu):
                                  entries=ent)
         print(f"4096 complex path={path} err={r['err']:.3e}", flush=True)
         assert r["err"] <= BAR
+
+
+@pytest.mark.parametrize("variant,kp", [(0, 70016), (1, 70016), (2, 40000), (3, 40000)])
+def test_residue_gemm_deeper_than_one_accumulation(gpu, variant, kp):
+    """Depth beyond 32768: every residue-GEMM tile reduces its INT32 sums modulo m on the
+    way, so the stored residue is still exact (integer reference in float64, exact below
+    2^53)."""
+    import torch
+    rng = np.random.default_rng(140 + variant)
+    mp, np_ = 256, 512
+    m_all, _ = mwblas.ozaki_moduli(40)
+    mod_index = (1, 5, 28, 39)[variant]
+    m = int(m_all[mod_index])
+    h = (m - 1) // 2
+    A = rng.integers(-h, h + 1, size=(mp, kp), dtype=np.int64)
+    B = rng.integers(-h, h + 1, size=(np_, kp), dtype=np.int64)
+    dA = torch.from_numpy(A.astype(np.int8)).to(gpu)
+    dB = torch.from_numpy(B.astype(np.int8)).to(gpu)
+    dU = torch.zeros((np_, mp), dtype=torch.uint8, device=gpu)
+    print(f"deep residue gemm variant {variant} mod {m} kp {kp}", flush=True)
+    mwblas.testing_residue_gemm(dA, dB, dU, mp, np_, kp, mod_index, variant)
+    torch.cuda.synchronize()
+    expect = (A.astype(np.float64) @ B.T.astype(np.float64)).astype(np.int64) % m
+    assert np.array_equal(dU.cpu().numpy().astype(np.int64).T, expect)
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_residue_gemm_sum_beyond_int32(gpu, variant):
+    """127 * 127 over a depth of 135168 sums to 2.18e9 > 2^31: an INT32 accumulator without
+    the modular reduction would wrap (and 2^32 = 1 mod 255 would change the residue)."""
+    import torch
+    mp, np_, kp, mod_index = 256, 256, 135168, 1
+    m = int(mwblas.ozaki_moduli(40)[0][mod_index])
+    assert m == 255
+    dA = torch.full((mp, kp), 127, dtype=torch.int8, device=gpu)
+    dB = torch.full((np_, kp), 127, dtype=torch.int8, device=gpu)
+    dB[1::2, :] = -127
+    dU = torch.zeros((np_, mp), dtype=torch.uint8, device=gpu)
+    print(f"residue gemm variant {variant}, constant +-127 operands, kp {kp}", flush=True)
+    mwblas.testing_residue_gemm(dA, dB, dU, mp, np_, kp, mod_index, variant)
+    torch.cuda.synchronize()
+    got = dU.cpu().numpy().astype(np.int64)
+    assert np.all(got[0::2] == (127 * 127 * kp) % m)
+    assert np.all(got[1::2] == (-127 * 127 * kp) % m)
+
+
+@pytest.mark.parametrize("cplx,k", [(False, 70000), (True, 40000)])
+def test_exact_product_deeper_than_65536(gpu, cplx, k):
+    """Real k = 70000 and complex k = 40000 (2k = 80000) stay on the exact-product path and
+    within 1e-30 of binary128 on sampled entries."""
+    import torch
+    rng = np.random.default_rng(150 + cplx)
+    m, n = (256, 192) if not cplx else (160, 128)
+    gen = ddgen.cplx_matrix if cplx else ddgen.real_matrix
+    scal = ddgen.cplx_scalar if cplx else ddgen.real_scalar
+    A, B, C0 = gen(rng, k, m), gen(rng, k, n), gen(rng, m, n)
+    al, be = scal(rng, scale=2), scal(rng, scale=2)
+    ta = "C" if cplx else "T"
+    dA, dB, dC = (torch.from_numpy(x).to(gpu) for x in (A, B, C0))
+    mwblas.timer_reset()
+    mwblas.timer_enable(True)
+    try:
+        mwblas.gemm_device(cplx, ta, "N", m, n, k, al, dA, k, dB, k, be, dC, m,
+                           path=mwblas.PATH_OZAKI)
+        torch.cuda.synchronize()
+    finally:
+        mwblas.timer_enable(False)
+    kt = mwblas.timer_read()
+    assert kt.get("oz_gemm", (0, 0))[1] == 1 and kt.get("plain_real", (0, 0))[1] == 0
+    assert kt.get("plain_cplx", (0, 0))[1] == 0
+    ent = (rng.integers(0, m, 96), rng.integers(0, n, 96))
+    r = oracle.q128_gemm_err(cplx, ta, "N", m, n, k, al, A, k, B, k, be, C0, m,
+                             dC.cpu().numpy(), m, entries=ent)
+    nmod, bits = mwblas.ozaki_config(k, cplx)
+    print(f"deep {'complex' if cplx else 'real'} k={k}: {nmod} moduli, err={r['err']:.3e}",
+          flush=True)
+    assert r["err"] <= BAR
+    assert r["err"] <= 4 * 2.0 ** -bits

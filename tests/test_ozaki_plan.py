@@ -24,7 +24,8 @@ def test_moduli_pairwise_coprime_and_weights_are_inverses():
 
 
 @pytest.mark.parametrize("k,cplx", [(1, False), (512, False), (8192, False), (65536, False),
-                                    (4096, True), (32768, True)])
+                                    (70000, False), (1 << 20, False), (4096, True),
+                                    (32768, True), (40000, True)])
 def test_plan_covers_the_product_range(k, cplx):
     nmod, bits = mwblas.ozaki_config(k, cplx)
     mm, _ = mwblas.ozaki_moduli(nmod)
@@ -40,10 +41,26 @@ def test_plan_covers_the_product_range(k, cplx):
 
 
 def test_depth_limit_is_rejected():
+    """The residue GEMMs reduce their sums modulo m every 32768 of depth, so the INT32
+    accumulation sets no depth limit; the plans stop at k_eff = 2^24 (k real, 2k complex)."""
+    mwblas.ozaki_config(1 << 24)
+    mwblas.ozaki_config(1 << 23, cplx=True)
     with pytest.raises(mwblas.MwgemmError):
-        mwblas.ozaki_config(65537)
+        mwblas.ozaki_config((1 << 24) + 1)
     with pytest.raises(mwblas.MwgemmError):
-        mwblas.ozaki_config(32769, cplx=True)
+        mwblas.ozaki_config((1 << 23) + 1, cplx=True)
+
+
+def test_explicit_exact_product_beyond_the_plans_fails_loudly():
+    """An explicit exact-product request deeper than the plans cover is an error, not a
+    silent plain-path run (AUTO takes the plain path there).  The depth is checked before
+    any pointer is used or any device call is made."""
+    k = (1 << 24) + 64
+    for cplx in (False, True):
+        w = 4 if cplx else 2
+        with pytest.raises(mwblas.MwgemmError, match="depth"):
+            mwblas.gemm_device(cplx, "T", "N", 4, 4, k, [1.0] + [0.0] * (w - 1), None, k, None,
+                               k, [0.0] * w, None, 4, path=mwblas.PATH_OZAKI)
 
 
 def test_fixed_point_crt_reconstruction_identity():
