@@ -464,7 +464,9 @@ int choose_path(int path, int64_t m, int64_t n, int64_t k, bool cplx)
 // ----------------------------------------------------------------------------------
 // Device-resident cores.
 // ----------------------------------------------------------------------------------
-int gemm_real_device(GemmArgsR g, int path, hipStream_t s, uint64_t a_tag)
+// noalloc: the no-allocation entry point (the exact product in the static scratch memory;
+// AUTO takes it only where it fits, an explicit OZAKI request that does not fit fails).
+int gemm_real_device(GemmArgsR g, int path, hipStream_t s, uint64_t a_tag, bool noalloc)
 {
     g.order = plain_order();
     if (g.m <= 0 || g.n <= 0) return 0;
@@ -481,7 +483,8 @@ int gemm_real_device(GemmArgsR g, int path, hipStream_t s, uint64_t a_tag)
         return 0;
     }
     if (choose_path(path, g.m, g.n, g.k, false) == MWGEMM_PATH_OZAKI) {
-        const int rc = ozaki_gemm_real(g, s, a_tag);
+        const int rc = noalloc ? ozaki_gemm_real_noalloc(g, s, path == MWGEMM_PATH_AUTO)
+                               : ozaki_gemm_real(g, s, a_tag);
         // Without its workspace the exact product runs on the plain path, which needs none
         // (the reference's driver likewise recomputes serially when its pack buffers cannot
         // be had, mgemm_parallel.cpp:180-186).
@@ -1135,9 +1138,16 @@ void drain_slots()
 }
 }  // namespace
 
+namespace {
+thread_local bool t_timers_paused = false;
+}
+
+TimerPause::TimerPause() : prev(t_timers_paused) { t_timers_paused = true; }
+TimerPause::~TimerPause() { t_timers_paused = prev; }
+
 int timer_begin(int cls, hipStream_t s)
 {
-    if (!g_timer_on.load(std::memory_order_relaxed)) return -1;
+    if (t_timers_paused || !g_timer_on.load(std::memory_order_relaxed)) return -1;
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) {
         (void)hipGetLastError();
@@ -1404,6 +1414,26 @@ int mwgemm_mgemm_device(char transa, char transb, int64_t m, int64_t n, int64_t 
     g.C = reinterpret_cast<ddv *>(dC); g.ldc = ldc;
     hipStream_t s = static_cast<hipStream_t>(stream);   // NULL: HIP's default stream
     return gemm_real_device(g, path, s);
+}
+
+int mwgemm_mgemm_device_noalloc(char transa, char transb, int64_t m, int64_t n, int64_t k,
+                                const mw_float64x2 *alpha, const mw_float64x2 *dA, int64_t lda,
+                                const mw_float64x2 *dB, int64_t ldb, const mw_float64x2 *beta,
+                                mw_float64x2 *dC, int64_t ldc, int path, void *stream)
+{
+    TimerPause no_timer_events;
+    GemmArgsR g{};
+    g.m = m; g.n = n; g.k = k;
+    g.opa = op_real(transa); g.opb = op_real(transb);
+    g.alpha = ddv{alpha->limbs[0], alpha->limbs[1]};
+    g.beta = ddv{beta->limbs[0], beta->limbs[1]};
+    g.A = reinterpret_cast<const ddv *>(dA); g.lda = lda;
+    g.B = reinterpret_cast<const ddv *>(dB); g.ldb = ldb;
+    g.C = reinterpret_cast<ddv *>(dC); g.ldc = ldc;
+    if (path < MWGEMM_PATH_PLAIN || path > MWGEMM_PATH_AUTO)
+        return record_error(-3, "no-allocation gemm: bad path %d", path);
+    hipStream_t s = static_cast<hipStream_t>(stream);   // NULL: HIP's default stream
+    return gemm_real_device(g, path, s, 0, true);
 }
 
 int mwgemm_wgemm_device(char transa, char transb, int64_t m, int64_t n, int64_t k,

@@ -50,7 +50,8 @@ hipError_t device_alloc(void **p, size_t bytes);
 bool device_alloc_allowed(size_t bytes);
 
 // Device-pointer GEMM cores (api.cpp).  a_tag: see ozaki_gemm_real.
-int gemm_real_device(GemmArgsR g, int path, hipStream_t s, uint64_t a_tag = 0);
+int gemm_real_device(GemmArgsR g, int path, hipStream_t s, uint64_t a_tag = 0,
+                     bool noalloc = false);
 int gemm_cplx_device(GemmArgsZ g, int path, hipStream_t s, uint64_t a_tag = 0);
 
 // Autotuner table (autotune.cpp).  Kinds of tuned decisions:

@@ -642,39 +642,46 @@ hipError_t fill_zero(unsigned long long *p, int64_t n, hipStream_t s)
 
 }  // namespace
 
-int ozaki_gemm_real(const GemmArgsR &g, hipStream_t s, uint64_t a_tag)
+namespace {
+// Workspace of one real exact-product call.  K is padded to 64 (the residue tiles consume
+// 64-deep steps), rows and columns to 256.  Layout: [A maxima | A residues | B maxima |
+// B residues | reduced products]: the op(A) part comes first, so that it stays in place for
+// calls with a different n.
+struct OzLayoutR {
+    int64_t Mp, Np, Kc;
+    size_t oRA, oA, oRB, oB, oU, bytes;
+};
+
+OzLayoutR oz_layout_real(int64_t m, int64_t n, int64_t k, int nmod)
 {
-    std::lock_guard<std::mutex> lk(g_oz_mu);
-    OzPlan plan;
-    if (!ozaki_make_plan(g.k, ozaki_target_bits(), ozaki_forced_nmod(), &plan))
-        return record_error(-5, "ozaki: contraction depth %lld out of range", (long long)g.k);
-    OzDevice *d;
-    if (int rc = oz_device(&d)) return rc;
-    ModConst *mcs;
-    if (int rc = oz_table(d, plan, &mcs)) return rc;
+    OzLayoutR L;
+    L.Mp = round_up(m, 256);
+    L.Np = round_up(n, 256);
+    L.Kc = round_up(k, 64);
+    const size_t szA = (size_t)nmod * L.Mp * L.Kc, szB = (size_t)nmod * L.Np * L.Kc;
+    const size_t szU = (size_t)nmod * L.Np * L.Mp;
+    L.oRA = 0;
+    L.oA = round_up(L.Mp * 8, 256);
+    L.oRB = L.oA + round_up(szA, 256);
+    L.oB = L.oRB + round_up(L.Np * 8, 256);
+    L.oU = L.oB + round_up(szB, 256);
+    L.bytes = L.oU + szU + 1024;
+    return L;
+}
 
-    // K padded to 64 (the residue tiles consume 64-deep steps), rows and columns to 256.
-    // Layout: [A maxima | A residues | B maxima | B residues | reduced products]: the op(A)
-    // part comes first, so that it stays in place for calls with a different n.
-    const int64_t Mp = round_up(g.m, 256), Np = round_up(g.n, 256), Kc = round_up(g.k, 64);
+// The kernels of one real exact-product call on the workspace at base (layout L); with
+// reuse_a, op(A)'s maxima and residues are already in place.  Only kernel launches: the
+// maxima are cleared by a fill kernel, nothing is allocated.
+int oz_enqueue_real(const GemmArgsR &g, const OzPlan &plan, const ModConst *mcs, char *base,
+                    const OzLayoutR &L, bool reuse_a, hipStream_t s)
+{
+    const int64_t Mp = L.Mp, Np = L.Np, Kc = L.Kc;
     const int nmod = plan.nmod;
-    const size_t szA = (size_t)nmod * Mp * Kc, szB = (size_t)nmod * Np * Kc;
-    const size_t szU = (size_t)nmod * Np * Mp;
-    const size_t oRA = 0, oA = round_up(Mp * 8, 256), oRB = oA + round_up(szA, 256);
-    const size_t oB = oRB + round_up(Np * 8, 256), oU = oB + round_up(szB, 256);
-    char *base;
-    bool moved = false;
-    if (int rc = oz_workspace(d, oU + szU + 1024, s, &base, &moved)) return rc;
-    unsigned long long *rA = reinterpret_cast<unsigned long long *>(base + oRA);
-    unsigned long long *rB = reinterpret_cast<unsigned long long *>(base + oRB);
-    int8_t *resA = reinterpret_cast<int8_t *>(base + oA);
-    int8_t *resB = reinterpret_cast<int8_t *>(base + oB);
-    uint8_t *Ub = reinterpret_cast<uint8_t *>(base + oU);
-    const bool reuse_a = a_tag != 0 && !moved && d->a_tag == a_tag && d->a_m == g.m &&
-                         d->a_k == g.k && d->a_nmod == nmod && d->a_bits == plan.bits &&
-                         !d->a_cplx;
-    d->a_tag = 0;   // set again below once this call's op(A) part is enqueued
-
+    unsigned long long *rA = reinterpret_cast<unsigned long long *>(base + L.oRA);
+    unsigned long long *rB = reinterpret_cast<unsigned long long *>(base + L.oRB);
+    int8_t *resA = reinterpret_cast<int8_t *>(base + L.oA);
+    int8_t *resB = reinterpret_cast<int8_t *>(base + L.oB);
+    uint8_t *Ub = reinterpret_cast<uint8_t *>(base + L.oU);
     const bool a_kc = g.opa != OP_N;   // op(A) row i contiguous in k when A is transposed
     const bool b_kc = g.opb == OP_N;   // op(B) column j contiguous in k when B is not
     {
@@ -694,12 +701,6 @@ int ozaki_gemm_real(const GemmArgsR &g, hipStream_t s, uint64_t a_tag)
         OZ_TRY(launch_split(g.B, g.ldb, g.n, g.k, Kc, b_kc, rB, plan.bits, 0, mcs, nmod, resB, 1,
                             (int8_t *)nullptr, 0, Kc, Np * Kc, s));
     }
-    d->a_tag = a_tag;
-    d->a_m = g.m;
-    d->a_k = g.k;
-    d->a_nmod = nmod;
-    d->a_bits = plan.bits;
-    d->a_cplx = false;
     {
         ScopedTimer tm(KC_OZ_GEMM, s);
         OZ_TRY(launch_gemm(resA, resB, Ub, Mp, Np, Kc, Mp * Kc, Np * Kc, Np * Mp, mcs, 1, nmod, g.variant, s));
@@ -712,7 +713,105 @@ int ozaki_gemm_real(const GemmArgsR &g, hipStream_t s, uint64_t a_tag)
                            bpc, g.alpha, g.beta, g.beta_mode, g.C, g.ldc);
         OZ_TRY(hipGetLastError());
     }
+    return 0;
+}
+}  // namespace
+
+int ozaki_gemm_real(const GemmArgsR &g, hipStream_t s, uint64_t a_tag)
+{
+    std::lock_guard<std::mutex> lk(g_oz_mu);
+    OzPlan plan;
+    if (!ozaki_make_plan(g.k, ozaki_target_bits(), ozaki_forced_nmod(), &plan))
+        return record_error(-5, "ozaki: contraction depth %lld out of range", (long long)g.k);
+    OzDevice *d;
+    if (int rc = oz_device(&d)) return rc;
+    ModConst *mcs;
+    if (int rc = oz_table(d, plan, &mcs)) return rc;
+    const OzLayoutR L = oz_layout_real(g.m, g.n, g.k, plan.nmod);
+    char *base;
+    bool moved = false;
+    if (int rc = oz_workspace(d, L.bytes, s, &base, &moved)) return rc;
+    const bool reuse_a = a_tag != 0 && !moved && d->a_tag == a_tag && d->a_m == g.m &&
+                         d->a_k == g.k && d->a_nmod == plan.nmod && d->a_bits == plan.bits &&
+                         !d->a_cplx;
+    d->a_tag = 0;   // set again once this call's op(A) part is enqueued
+    if (int rc = oz_enqueue_real(g, plan, mcs, base, L, reuse_a, s)) return rc;
+    d->a_tag = a_tag;
+    d->a_m = g.m;
+    d->a_k = g.k;
+    d->a_nmod = plan.nmod;
+    d->a_bits = plan.bits;
+    d->a_cplx = false;
     return oz_workspace_done(d, s);
+}
+
+// ==================================================================================
+// The no-allocation variant of the real device entry point.
+// ==================================================================================
+// Device memory of the no-allocation entry point, one copy per GPU, held until the process
+// ends: the modulus tables of every modulus count (written once per device at set-up, read
+// only afterwards), then the workspace of one exact-product call.  The host never uses the
+// array's name as an address: set-up looks up each device's copy with hipGetSymbolAddress
+// and keeps that pointer.  Every call on a GPU reuses the same memory (a call clears what
+// must start at zero), so calls that overlap on one GPU race.
+constexpr size_t NOALLOC_TABLES = ((sizeof(ModConst) * OZ_MAX_MOD * OZ_MAX_MOD + 255) / 256) * 256;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_noalloc_mu;
+std::vector<char *> g_noalloc_ptr;   // per device: its copy of scratch, once set up
+
+// Set-up on the current device (once): the address of its copy of scratch, and the modulus
+// tables uploaded into its head.  Must not run under stream capture (the upload is a
+// synchronous copy).
+int noalloc_scratch(hipStream_t s, char **out)
+{
+    int dev = 0;
+    OZ_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if ((int)g_noalloc_ptr.size() <= dev) g_noalloc_ptr.resize(dev + 1, nullptr);
+    if (!g_noalloc_ptr[dev]) {
+        if (stream_capturing(s))
+            return record_error(-4, "no-allocation gemm: the first call on a device sets up its "
+                                    "scratch memory and must not be captured");
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            (void)hipGetLastError();
+            return record_error((int)e, "no-allocation gemm: hipGetSymbolAddress(scratch) failed: %s",
+                                hipGetErrorString(e));
+        }
+        static ModConst host[OZ_MAX_MOD][OZ_MAX_MOD];   // [nmod - 1][t]
+        for (int nm = 1; nm <= OZ_MAX_MOD; ++nm) {
+            OzPlan p;
+            if (!ozaki_make_plan(1, 8, nm, &p)) return record_error(-5, "no-allocation gemm: plan");
+            for (int t = 0; t < nm; ++t) host[nm - 1][t] = p.mc[t];
+        }
+        OZ_TRY(hipMemcpy(ptr, host, sizeof host, hipMemcpyHostToDevice));
+        g_noalloc_ptr[dev] = static_cast<char *>(ptr);
+    }
+    *out = g_noalloc_ptr[dev];
+    return 0;
+}
+}  // namespace
+
+int ozaki_gemm_real_noalloc(const GemmArgsR &g, hipStream_t s, bool may_fall_back)
+{
+    OzPlan plan;
+    if (!ozaki_make_plan(g.k, ozaki_target_bits(), ozaki_forced_nmod(), &plan))
+        return record_error(-5, "ozaki: contraction depth %lld out of range", (long long)g.k);
+    const OzLayoutR L = oz_layout_real(g.m, g.n, g.k, plan.nmod);
+    if (L.bytes > NOALLOC_SCRATCH_BYTES - NOALLOC_TABLES) {
+        if (may_fall_back) return OZ_NO_WORKSPACE;
+        return record_error(-7, "no-allocation gemm: the exact product of %lld x %lld x %lld needs "
+                                "%zu bytes of workspace, more than the %zu bytes of its scratch "
+                                "memory", (long long)g.m, (long long)g.n, (long long)g.k,
+                            L.bytes, NOALLOC_SCRATCH_BYTES - NOALLOC_TABLES);
+    }
+    char *ptr = nullptr;
+    if (int rc = noalloc_scratch(s, &ptr)) return rc;
+    const ModConst *mcs = reinterpret_cast<const ModConst *>(ptr) + (size_t)(plan.nmod - 1) * OZ_MAX_MOD;
+    return oz_enqueue_real(g, plan, mcs, ptr + NOALLOC_TABLES, L, false, s);
 }
 
 int ozaki_gemm_cplx(const GemmArgsZ &g, hipStream_t s, uint64_t a_tag)

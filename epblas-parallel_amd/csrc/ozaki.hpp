@@ -56,6 +56,11 @@ constexpr int OZ_NO_WORKSPACE = 1;
 // unmoved workspace, reuses op(A)'s maxima and residues instead of recomputing them; any
 // other call recomputes them.
 int ozaki_gemm_real(const GemmArgsR &g, hipStream_t s, uint64_t a_tag = 0);
+// The same product with its workspace in the library's static per-device scratch memory
+// (NOALLOC_SCRATCH_BYTES, no device allocation).  A product whose workspace does not fit
+// returns OZ_NO_WORKSPACE when may_fall_back, else an error.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(128) << 20;
+int ozaki_gemm_real_noalloc(const GemmArgsR &g, hipStream_t s, bool may_fall_back);
 int ozaki_gemm_cplx(const GemmArgsZ &g, hipStream_t s, uint64_t a_tag = 0);
 
 // Test hook: U[j*ldu + i] = (sum_k A[i*kp + k] * B[j*kp + k]) mod m for one

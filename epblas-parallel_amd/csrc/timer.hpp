@@ -26,6 +26,14 @@ int timer_begin(int cls, hipStream_t s);
 // Records the stop event of slot id on s (no-op for id < 0).
 void timer_end(int id, hipStream_t s);
 
+// While alive, the kernel timers record nothing on the calling thread (their events are
+// created per launch, which the no-allocation entry point must not do).
+struct TimerPause {
+    bool prev;
+    TimerPause();
+    ~TimerPause();
+};
+
 struct ScopedTimer {
     int id;
     hipStream_t s;

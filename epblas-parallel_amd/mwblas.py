@@ -335,6 +335,23 @@ def gemm_device(cplx, transa, transb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC
         _check(rc)
 
 
+def gemm_device_noalloc(transa, transb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc,
+                        path=PATH_PLAIN, stream=None):
+    """Real device-pointer GEMM through mwgemm_mgemm_device_noalloc: the same result as
+    gemm_device, with the exact-product workspace in the library's static scratch memory."""
+    f = lib().mwgemm_mgemm_device_noalloc
+    f.restype = ctypes.c_int
+    i64 = ctypes.c_int64
+    f.argtypes = [ctypes.c_char, ctypes.c_char, i64, i64, i64, _vp, _vp, i64, _vp, i64, _vp, _vp,
+                  i64, ctypes.c_int, _vp]
+    al, be = _scalar(alpha, 2), _scalar(beta, 2)
+    clear_error()
+    rc = f(transa.encode(), transb.encode(), m, n, k, _ptr(al), _ptr(dA), lda, _ptr(dB), ldb,
+           _ptr(be), _ptr(dC), ldc, path, None if stream is None else ctypes.c_void_p(stream))
+    if rc != 0:
+        _check(rc)
+
+
 def timer_enable(on=True):
     return lib().mwgemm_timer_enable(1 if on else 0)
 

@@ -208,6 +208,23 @@ MWGEMM_API int mwgemm_mgemm_device(char transa, char transb, int64_t m, int64_t 
                         const mw_float64x2 *alpha, const mw_float64x2 *dA, int64_t lda,
                         const mw_float64x2 *dB, int64_t ldb, const mw_float64x2 *beta,
                         mw_float64x2 *dC, int64_t ldc, int path, void *stream);
+/* No counterpart in the reference: mwgemm_mgemm_device for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and stream semantics; it calls no hipMalloc, hipFree
+ * or other allocating routine, and kernel timers are not recorded for it.  The plain path
+ * needs no memory beyond the operands.  The exact-product path keeps its workspace in a
+ * static per-device scratch array of 128 MiB (held by each GPU the library runs on until
+ * the process ends, besides 138 KiB of modulus tables in its head); a product whose
+ * workspace does not fit (about 1100^3 for a square one) is rejected with an error when
+ * OZAKI is requested, and computed on the plain path under AUTO.  The first call on a device
+ * sets up its scratch (address lookup, table upload) and must not be captured.  Every call
+ * on a GPU shares that GPU's scratch: calls that overlap on one GPU race. */
+MWGEMM_API int mwgemm_mgemm_device_noalloc(char transa, char transb, int64_t m, int64_t n,
+                                           int64_t k, const mw_float64x2 *alpha,
+                                           const mw_float64x2 *dA, int64_t lda,
+                                           const mw_float64x2 *dB, int64_t ldb,
+                                           const mw_float64x2 *beta, mw_float64x2 *dC,
+                                           int64_t ldc, int path, void *stream);
 MWGEMM_API int mwgemm_wgemm_device(char transa, char transb, int64_t m, int64_t n, int64_t k,
                         const mw_complex64x2 *alpha, const mw_complex64x2 *dA, int64_t lda,
                         const mw_complex64x2 *dB, int64_t ldb, const mw_complex64x2 *beta,
