@@ -193,6 +193,66 @@ def test_k_chunk_flush_period_is_128():
     assert not np.array_equal(split(64), full)
 
 
+@pytest.mark.parametrize("ta", ["N", "C"])
+def test_complex_integer_operands_give_the_exact_integer_product(ta):
+    """Integer real and imaginary parts: every complex DD product and sum is exact, so C is
+    the exact Gaussian-integer result of alpha op(A) B + beta C0 (a dropped cross term, a
+    wrong sign in the imaginary part or a missing conjugation fails)."""
+    rng = np.random.default_rng(15)
+    prev = oracle.set_order(oracle.ORDER_SIMD)
+    try:
+        for (m, n, k) in [(13, 9, 300), (32, 33, 257), (5, 7, 1)]:
+            ar, ac = (m, k) if ta == "N" else (k, m)
+            A = ddgen.cplx_matrix(rng, ar, ac, kind="integer", bound=64)
+            B = ddgen.cplx_matrix(rng, k, n, kind="integer", bound=64)
+            C0 = ddgen.cplx_matrix(rng, m, n, kind="integer", bound=1000)
+            alpha, beta = np.array([2.0, 0.0, -1.0, 0.0]), np.array([-3.0, 0.0, 1.0, 0.0])
+            C = C0.copy()
+            oracle.port_wgemm(ta, "N", m, n, k, alpha, A, ar, B, k, beta, C, m)
+            za = (A[..., 0] + 1j * A[..., 2]).T.astype(np.complex128)      # (ar, ac)
+            if ta == "C":
+                za = za.conj().T
+            zb = (B[..., 0] + 1j * B[..., 2]).T.astype(np.complex128)      # (k, n)
+            zc = (C0[..., 0] + 1j * C0[..., 2]).T.astype(np.complex128)
+            exact = (2 - 1j) * (za @ zb) + (-3 + 1j) * zc                    # exact below 2^53
+            assert np.array_equal(C[..., 0].T, exact.real)
+            assert np.array_equal(C[..., 2].T, exact.imag)
+            assert np.all(C[..., 1] == 0.0) and np.all(C[..., 3] == 0.0)
+    finally:
+        oracle.set_order(prev)
+
+
+def test_complex_k_chunk_flush_period_is_128():
+    """The complex AVX2 kernel flushes its accumulators into C every 128 of depth from
+    k = 0, as the real one does (wgemm_serial.cpp:207-234): a split of K at 128 or 256 into
+    two calls (beta = 1 on the second) is bitwise one call, a split at 64 is not."""
+    rng = np.random.default_rng(16)
+    m, n, k = 13, 9, 300
+    A = ddgen.cplx_matrix(rng, m, k)
+    B = ddgen.cplx_matrix(rng, k, n)
+    C0 = ddgen.cplx_matrix(rng, m, n)
+    al, be = ddgen.cplx_scalar(rng), ddgen.cplx_scalar(rng)
+    one = np.array([1.0, 0.0, 0.0, 0.0])
+    prev = oracle.set_order(oracle.ORDER_SIMD)
+    try:
+        full = C0.copy()
+        oracle.port_wgemm("N", "N", m, n, k, al, A, m, B, k, be, full, m)
+
+        def split(at):
+            C = C0.copy()
+            oracle.port_wgemm("N", "N", m, n, at, al, A, m, B, k, be, C, m)
+            A2 = np.ascontiguousarray(A[at:])
+            B2 = np.ascontiguousarray(B[:, at:, :])
+            oracle.port_wgemm("N", "N", m, n, k - at, al, A2, m, B2, k - at, one, C, m)
+            return C
+
+        assert np.array_equal(split(128), full)
+        assert np.array_equal(split(256), full)
+        assert not np.array_equal(split(64), full)
+    finally:
+        oracle.set_order(prev)
+
+
 def test_driver_special_cases():
     rng = np.random.default_rng(9)
     m, n, k = 9, 7, 5
