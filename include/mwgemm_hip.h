@@ -139,7 +139,8 @@ MWGEMM_API void wgemmtr_serial(char uplo, char transa, char transb, ptrdiff_t n,
  * (macro common/epblas_facade.h:132-138; numerics mgemv.cpp:60-275).  Bit-for-bit the
  * reference's result; for op 'N' that includes its OpenMP row bands (the rows after the last
  * whole 4-block of each band use faithful operations), reproduced for the team size
- * MWGEMV_THREADS, else OMP_NUM_THREADS, else the hardware thread count (m < 64: one band).
+ * MWGEMV_THREADS, else the OpenMP default team (the first entry of OMP_NUM_THREADS, else the
+ * CPUs of the process affinity mask, capped by OMP_THREAD_LIMIT; m < 64: one band).
  * m == 0 or n == 0 leaves y untouched.  Zero increments and short lda are rejected. */
 MWGEMM_API void mgemv_(const char *trans, const int *m, const int *n, const mw_float64x2 *alpha,
             const mw_float64x2 *a, const int *lda, const mw_float64x2 *x, const int *incx,
