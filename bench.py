@@ -8,9 +8,14 @@ Reported value = effective DD-GFLOP/s = 2*M*N*K / t.  The plain FP64 TwoProd/Two
 runs on the same inputs for the ratio.  The roofline entry is the dominant kernel (the INT8
 residue GEMM) timed with HIP events on the library's stream; the CPU baseline is the
 oracle's restatement of the reference OpenMP mgemm_ timed on this host on a bounded row
-sample of the same problem.
+sample of the same problem (minimum of 3 runs after a warm-up) and on the whole 512^3
+workload (minimum of 40 runs after a warm-up).  The same leg, outside every timed region,
+checks the normwise error of every GPU result against the binary128 oracle
+("err_vs_binary128": the headline's both paths and the complex leg on sampled entries, the
+512^3 workload on all entries, a sample of the batched problems).
 
-Further objects on the same line (each with its own workload): "library_int8_gemm" (the
+Further objects on the same line (each with its own workload): "gemm_512" (the 512^3
+workload on both GPU paths), "library_int8_gemm" (the
 vendor INT8 GEMM at one residue GEMM's shape, a reference point for the roofline),
 "batched" (2000 small GEMMs in one captured launch per path), "complex" (4096^3 with
 op(A)='C', both paths), "gemmtr" (the triangular-output GEMM, 4096 x 4096 upper triangle,
@@ -50,41 +55,153 @@ def gen_wide(n_rows, n_cols, gen, device):
     return torch.stack([hi, lo], dim=-1).contiguous()
 
 
-def cpu_baseline(A, B, n, k, target_s=20.0):
-    """The oracle's CPU restatement of the reference mgemm_ (AVX2 + OpenMP jc split) on the
-    first rows of the same problem.  Returns the bench JSON object."""
+ERR_BAR = 1e-30
+ERR_METRIC = ("max|C - C_binary128| / max(max|C_binary128|, |alpha| K max|A| max|B| + "
+              "|beta| max|C0|) (the reference fuzz metric, fuzz_util.fypp:252-284)")
+
+
+def cpu_baseline_leg(ah, bh, n, k, gemm512, checks, target_s=4.0, reps=3, reps_512=40):
+    """The leg that uses the oracle, outside every timed GPU region.
+
+    CPU baseline: the oracle's CPU restatement of the reference OpenMP mgemm_ (AVX2 kernel,
+    jc split over 256-column panels), timed on this host (minimum over repetitions after a
+    warm-up, BASELINE protocol): on the first rows of the 8192^3 problem (same inputs), and
+    on the whole 512^3 workload.  Checker: the normwise error of every GPU result collected
+    in ``checks`` against the binary128 oracle.  Returns (cpu_baseline, err_vs_binary128)."""
     import numpy as np
     import oracle
     L = oracle.lib()
     threads = L.port_max_threads()
     alpha = np.array([1.0, 0.0])
     beta = np.zeros(2)
-    Bh = B.cpu().numpy()
 
     def run(rows):
-        Ah = A[:, :rows, :].contiguous().cpu().numpy()
+        Ah = np.ascontiguousarray(ah[:, :rows, :])
         C = np.zeros((n, rows, 2))
         t0 = time.perf_counter()
-        oracle.port_mgemm("N", "N", rows, n, k, alpha, Ah, rows, Bh, k, beta, C, rows)
+        oracle.port_mgemm("N", "N", rows, n, k, alpha, Ah, rows, bh, k, beta, C, rows)
         return time.perf_counter() - t0
 
     probe_rows = 64
-    t_probe = run(probe_rows)
-    rows = int(min(A.shape[1], max(64, probe_rows * target_s / max(t_probe, 1e-3))))
+    t_probe = run(probe_rows)                     # also the warm-up
+    rows = int(min(ah.shape[1], max(64, probe_rows * target_s / max(t_probe, 1e-3))))
     rows = max(64, rows // 64 * 64)
-    t = run(rows)
+    t = min(run(rows) for _ in range(reps))
     gflops = 2.0 * rows * n * k / t / 1e9
     busy = min(threads, (n + 255) // 256)   # the reference splits N into 256-wide panels
-    return {"value": round(gflops, 3), "unit": "DD-GFLOP/s", "cores": busy, "kind": "port",
-            "sample": f"rows 0..{rows} of the {A.shape[1]}x{n}x{k} NN problem (same inputs), "
-                      f"{t:.1f} s after a 64-row warm-up, OMP threads={threads} (jc split keeps "
-                      f"{busy} busy), AVX2={bool(L.port_has_avx2())}"}
+    base = {"value": round(gflops, 3), "unit": "DD-GFLOP/s", "cores": busy, "kind": "port",
+            "sample": f"rows 0..{rows} of the {ah.shape[1]}x{n}x{k} NN problem (same inputs): "
+                      f"minimum of {reps} runs of {t:.2f} s after a {probe_rows}-row warm-up; "
+                      f"OMP threads={threads} (jc split keeps {busy} busy), "
+                      f"AVX2={bool(L.port_has_avx2())}"}
+    if gemm512 is not None:
+        a5, b5, m5 = gemm512["A"], gemm512["B"], gemm512["m"]
+        c5 = np.zeros((m5, m5, 2))
+        oracle.port_mgemm("N", "N", m5, m5, m5, alpha, a5, m5, b5, m5, beta, c5, m5)
+        times = []
+        for _ in range(reps_512):
+            t0 = time.perf_counter()
+            oracle.port_mgemm("N", "N", m5, m5, m5, alpha, a5, m5, b5, m5, beta, c5, m5)
+            times.append(time.perf_counter() - t0)
+        t5 = min(times)
+        busy5 = min(threads, (m5 + 255) // 256)
+        base["gemm_512"] = {
+            "workload": f"real DD GEMM {m5}^3 NN, alpha=(1,0), beta=(0,0), uniform entries, "
+                        f"whole problem",
+            "value": round(2.0 * m5 ** 3 / t5 / 1e9, 3), "unit": "DD-GFLOP/s",
+            "ms_per_call": round(t5 * 1e3, 3), "cores": busy5,
+            "protocol": f"one warm-up call, then the minimum of {reps_512} calls "
+                        f"(CLOCK_MONOTONIC wall time); OMP threads={threads}, the jc split "
+                        f"keeps {busy5} busy",
+            "bitwise_equal_to_gpu_plain": bool(np.array_equal(c5, gemm512["C_plain"]))}
+    errs = {"bar": ERR_BAR, "metric": ERR_METRIC}
+    ok = True
+    for name, c in checks.items():
+        r = oracle.q128_gemm_err(c["cplx"], c["ta"], c["tb"], c["m"], c["n"], c["k"],
+                                 c["alpha"], c["A"], c["lda"], c["B"], c["ldb"], c["beta"],
+                                 c["C0"], c["ldc"], c["C"], c["ldc"], entries=c.get("entries"))
+        ent = c.get("entries")
+        e = {"err": float(r["err"]),
+             "entries": "all" if ent is None else f"{len(ent[0])} sampled",
+             "ref_tol": max(10, 4 * c["k"]) * 2.0 ** -104}
+        if "note" in c:
+            e["note"] = c["note"]
+        errs[name] = e
+        ok = ok and e["err"] <= ERR_BAR
+    for name, group in _groups(checks).items():
+        errs[name] = {"err": max(errs[g]["err"] for g in group), "problems": len(group),
+                      "entries": errs[group[0]]["entries"] + " per problem"}
+        for g in group:
+            del errs[g]
+    errs["all_within_bar"] = bool(ok)
+    return base, errs
 
 
-def batched_bench(dev, reps=5, count=2000, seed=2000):
+def _groups(checks):
+    """Checks named 'group/i' are reported as one entry per group (the worst error)."""
+    out = {}
+    for name in checks:
+        if "/" in name:
+            out.setdefault(name.split("/")[0], []).append(name)
+    return out
+
+
+def _check(cplx, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C0, ldc, C, entries=None,
+           note=None):
+    import numpy as np
+    c = {"cplx": cplx, "ta": ta, "tb": tb, "m": m, "n": n, "k": k,
+         "alpha": np.asarray(alpha, dtype=np.float64), "A": A, "lda": lda, "B": B, "ldb": ldb,
+         "beta": np.asarray(beta, dtype=np.float64), "C0": C0, "ldc": ldc, "C": C,
+         "entries": entries}
+    if note:
+        c["note"] = note
+    return c
+
+
+def gemm_512_bench(dev, reps=20, seed=512):
+    """The reference's CPU-sized workload on the GPU: real DD GEMM 512^3 NN, alpha=(1,0),
+    beta=(0,0), uniform entries (the same seeded operands as the GPU test of it), both paths,
+    device-resident; the best of ``reps`` timed calls.  Returns (bench object, host data for
+    the CPU leg and the checker)."""
+    import numpy as np
+    import torch
+    import ddgen
+    import mwblas
+    m = 512
+    rng = np.random.default_rng(seed)
+    A = ddgen.real_matrix(rng, m, m)
+    B = ddgen.real_matrix(rng, m, m)
+    alpha, beta = np.array([1.0, 0.0]), np.zeros(2)
+    dA, dB = torch.from_numpy(A).to(dev), torch.from_numpy(B).to(dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    out = {"workload": f"real DD GEMM {m}^3 NN, alpha=(1,0), beta=(0,0), uniform entries, "
+                       f"device-resident", "dd_flops": 2.0 * m ** 3}
+    host = {"A": A, "B": B, "m": m}
+    for name, path in (("plain", mwblas.PATH_PLAIN), ("ozaki", mwblas.PATH_OZAKI)):
+        dC = torch.zeros((m, m, 2), dtype=torch.float64, device=dev)
+        mwblas.gemm_device(False, "N", "N", m, m, m, alpha, dA, m, dB, m, beta, dC, m,
+                           path=path, stream=stream)
+        torch.cuda.synchronize()
+        best = None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            mwblas.gemm_device(False, "N", "N", m, m, m, alpha, dA, m, dB, m, beta, dC, m,
+                               path=path, stream=stream)
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+            best = t if best is None else min(best, t)
+        out[name] = {"value": round(2.0 * m ** 3 / best / 1e9, 2), "unit": "DD-GFLOP/s",
+                     "ms_per_call": round(best * 1e3, 4)}
+        host["C_" + name] = dC.cpu().numpy()
+    return out, host
+
+
+def batched_bench(dev, reps=5, count=2000, seed=2000, checks=None, check_every=125):
     """The batched workload: count real DD GEMMs, M,N,K log-uniform integers in [16,512],
     NN, alpha=beta=1, uniform entries with non-zero low limbs; one captured plan per path,
-    replayed reps times.  Effective DD-GFLOP/s = sum(2MNK) / t."""
+    replayed reps times.  Effective DD-GFLOP/s = sum(2MNK) / t.  For the checker, every
+    check_every-th problem is restored and the plan run once more outside the timing; those
+    problems' results go into ``checks`` (64 sampled entries each)."""
     import numpy as np
     import torch
     import mwblas
@@ -106,6 +223,9 @@ def batched_bench(dev, reps=5, count=2000, seed=2000):
     C = [dd(int(m), int(n)) for m, n in zip(ms, ns)]
     one = np.tile([1.0, 0.0], (count, 1))
     flops = float(np.sum(2.0 * ms * ns * ks))
+    sample = list(range(0, count, check_every)) if checks is not None else []
+    C0s = {i: C[i].clone() for i in sample}
+    crng = np.random.default_rng(seed + 1)
     out = {"workload": f"{count} real DD GEMMs, M,N,K log-uniform in [16,512], NN, alpha=beta=1",
            "dd_flops": flops}
     stream = torch.cuda.current_stream().cuda_stream
@@ -122,6 +242,18 @@ def batched_bench(dev, reps=5, count=2000, seed=2000):
         out[name] = {"value": round(flops / t / 1e9, 2), "unit": "DD-GFLOP/s",
                      "ms_per_launch": round(t * 1e3, 3), "tiles": info["tiles"],
                      "moduli": info["moduli"], "launch": "one hipGraph replay"}
+        if sample:
+            for i in sample:
+                C[i].copy_(C0s[i])
+            plan.run(stream)
+            torch.cuda.synchronize()
+            for i in sample:
+                m, n, k = int(ms[i]), int(ns[i]), int(ks[i])
+                ent = (crng.integers(0, m, 64), crng.integers(0, n, 64))
+                checks[f"batched_{name}/{i}"] = _check(
+                    False, "N", "N", m, n, k, [1.0, 0.0], A[i].cpu().numpy(), m,
+                    B[i].cpu().numpy(), k, [1.0, 0.0], C0s[i].cpu().numpy(), m,
+                    C[i].cpu().numpy(), entries=ent)
         plan.close()
     return out
 
@@ -148,9 +280,10 @@ def random_scalar(rng, parts):
     return out
 
 
-def complex_bench(dev, size=4096, oz_reps=5, plain_reps=1, seed=4096):
+def complex_bench(dev, size=4096, oz_reps=5, plain_reps=1, seed=4096, checks=None):
     """Complex DD GEMM C := alpha*A^H*B + beta*C at size^3 (op(A)='C', op(B)='N'), random
-    complex alpha and beta, inputs in HBM, on both paths.  Effective rate = 8MNK / t."""
+    complex alpha and beta, inputs in HBM, on both paths.  Effective rate = 8MNK / t.  Both
+    results go into ``checks`` (128 sampled entries) for the checker."""
     import numpy as np
     import torch
     import mwblas
@@ -209,6 +342,14 @@ def complex_bench(dev, size=4096, oz_reps=5, plain_reps=1, seed=4096):
         diff = (d[..., 0::2] + d[..., 1::2]).abs().max().item()
         out["ozaki_vs_plain_normwise"] = diff / (anorm * 2 * k * amax * bmax + bnorm * cmax)
         out["speedup_vs_plain"] = round(out["ozaki"]["value"] / out["plain"]["value"], 2)
+    if checks is not None:
+        crng = np.random.default_rng(seed + 1)
+        ent = (crng.integers(0, m, 128), crng.integers(0, n, 128))
+        Ah, Bh, C0h = A.cpu().numpy(), B.cpu().numpy(), C0.cpu().numpy()
+        for name, C in res.items():
+            checks[f"complex_{name}_{size}"] = _check(True, "C", "N", m, n, k, alpha, Ah, k, Bh,
+                                                      k, beta, C0h, m, C.cpu().numpy(),
+                                                      entries=ent)
     return out
 
 
@@ -543,6 +684,19 @@ def main():
         diff = ((C[..., 0] - Cp[..., 0]) + (C[..., 1] - Cp[..., 1])).abs().max().item()
         agree = diff / (k * amax * bmax)
 
+    # results for the checker (outside every timed region): 256 sampled entries each
+    checks = {}
+    if not args.no_cpu:
+        import numpy as np
+        ent = (np.random.default_rng(8193).integers(0, m, 256),
+               np.random.default_rng(8194).integers(0, n, 256))
+        ah, bh = A.cpu().numpy(), B.cpu().numpy()
+        zero = np.zeros((n, m, 2))          # C0 (beta = 0)
+        for name, out in (("ozaki", C), ("plain", None if args.skip_plain else Cp)):
+            if out is not None:
+                checks[f"{name}_{m}"] = _check(False, "N", "N", m, n, k, alpha, ah, m, bh, k,
+                                               beta, zero, m, out.cpu().numpy(), entries=ent)
+
     flops = 2.0 * m * n * k
     oz_gflops = flops / t_oz / 1e9
     plain_gflops = flops / t_plain / 1e9 if t_plain else None
@@ -597,10 +751,17 @@ def main():
         result["library_int8_gemm"] = library_int8_bench(dev, n)
     except (RuntimeError, AttributeError) as e:   # reference point only
         result["library_int8_gemm"] = {"error": str(e)[:200]}
+    gemm512 = None
+    result["gemm_512"], gemm512 = gemm_512_bench(dev)
+    if not args.no_cpu:
+        for name in ("plain", "ozaki"):
+            checks[f"{name}_512"] = _check(False, "N", "N", 512, 512, 512, [1.0, 0.0],
+                                           gemm512["A"], 512, gemm512["B"], 512, [0.0, 0.0],
+                                           np.zeros((512, 512, 2)), 512, gemm512["C_" + name])
     if not args.no_batched:
-        result["batched"] = batched_bench(dev)
+        result["batched"] = batched_bench(dev, checks=None if args.no_cpu else checks)
     if not args.no_complex:
-        result["complex"] = complex_bench(dev)
+        result["complex"] = complex_bench(dev, checks=None if args.no_cpu else checks)
     if not args.no_gemmtr:
         result["gemmtr"] = gemmtr_bench(dev)
         result["plain_orders"] = plain_orders_bench(dev)
@@ -610,7 +771,8 @@ def main():
         result["host_boundary"] = host_boundary_bench()
         result["serial_trailing"] = serial_trailing_bench()
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(A, B, n, k)
+        result["cpu_baseline"], result["err_vs_binary128"] = cpu_baseline_leg(
+            ah, bh, n, k, gemm512, checks)
     print(json.dumps(result))
 
 
