@@ -228,8 +228,16 @@ def batched_bench(dev, reps=5, count=2000, seed=2000, checks=None, check_every=1
     crng = np.random.default_rng(seed + 1)
     out = {"workload": f"{count} real DD GEMMs, M,N,K log-uniform in [16,512], NN, alpha=beta=1",
            "dd_flops": flops}
+    # the library's batched autotuner times plain vs exact product (128 / 64 residue tiles)
+    # for this batch class; the exact-product plan then takes the tuned residue tile and the
+    # AUTO plan the tuned path
+    choice, tuned_ms = mwblas.batched_autotune("N", "N", ms, ns, ks)
+    out["autotune"] = {"path": "ozaki" if choice // 16 else "plain",
+                       "residue_tile": 64 if choice % 16 == 1 else 128,
+                       "best_ms": round(tuned_ms, 3)}
     stream = torch.cuda.current_stream().cuda_stream
-    for name, path in (("plain", mwblas.PATH_PLAIN), ("ozaki", mwblas.PATH_OZAKI)):
+    for name, path in (("plain", mwblas.PATH_PLAIN), ("ozaki", mwblas.PATH_OZAKI),
+                       ("auto", mwblas.PATH_AUTO)):
         plan = mwblas.BatchedPlan("N", "N", ms, ns, ks, one, A, ms, B, ks, one, C, ms, path=path)
         plan.run(stream)
         torch.cuda.synchronize()
@@ -242,7 +250,9 @@ def batched_bench(dev, reps=5, count=2000, seed=2000, checks=None, check_every=1
         out[name] = {"value": round(flops / t / 1e9, 2), "unit": "DD-GFLOP/s",
                      "ms_per_launch": round(t * 1e3, 3), "tiles": info["tiles"],
                      "moduli": info["moduli"], "launch": "one hipGraph replay"}
-        if sample:
+        if name == "auto":
+            out[name]["path"] = "ozaki" if info["path"] == mwblas.PATH_OZAKI else "plain"
+        if sample and name != "auto":
             for i in sample:
                 C[i].copy_(C0s[i])
             plan.run(stream)

@@ -218,26 +218,49 @@ void k_batch_split(const BatchProb *__restrict__ probs, UnitSpace us, int bits,
                            rows * p.Kc, r, kg);
 }
 
+// Work queues of the residue GEMMs, one per XCD.  Each XCD has its own L2, and workgroups
+// are dispatched round-robin over the XCDs, so workgroup b of the persistent grid runs on
+// XCD b % XCD_QUEUES.  The plan assigns whole problems to queues (balanced by cost), so the
+// tiles of one problem run on one XCD at about the same time and read its residue panels
+// from one L2 instead of fetching them once per XCD.  A workgroup takes tiles from its own
+// XCD's queue, then from the others in turn (so no queue is left behind), and returns once
+// every queue is drained.  counters[q] is queue q's cursor (reset before every run);
+// qstart[q] .. qstart[q + 1] are its entries of the tile list.
+constexpr int XCD_QUEUES = 8;
+
+DDI int next_queued_tile(int *__restrict__ counters, const int *__restrict__ qstart, int &visited)
+{
+    const int home = (int)(blockIdx.x % XCD_QUEUES);
+    while (visited < XCD_QUEUES) {
+        const int q = (home + visited) % XCD_QUEUES;
+        const int i = atomicAdd(&counters[q], 1);
+        if (i < qstart[q + 1] - qstart[q]) return qstart[q] + i;
+        ++visited;
+    }
+    return -1;
+}
+
 // Residue GEMMs: persistent queue over (problem, first modulus, tm, tn) tiles of 64*T x 64*T,
 // each running the products of `group` consecutive moduli as one slab pipeline.
 template <int T>
 __global__ __launch_bounds__(256, 2)
 void k_batch_oz_gemm(const BatchProb *__restrict__ probs, const int4 *__restrict__ tiles,
-                     int ntiles, int *__restrict__ counter, const ModConst *__restrict__ mcs,
-                     int nmod, int group)
+                     const int *__restrict__ qstart, int *__restrict__ counters,
+                     const ModConst *__restrict__ mcs, int nmod, int group)
 {
     constexpr int BT = 64 * T;
     __shared__ __attribute__((aligned(16))) int8_t sa[2][BT * 80];
     __shared__ __attribute__((aligned(16))) int8_t sb[2][BT * 80];
     __shared__ int s_tile;
+    int visited = 0;
     for (;;) {
-        if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
+        if (threadIdx.x == 0) s_tile = next_queued_tile(counters, qstart, visited);
         __syncthreads();
         // the same for the whole workgroup: the tile record and its problem load into
         // scalar registers
         const int t = __builtin_amdgcn_readfirstlane(s_tile);
         __syncthreads();
-        if (t >= ntiles) return;
+        if (t < 0) return;
         const int4 tr = tiles[t];      // (problem, first modulus, tm, tn)
         const BatchProb &p = probs[tr.x];
         const int64_t i0 = (int64_t)tr.z * BT, j0 = (int64_t)tr.w * BT;
@@ -254,17 +277,18 @@ void k_batch_oz_gemm(const BatchProb *__restrict__ probs, const int4 *__restrict
 // memory at every slab, the ring keeps three slabs in flight across modulus boundaries.
 __global__ __launch_bounds__(256, 2)
 void k_batch_oz_ring(const BatchProb *__restrict__ probs, const int4 *__restrict__ tiles,
-                     int ntiles, int *__restrict__ counter, const ModConst *__restrict__ mcs,
-                     int nmod, int group)
+                     const int *__restrict__ qstart, int *__restrict__ counters,
+                     const ModConst *__restrict__ mcs, int nmod, int group)
 {
     __shared__ __attribute__((aligned(1024))) int8_t smem[RING_NST * ring_slot<128>()];
     __shared__ int s_tile;
+    int visited = 0;
     for (;;) {
-        if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
+        if (threadIdx.x == 0) s_tile = next_queued_tile(counters, qstart, visited);
         __syncthreads();
         const int t = __builtin_amdgcn_readfirstlane(s_tile);
         __syncthreads();
-        if (t >= ntiles) return;
+        if (t < 0) return;
         const int4 tr = tiles[t];      // (problem, first modulus, tm, tn)
         const BatchProb &p = probs[tr.x];
         const int64_t i0 = (int64_t)tr.z * 128, j0 = (int64_t)tr.w * 128;
@@ -336,7 +360,8 @@ struct BatchPlan {
     BatchProb *probs = nullptr;
     int4 *tiles = nullptr;
     int ntiles = 0;
-    int *counters = nullptr;            // [0]: plain / gemm queue
+    int *counters = nullptr;            // [0]: plain queue; [0, 8): residue-GEMM queues
+    int *qstart = nullptr;              // residue-GEMM queue q: tiles qstart[q] .. qstart[q+1]
     int64_t *us_ra = nullptr, *us_rb = nullptr, *us_sa = nullptr, *us_sb = nullptr, *us_crt = nullptr;
     int64_t n_ra = 0, n_rb = 0, n_sa = 0, n_sb = 0, n_crt = 0;
     unsigned long long *rmax_base = nullptr;
@@ -388,7 +413,7 @@ static int batch_enqueue(BatchPlan *P, hipStream_t s)
         ScopedTimer tm(KC_BATCH_OZ_SPLIT, s);
         const int64_t nfill = P->rmax_count > 1 ? P->rmax_count : 1;
         hipLaunchKernelGGL(k_batch_fill, dim3(blocks_for(nfill)), dim3(256), 0, s, P->rmax_base,
-                           P->rmax_count, P->counters, 1);
+                           P->rmax_count, P->counters, XCD_QUEUES);
         B_TRY(hipGetLastError());
         UnitSpace ra{P->us_ra, P->count}, rb{P->us_rb, P->count};
         UnitSpace sa{P->us_sa, P->count}, sb{P->us_sb, P->count};
@@ -417,10 +442,12 @@ static int batch_enqueue(BatchPlan *P, hipStream_t s)
         ScopedTimer tm(KC_BATCH_OZ_GEMM, s);
         if (P->oz_tile == 128)
             hipLaunchKernelGGL(k_batch_oz_ring, dim3((unsigned)P->grid_persist), dim3(256), 0, s,
-                               P->probs, P->tiles, P->ntiles, P->counters, P->mcs, P->oz.nmod, P->mod_group);
+                               P->probs, P->tiles, P->qstart, P->counters, P->mcs, P->oz.nmod,
+                               P->mod_group);
         else
             hipLaunchKernelGGL((k_batch_oz_gemm<1>), dim3((unsigned)P->grid_persist), dim3(256), 0, s,
-                               P->probs, P->tiles, P->ntiles, P->counters, P->mcs, P->oz.nmod, P->mod_group);
+                               P->probs, P->tiles, P->qstart, P->counters, P->mcs, P->oz.nmod,
+                               P->mod_group);
         B_TRY(hipGetLastError());
     }
     if (P->n_crt) {
@@ -584,13 +611,35 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
             tl.swap(split);
         }
     }
-    // Longest jobs first: plain tiles cost ~K, residue tiles ~Kc.
+    // Longest jobs first: plain tiles cost ~K.  Residue tiles go to one queue per XCD: whole
+    // problems, the costliest first, each to the queue with the least work so far (tiles
+    // x Kc), its tiles together.
+    std::vector<int> qstart(XCD_QUEUES + 1, 0);
     if (!oz) {
         auto cost = [&](const int4 &t) { return host[t.x].active ? host[t.x].k * (t.w == 1 ? 1 : 2) : 0; };
         std::stable_sort(tl.begin(), tl.end(), [&](const int4 &a, const int4 &b) { return cost(a) > cost(b); });
+    } else if (!tl.empty()) {
+        std::vector<std::vector<int4>> per(count);
+        for (const int4 &t : tl) per[t.x].push_back(t);
+        std::vector<int> order;
+        for (int64_t p = 0; p < count; ++p)
+            if (!per[p].empty()) order.push_back((int)p);
+        auto pcost = [&](int p) { return (double)per[p].size() * (double)host[p].Kc; };
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return pcost(a) > pcost(b); });
+        std::vector<std::vector<int4>> queue(XCD_QUEUES);
+        std::vector<double> load(XCD_QUEUES, 0.0);
+        for (int p : order) {
+            const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            load[q] += pcost(p);
+            queue[q].insert(queue[q].end(), per[p].begin(), per[p].end());
+        }
+        tl.clear();
+        for (int q = 0; q < XCD_QUEUES; ++q) {
+            qstart[q] = (int)tl.size();
+            tl.insert(tl.end(), queue[q].begin(), queue[q].end());
+        }
     }
-    else
-        std::stable_sort(tl.begin(), tl.end(), [&](const int4 &a, const int4 &b) { return host[a.x].Kc > host[b.x].Kc; });
+    qstart[XCD_QUEUES] = (int)tl.size();
     for (int64_t p = 0; p < count; ++p) {
         ra[p + 1] += ra[p]; rb[p + 1] += rb[p]; sa[p + 1] += sa[p]; sb[p + 1] += sb[p]; cr[p + 1] += cr[p];
     }
@@ -606,6 +655,7 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
     const size_t o_probs = take(sizeof(BatchProb) * std::max<int64_t>(count, 1));
     const size_t o_tiles = take(sizeof(int4) * std::max<size_t>(tl.size(), 1));
     const size_t o_cnt = take(sizeof(int) * 64);
+    const size_t o_q = take(sizeof(int) * (XCD_QUEUES + 1));
     const size_t pre = sizeof(int64_t) * (count + 1);
     const size_t o_ra = take(pre), o_rb = take(pre), o_sa = take(pre), o_sb = take(pre), o_cr = take(pre);
     const size_t o_rmax = take(sizeof(unsigned long long) * std::max<int64_t>(rmax_n, 1));
@@ -627,6 +677,7 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
     P->probs = reinterpret_cast<BatchProb *>(base + o_probs);
     P->tiles = reinterpret_cast<int4 *>(base + o_tiles);
     P->counters = reinterpret_cast<int *>(base + o_cnt);
+    P->qstart = reinterpret_cast<int *>(base + o_q);
     P->us_ra = reinterpret_cast<int64_t *>(base + o_ra);
     P->us_rb = reinterpret_cast<int64_t *>(base + o_rb);
     P->us_sa = reinterpret_cast<int64_t *>(base + o_sa);
@@ -650,7 +701,8 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
         hipMemcpy(P->us_rb, rb.data(), pre, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(P->us_sa, sa.data(), pre, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(P->us_sb, sb.data(), pre, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(P->us_crt, cr.data(), pre, hipMemcpyHostToDevice) != hipSuccess)
+        hipMemcpy(P->us_crt, cr.data(), pre, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(P->qstart, qstart.data(), sizeof(int) * (XCD_QUEUES + 1), hipMemcpyHostToDevice) != hipSuccess)
     {
         (void)hipGetLastError();
         rc = record_error(-1, "batched mgemm: plan upload failed");
