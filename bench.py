@@ -233,7 +233,8 @@ def batched_bench(dev, reps=5, count=2000, seed=2000, checks=None, check_every=1
     # AUTO plan the tuned path
     choice, tuned_ms = mwblas.batched_autotune("N", "N", ms, ns, ks)
     out["autotune"] = {"path": "ozaki" if choice // 16 else "plain",
-                       "residue_tile": 64 if choice % 16 == 1 else 128,
+                       "residue_tile": ("128 x 128", "64 x 64", "64 x 64, reconstruction fused")[
+                           choice % 16] if choice // 16 else None,
                        "best_ms": round(tuned_ms, 3)}
     stream = torch.cuda.current_stream().cuda_stream
     for name, path in (("plain", mwblas.PATH_PLAIN), ("ozaki", mwblas.PATH_OZAKI),

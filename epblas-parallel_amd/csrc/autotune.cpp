@@ -240,7 +240,8 @@ int mwgemm_batched_autotune(int64_t count, char transa, char transb, const int64
         B[p] = reinterpret_cast<const mw_float64x2 *>(buf + ob[p]);
         C[p] = reinterpret_cast<mw_float64x2 *>(buf + oc[p]);
     }
-    // candidates: plain; Ozaki with 128 / 64 residue tiles (when the depth allows it)
+    // candidates: plain; Ozaki with 128 / 64 residue tiles, or 64 with the reconstruction
+    // fused (when the depth allows it)
     std::vector<int> cand = {0};
     if (kmax > 0 && ozaki_supported(kmax))
         for (int v = 0; v < BATCH_OZ_VARIANTS; ++v) cand.push_back(16 + v);

@@ -218,19 +218,26 @@ void k_batch_split(const BatchProb *__restrict__ probs, UnitSpace us, int bits,
                            rows * p.Kc, r, kg);
 }
 
-// Work queues of the residue GEMMs, one per XCD.  Each XCD has its own L2, and workgroups
-// are dispatched round-robin over the XCDs, so workgroup b of the persistent grid runs on
-// XCD b % XCD_QUEUES.  The plan assigns whole problems to queues (balanced by cost), so the
-// tiles of one problem run on one XCD at about the same time and read its residue panels
-// from one L2 instead of fetching them once per XCD.  A workgroup takes tiles from its own
-// XCD's queue, then from the others in turn (so no queue is left behind), and returns once
-// every queue is drained.  counters[q] is queue q's cursor (reset before every run);
-// qstart[q] .. qstart[q + 1] are its entries of the tile list.
+// Work queues of the residue GEMMs, one per XCD.  Each XCD has its own L2.  The plan assigns
+// whole problems to queues (balanced by cost), so the tiles of one problem run on one XCD at
+// about the same time and read its residue panels from one L2 instead of fetching them once
+// per XCD.  A workgroup's home queue is the XCD it runs on, read from the hardware register
+// XCC_ID (any value it returns is a valid queue index, so the mapping can only cost reuse,
+// never correctness).  A workgroup takes tiles from its home queue, then from the others in
+// turn (so no queue is left behind), and returns once every queue is drained.  counters[q]
+// is queue q's cursor (reset before every run); qstart[q] .. qstart[q + 1] are its entries
+// of the tile list.
 constexpr int XCD_QUEUES = 8;
+
+// s_getreg_b32 of HW_REG_XCC_ID (hardware register 20), bits [3:0]: the XCD of this wave.
+DDI int xcd_id()
+{
+    return (int)(__builtin_amdgcn_s_getreg(20 | (0 << 6) | ((4 - 1) << 11)) & (XCD_QUEUES - 1));
+}
 
 DDI int next_queued_tile(int *__restrict__ counters, const int *__restrict__ qstart, int &visited)
 {
-    const int home = (int)(blockIdx.x % XCD_QUEUES);
+    const int home = xcd_id();
     while (visited < XCD_QUEUES) {
         const int q = (home + visited) % XCD_QUEUES;
         const int i = atomicAdd(&counters[q], 1);
@@ -304,6 +311,75 @@ void k_batch_oz_ring(const BatchProb *__restrict__ probs, const int4 *__restrict
     }
 }
 
+// Residue GEMMs and reconstruction fused: 64 x 64 output tiles through the DMA-fed ring (four
+// waves, one 32 x 32 MFMA tile each), all moduli of a tile in one stage sequence.  After each
+// modulus the wave reduces its accumulators modulo m_t and adds u_t times the weight limbs of
+// round(2^128 y_t / m_t) into four exact FP64 sums per entry (the same sums the separate
+// reconstruction kernel forms, so the results are identical); after the last modulus it
+// reconstructs its 16 entries per lane and updates C.  The reduced products never leave the
+// registers, so the U arrays (write and read-back) disappear.
+__global__ __launch_bounds__(256, 2)
+void k_batch_oz_fused(const BatchProb *__restrict__ probs, const int4 *__restrict__ tiles,
+                      const int *__restrict__ qstart, int *__restrict__ counters,
+                      const ModConst *__restrict__ mcs, int nmod, double Mhi, double Mlo,
+                      int bits)
+{
+    __shared__ __attribute__((aligned(1024))) int8_t smem[RING_NST * ring_slot<64>()];
+    __shared__ int s_tile;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wm = w & 1, wn = w >> 1;
+    int visited = 0;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = next_queued_tile(counters, qstart, visited);
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(s_tile);
+        __syncthreads();
+        if (t < 0) return;
+        const int4 tr = tiles[t];      // (problem, 0, tm, tn)
+        const BatchProb &p = probs[tr.x];
+        const int64_t i0 = (int64_t)tr.z * 64, j0 = (int64_t)tr.w * 64;
+        const int64_t sA = p.Mp * p.Kc, sB = p.Np * p.Kc;
+        const int ra = (int)(p.Mp - i0 < 64 ? p.Mp - i0 : 64);
+        const int rb = (int)(p.Np - j0 < 64 ? p.Np - j0 : 64);
+        // this lane's entries: column j0 + cl, rows i0 + rl + 8g + q for entry 4g + q
+        double sums[16][1][4];
+#pragma unroll
+        for (int e = 0; e < 16; ++e)
+#pragma unroll
+            for (int l = 0; l < 4; ++l) sums[e][0][l] = 0.0;
+        auto crt_step = [&](int mt, v16i (&acc)[1][1]) {
+            const ModConst mc = mcs[mt];
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const double u = (double)crt_residue(acc[0][0][e], mc);
+#pragma unroll
+                for (int l = 0; l < 4; ++l) sums[e][0][l] = __builtin_fma(u, mc.w[l], sums[e][0][l]);
+            }
+        };
+        oz_gemm_tile_ring<2, 2, 1, 1, true, true>(p.resA + i0 * p.Kc, p.resB + j0 * p.Kc,
+                                                  (uint8_t *)nullptr, p.Kc, p.Mp, mcs, nmod, sA,
+                                                  sB, 0, i0, j0, smem, ra, rb, crt_step);
+        const int64_t j = j0 + wn * 32 + (lane & 31);
+        const int64_t rl = i0 + wm * 32 + 4 * (lane >> 5);
+        if (j < p.n) {
+            bool badj;
+            const int tau = row_sigma(p.rB[j], bits, badj);
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int64_t i = rl + 8 * (e >> 2) + (e & 3);
+                if (i < p.m) {
+                    bool badi;
+                    const int sigma = row_sigma(p.rA[i], bits, badi);
+                    oz_crt_store(&p.C[j * p.ldc + i],
+                                 oz_crt_value(sums[e], p.alpha, Mhi, Mlo, -128 - sigma - tau,
+                                              badi || badj),
+                                 p.beta, p.beta_mode);
+                }
+            }
+        }
+    }
+}
+
 // Reconstruction: unit = (column j, 4-row group); inactive problems get the pre-pass only.
 __global__ __launch_bounds__(256)
 void k_batch_crt(const BatchProb *__restrict__ probs, UnitSpace us, int nmod,
@@ -371,6 +447,7 @@ struct BatchPlan {
     hipStream_t done_stream = nullptr;  // another stream waits for it (the runs share the
     bool done_recorded = false;         // plan's work counters and workspace)
     int oz_tile = 128;                  // residue-GEMM tile edge (128 or 64)
+    bool fused = false;                 // residue GEMMs and reconstruction in one kernel
     int mod_group = 0;                  // moduli per residue tile (all of them unless split)
     OzPlan oz{};
     ModConst *mcs = nullptr;
@@ -440,7 +517,11 @@ static int batch_enqueue(BatchPlan *P, hipStream_t s)
     }
     if (P->ntiles) {
         ScopedTimer tm(KC_BATCH_OZ_GEMM, s);
-        if (P->oz_tile == 128)
+        if (P->fused)
+            hipLaunchKernelGGL(k_batch_oz_fused, dim3((unsigned)P->grid_persist), dim3(256), 0, s,
+                               P->probs, P->tiles, P->qstart, P->counters, P->mcs, P->oz.nmod,
+                               P->oz.Mhi, P->oz.Mlo, P->oz.bits);
+        else if (P->oz_tile == 128)
             hipLaunchKernelGGL(k_batch_oz_ring, dim3((unsigned)P->grid_persist), dim3(256), 0, s,
                                P->probs, P->tiles, P->qstart, P->counters, P->mcs, P->oz.nmod,
                                P->mod_group);
@@ -535,8 +616,21 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
         cr(count + 1, 0);
     const bool oz = path == MWGEMM_PATH_OZAKI;
     const bool a_kc = P->ta != 0, b_kc = P->tb == 0;   // as in batch_enqueue
-    if (oz_variant < 0) oz_variant = tune_lookup(TK_BATCH, count, vol, kmax) % 16;
-    P->oz_tile = (oz_variant == 1) ? 64 : 128;
+    // Residue-GEMM variants: 0 = 128 x 128 ring tiles, then a reconstruction kernel (the
+    // default without a tuned choice); 1 = 64 x 64 register-staged tiles, then the
+    // reconstruction kernel; 2 = 64 x 64 ring tiles with the reconstruction fused.
+    // MWGEMM_BATCH_RESIDUE_VARIANT overrides the tuned choice.
+    if (oz_variant < 0) {
+        const char *e = getenv("MWGEMM_BATCH_RESIDUE_VARIANT");
+        const int ev = e ? atoi(e) : -1;
+        if (ev >= 0 && ev < BATCH_OZ_VARIANTS) oz_variant = ev;
+    }
+    if (oz_variant < 0) {
+        const int tv = tune_lookup(TK_BATCH, count, vol, kmax);
+        oz_variant = tv < 0 ? 0 : tv % 16;
+    }
+    P->fused = oz_variant == 2;
+    P->oz_tile = (oz_variant == 0) ? 128 : 64;
     const int64_t OT = P->oz_tile;
     if (oz && kmax > 0 && !ozaki_make_plan(kmax, ozaki_target_bits(), ozaki_forced_nmod(), &P->oz)) {
         plan_free(P);
@@ -575,17 +669,20 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
             }
             continue;
         }
-        cr[p + 1] = empty ? 0 : rup(q.n * ((q.m + 3) / 4), 64);
+        // the reconstruction kernel's units; with the fused kernel only inactive problems
+        // need it (their beta pre-pass)
+        cr[p + 1] = empty || (P->fused && q.active) ? 0 : rup(q.n * ((q.m + 3) / 4), 64);
         if (empty || !q.active) continue;
-        // Residue rows are padded to 32 for the ring's 128-row tiles (it clamps its loads and
-        // stores to the rows that exist), to 64 for the 64-row register-staged tiles.
-        const int64_t PAD = OT == 128 ? 32 : OT;
+        // Residue rows are padded to 32 for the ring tiles (they clamp their loads and stores
+        // to the rows that exist), to 64 for the 64-row register-staged tiles.
+        const int64_t PAD = (OT == 128 || P->fused) ? 32 : OT;
         q.Mp = rup(q.m, PAD); q.Np = rup(q.n, PAD); q.Kc = rup(q.k, 64);
         offR[p] = rmax_n;
         rmax_n += q.Mp + q.Np;
         offA[p] = ws; ws += rup((int64_t)nmod * q.Mp * q.Kc, 256);
         offB[p] = ws; ws += rup((int64_t)nmod * q.Np * q.Kc, 256);
-        offU[p] = ws; ws += rup((int64_t)nmod * q.Np * q.Mp, 256);
+        offU[p] = ws;
+        if (!P->fused) ws += rup((int64_t)nmod * q.Np * q.Mp, 256);
         // unit counts of the maxima and split kernels (layouts documented there), each
         // padded to whole waves so that a wave never spans two problems
         const int64_t nch = (q.k + 63) / 64, nch4 = (q.k + 255) / 256, G = q.Kc / 16;
@@ -598,7 +695,8 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
     }
     // Residue tiles run all moduli in one slab pipeline; only when that leaves fewer than
     // about four tiles per CU are the moduli split into groups (tile .y = first modulus).
-    if (oz && !tl.empty()) {
+    // The fused reconstruction needs every modulus of a tile in one workgroup.
+    if (oz && !tl.empty() && !P->fused) {
         const int64_t want = 4 * (int64_t)cus;
         int groups = (int)std::min<int64_t>(nmod, (want + (int64_t)tl.size() - 1) / (int64_t)tl.size());
         groups = std::max(groups, 1);

@@ -59,7 +59,8 @@ enum TuneKind {
     TK_PLAIN_REAL = 0,   // variant: 0 = 64x64 tile, 1 = 32x64 tile
     TK_PLAIN_CPLX = 1,   // variant: 0 = 32x32 tile
     TK_OZ_GEMM = 2,      // variant: residue tile 0 = 256x256, 1 = 128x128, 2 = 256x128, 3 = 128x256
-    TK_BATCH = 3,        // variant: path * 16 + residue-tile variant (0 = 128, 1 = 64)
+    TK_BATCH = 3,        // variant: path * 16 + residue-tile variant (0 = 128, 1 = 64,
+                         // 2 = 64 with the reconstruction fused)
     TK_COUNT
 };
 // Tuned variant for a shape class, or -1 when none has been recorded.
@@ -67,7 +68,7 @@ int tune_lookup(int kind, int64_t a, int64_t b, int64_t c);
 void tune_record(int kind, int64_t a, int64_t b, int64_t c, int variant, double ms);
 constexpr int PLAIN_REAL_VARIANTS = 2;
 constexpr int OZ_GEMM_VARIANTS = 4;
-constexpr int BATCH_OZ_VARIANTS = 2;
+constexpr int BATCH_OZ_VARIANTS = 3;
 
 // plain_gemm.hip
 hipError_t plain_gemm_real(const GemmArgsR &g, hipStream_t s);

@@ -459,13 +459,17 @@ DDI void wait_lds() { __builtin_amdgcn_s_waitcnt(15 | (7 << 4) | (0 << 8) | (3 <
 // rows_a rows of the A panel and rows_b rows of the B panel exist (multiples of 32): rows
 // beyond them are fetched as copies of the last row (valid, cached memory) and their
 // results are not stored.  With FOLD, the accumulators are reduced every OZ_FOLD_DEPTH of
-// depth (needed only when kp exceeds it).
-template <int WM, int WN, int TM, int TN, bool MULTI, bool CLAMP = false, bool FOLD = true>
+// depth (needed only when kp exceeds it).  epi: when given, it replaces the store of each
+// modulus's reduced products: epi(t, acc) receives modulus t's INT32 accumulators (the
+// wave's TM x TN MFMA tiles) and U is not written.
+struct NoRingEpilogue {};
+template <int WM, int WN, int TM, int TN, bool MULTI, bool CLAMP = false, bool FOLD = true,
+          typename Epi = NoRingEpilogue>
 DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restrict__ Bb,
                            uint8_t *__restrict__ Ub, int64_t kp, int64_t ldu,
                            const ModConst *__restrict__ mcs, int nmod, int64_t sA, int64_t sB,
                            int64_t sU, int64_t i0, int64_t j0, int8_t *smem, int rows_a = 0,
-                           int rows_b = 0)
+                           int rows_b = 0, Epi &&epi = Epi{})
 {
     constexpr int BT = 32 * TM * WM, NW = WM * WN;
     static_assert(BT == 32 * TN * WN, "square tile");
@@ -555,8 +559,13 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
         __builtin_amdgcn_s_barrier();
         asm volatile("" ::: "memory");
     };
-    // Reduce the accumulators of modulus t and store them, four rows per 32-bit word.
+    // Reduce the accumulators of modulus t and store them, four rows per 32-bit word (or
+    // hand them to the caller's epilogue).
     auto flush = [&](int t) {
+        if constexpr (!std::is_same<std::decay_t<Epi>, NoRingEpilogue>::value) {
+            epi(t, acc);
+            return;
+        }
         const ModConst mc = mcs[t];
         uint8_t *U = Ub + (int64_t)t * sU;
 #pragma unroll

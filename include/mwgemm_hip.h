@@ -286,8 +286,10 @@ MWGEMM_API int mwgemm_mgemm_batched_device(int64_t count, char transa, char tran
 /* Autotuner.  mwgemm_autotune times every tile variant of the given path (PLAIN or OZAKI)
  * on synthetic device operands of shape m x n x k (allocated and freed here) and records the
  * fastest for that shape class; mwgemm_batched_autotune does the same for a batch (plain
- * vs Ozaki with 128 or 64 residue tiles), and plans created with path AUTO then use the
- * recorded choice.  Decisions persist across processes through the file named by
+ * vs Ozaki with 128 x 128 residue tiles (variant 0), 64 x 64 tiles (1), or 64 x 64 tiles
+ * with the reconstruction fused into them (2)), and plans created with path AUTO, or OZAKI
+ * for the residue tile, then use the recorded choice (MWGEMM_BATCH_RESIDUE_VARIANT=0|1|2
+ * overrides the residue tile).  Decisions persist across processes through the file named by
  * MWGEMM_TUNE_FILE (appended on every record, read on first use).  *variant / *choice
  * receive the winner (batch: path * 16 + residue-tile variant), *best_ms its time, and
  * all_ms (optional, all_cap entries) every variant's time.

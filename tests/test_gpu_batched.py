@@ -116,14 +116,19 @@ def test_plan_runs_alternating_streams(gpu, path):
     plan.close()
 
 
+@pytest.mark.parametrize("variant", [0, 1, 2], ids=["ring128", "tile64", "fused64"])
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
-def test_ozaki_batched_vs_binary128(gpu, ta, tb):
+def test_ozaki_batched_vs_binary128(gpu, ta, tb, variant, monkeypatch):
     """Every operand layout of the residue split and maxima (row- and column-contiguous A
-    and B), problems of ragged sizes so that unit spaces are padded to whole waves."""
+    and B), problems of ragged sizes so that unit spaces are padded to whole waves, through
+    each residue-GEMM variant (128 x 128 ring tiles, 64 x 64 register-staged tiles, 64 x 64
+    ring tiles with the reconstruction fused into them)."""
     import torch
+    monkeypatch.setenv("MWGEMM_BATCH_RESIDUE_VARIANT", str(variant))
     rng = np.random.default_rng(33 + 2 * (ta == "T") + (tb == "T"))
     probs = _problems(rng, 40, 1, 300, ta, tb)
     dA, dB, dC = _device(probs, gpu)
+    print(f"batched exact product {ta}{tb}, residue variant {variant}", flush=True)
     plan = mwblas.BatchedPlan(ta, tb, *_args(probs, dA, dB, dC), path=mwblas.PATH_OZAKI)
     assert plan.info()["path"] == mwblas.PATH_OZAKI
     stream = torch.cuda.current_stream().cuda_stream
