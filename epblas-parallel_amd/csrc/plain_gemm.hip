@@ -18,6 +18,9 @@
 // The reference's scalar kernel (hosts without AVX2/FMA) is the second order, SCALAR:
 //   C := beta-prepass(C); for k ascending: C := faithful_add(C, faithful_mul(alpha*b_kj, a_ik))
 //                                             (mgemm_serial.cpp:87-99; wgemm_serial.cpp:102-115)
+#include <algorithm>
+#include <atomic>
+
 #include "plain_tile.hpp"
 #include "timer.hpp"
 
@@ -145,31 +148,50 @@ DDI void wgemmtr_diag_lane(int64_t t, int64_t n, int64_t k, int64_t nb, bool upp
 // ----------------------------------------------------------------------------------
 // Real kernel.
 // ----------------------------------------------------------------------------------
+// gemmtr (TRI): workgroups [0, rect_wgs) take the live rectangle tiles in the compact order of
+// tri_live_tile; tiles [0, split_from) run whole, and each later live tile runs as two
+// BM x BN/2 halves (two workgroups), so the last, partly filled round of resident workgroups
+// lasts half a tile time (SIMD order only).  Workgroups past rect_wgs take the diagonal
+// triangles.  nb32: tri_nb clamped to n + 1.
 template <int TM, int TN, int BK, bool TA, bool TB, int TRI = TRI_NONE, bool SCALAR = false>
 __global__ __launch_bounds__(256)
 void k_mgemm_plain(int64_t m, int64_t n, int64_t k, ddv alpha, ddv beta, int beta_mode,
                    const ddv *__restrict__ A, int64_t lda, const ddv *__restrict__ B,
-                   int64_t ldb, ddv *__restrict__ C, int64_t ldc, int tiles_m, int rect_tiles,
-                   int64_t tri_nb)
+                   int64_t ldb, ddv *__restrict__ C, int64_t ldc, int tiles_m, int rect_wgs,
+                   int64_t tri_nb, int split_from, int nb32)
 {
     constexpr int BM = 16 * TM, BN = 16 * TN;
     __shared__ __attribute__((aligned(16))) ddv As[BK][BM];
     __shared__ __attribute__((aligned(16))) ddv Bs[BK][BN];
-    if constexpr (TRI != TRI_NONE) {
-        // gemmtr: workgroups past the rectangle's tiles take the diagonal triangles.
-        if ((int)blockIdx.x >= rect_tiles) {
-            mgemmtr_diag_lane<TA, TB>((int64_t)(blockIdx.x - rect_tiles) * 256 + threadIdx.x,
-                                      n, k, tri_nb, TRI == TRI_UPPER, alpha, beta, beta_mode,
-                                      A, lda, B, ldb, C, ldc);
+    if constexpr (TRI == TRI_NONE) {
+        const int64_t i0 = (int64_t)(blockIdx.x % tiles_m) * BM;
+        const int64_t j0 = (int64_t)(blockIdx.x / tiles_m) * BN;
+        mgemm_plain_tile<TM, TN, BK, TA, TB, TRI, SCALAR>(m, n, k, alpha, beta, beta_mode, A, lda,
+                                                          B, ldb, C, ldc, i0, j0, As, Bs);
+    } else {
+        const int bid = (int)blockIdx.x;
+        if (bid >= rect_wgs) {
+            mgemmtr_diag_lane<TA, TB>((int64_t)(bid - rect_wgs) * 256 + threadIdx.x, n, k, tri_nb,
+                                      TRI == TRI_UPPER, alpha, beta, beta_mode, A, lda, B, ldb,
+                                      C, ldc);
             return;
         }
+        int64_t i0, j0;
+        if (SCALAR || TN % 2 != 0 || bid < split_from) {
+            tri_live_tile<TRI, BM, BN>(bid, (int)n, nb32, tiles_m, i0, j0);
+            mgemm_plain_tile<TM, TN, BK, TA, TB, TRI, SCALAR>(m, n, k, alpha, beta, beta_mode, A,
+                                                              lda, B, ldb, C, ldc, i0, j0, As, Bs,
+                                                              tri_nb);
+        } else if constexpr (!SCALAR && TN % 2 == 0) {
+            const int h = bid - split_from;
+            tri_live_tile<TRI, BM, BN>(split_from + h / 2, (int)n, nb32, tiles_m, i0, j0);
+            j0 += (h & 1) * (BN / 2);
+            if (j0 >= n || !tri_tile_live<TRI>(i0, j0, BM, BN / 2, n, tri_nb)) return;
+            mgemm_plain_tile<TM, TN / 2, BK, TA, TB, TRI, false>(
+                m, n, k, alpha, beta, beta_mode, A, lda, B, ldb, C, ldc, i0, j0, As,
+                *reinterpret_cast<ddv (*)[BK][BN / 2]>(&Bs[0][0]), tri_nb);
+        }
     }
-    const int64_t i0 = (int64_t)(blockIdx.x % tiles_m) * BM;
-    const int64_t j0 = (int64_t)(blockIdx.x / tiles_m) * BN;
-    if constexpr (TRI != TRI_NONE)
-        if (!tri_tile_live<TRI>(i0, j0, BM, BN, n, tri_nb)) return;
-    mgemm_plain_tile<TM, TN, BK, TA, TB, TRI, SCALAR>(m, n, k, alpha, beta, beta_mode, A, lda, B,
-                                                      ldb, C, ldc, i0, j0, As, Bs, tri_nb);
 }
 
 // ----------------------------------------------------------------------------------
@@ -180,7 +202,7 @@ __global__ __launch_bounds__(256)
 void k_wgemm_plain(int64_t m, int64_t n, int64_t k, zddv alpha, zddv beta, int beta_mode,
                    const zddv *__restrict__ A, int64_t lda, const zddv *__restrict__ B,
                    int64_t ldb, zddv *__restrict__ C, int64_t ldc, int tiles_m, int rect_tiles,
-                   int64_t tri_nb, int diag_alt)
+                   int64_t tri_nb, int diag_alt, int nb32)
 {
     constexpr int BM = 16 * TM, BN = 16 * TN;
     static_assert(KCHUNK % BK == 0, "BK must divide the flush period");
@@ -217,10 +239,10 @@ void k_wgemm_plain(int64_t m, int64_t n, int64_t k, zddv alpha, zddv beta, int b
 
     const int tid = threadIdx.x;
     const int tx = tid & 15, ty = tid >> 4;
-    const int64_t i0 = (int64_t)(blockIdx.x % tiles_m) * BM;
-    const int64_t j0 = (int64_t)(blockIdx.x / tiles_m) * BN;
-    if constexpr (TRI != TRI_NONE)
-        if (!tri_tile_live<TRI>(i0, j0, BM, BN, n, tri_nb)) return;
+    // gemmtr: one workgroup per live rectangle tile, in the compact order of tri_live_tile
+    int64_t i0 = (int64_t)(blockIdx.x % tiles_m) * BM;
+    int64_t j0 = (int64_t)(blockIdx.x / tiles_m) * BN;
+    if constexpr (TRI != TRI_NONE) tri_live_tile<TRI, BM, BN>((int)blockIdx.x, (int)n, nb32, tiles_m, i0, j0);
     if constexpr (SCALAR) {
         scalar_order_tile<TM, TN, BK, OPA != OP_N, OPB != OP_N, TRI, OPA == OP_C, OPB == OP_C>(
             m, n, k, alpha, beta, beta_mode, A, lda, B, ldb, C, ldc, i0, j0, As, Bs, tri_nb);
@@ -345,19 +367,77 @@ namespace {
 constexpr int RTM = 4, RTN = 4, RBK = 16;   // real tile: 64 x 64, 16-deep slabs
 constexpr int CTM = 2, CTN = 2, CBK = 16;   // complex tile: 32 x 32
 
+// Compute units of the current device (cached per device).
+int device_cus()
+{
+    static std::atomic<int> cache[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return 0;
+    if (dev < 64) {
+        const int c = cache[dev].load(std::memory_order_relaxed);
+        if (c > 0) return c;
+    }
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    if (dev < 64) cache[dev].store(cus, std::memory_order_relaxed);
+    return cus;
+}
+
+// Live rectangle tiles of a gemmtr (see tri_live_rows).
+template <int TRI>
+int64_t tri_live_count(int BM, int BN, int n, int nb32, int tiles_m)
+{
+    int64_t live = 0;
+    const int tn = (n + BN - 1) / BN;
+    for (int jt = 0; jt < tn; ++jt) {
+        int first = 0, cnt = 0;
+        tri_live_rows<TRI>(jt, BM, BN, n, nb32, tiles_m, first, cnt);
+        live += cnt;
+    }
+    return live;
+}
+
 template <int TM, int TN, bool TA, bool TB, int TRI = TRI_NONE, bool SCALAR = false>
 hipError_t launch_real_tile(const GemmArgsR &g, hipStream_t s, int64_t tri_nb = 0)
 {
     constexpr int BM = 16 * TM, BN = 16 * TN;
+    const auto kern = k_mgemm_plain<TM, TN, RBK, TA, TB, TRI, SCALAR>;
     int tiles_m = (int)((g.m + BM - 1) / BM);
     int64_t tiles = (int64_t)tiles_m * ((g.n + BN - 1) / BN);
     const int64_t diag_blocks = TRI != TRI_NONE ? (g.n * tri_nb + 255) / 256 : 0;
     if (tiles + diag_blocks > INT32_MAX) return hipErrorInvalidValue;
+    int64_t rect_wgs = tiles, split_from = 0;
+    int nb32 = 0;
+    if constexpr (TRI != TRI_NONE) {
+        // one workgroup per live tile; the tiles of the last, partly filled round of resident
+        // workgroups as halves when those fit in one round
+        nb32 = (int)std::min<int64_t>(tri_nb, g.n + 1);
+        const int64_t live = tri_live_count<TRI>(BM, BN, (int)g.n, nb32, tiles_m);
+        split_from = live;
+        if (!SCALAR && TN % 2 == 0) {
+            static const int per_cu = [&] {
+                int b = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &b, reinterpret_cast<const void *>(kern), 256, 0) != hipSuccess) {
+                    (void)hipGetLastError();
+                    b = 0;
+                }
+                return b;
+            }();
+            const int64_t resident = (int64_t)per_cu * device_cus();
+            const int64_t r = resident > 0 ? live % resident : 0;
+            if (r > 0 && 2 * r <= resident) split_from = live - r;
+        }
+        rect_wgs = split_from + 2 * (live - split_from);
+        if (rect_wgs + diag_blocks > INT32_MAX) return hipErrorInvalidValue;
+    }
     ScopedTimer tm(TRI != TRI_NONE ? KC_GEMMTR : KC_PLAIN_REAL, s);
-    hipLaunchKernelGGL((k_mgemm_plain<TM, TN, RBK, TA, TB, TRI, SCALAR>),
-                       dim3((unsigned)(tiles + diag_blocks)), dim3(256), 0, s, g.m, g.n, g.k,
-                       g.alpha, g.beta, g.beta_mode, g.A, g.lda, g.B, g.ldb, g.C, g.ldc, tiles_m,
-                       (int)tiles, tri_nb);
+    hipLaunchKernelGGL(kern, dim3((unsigned)(rect_wgs + diag_blocks)), dim3(256), 0, s, g.m, g.n,
+                       g.k, g.alpha, g.beta, g.beta_mode, g.A, g.lda, g.B, g.ldb, g.C, g.ldc,
+                       tiles_m, (int)rect_wgs, tri_nb, (int)split_from, nb32);
     return hipGetLastError();
 }
 
@@ -384,11 +464,16 @@ hipError_t launch_cplx_tile(const GemmArgsZ &g, hipStream_t s, int64_t tri_nb)
     int64_t tiles = (int64_t)tiles_m * ((g.n + BN - 1) / BN);
     const int64_t diag_blocks = TRI != TRI_NONE ? (g.n * tri_nb + 255) / 256 : 0;
     if (tiles + diag_blocks > INT32_MAX) return hipErrorInvalidValue;
+    int nb32 = 0;
+    if constexpr (TRI != TRI_NONE) {   // one workgroup per live tile
+        nb32 = (int)std::min<int64_t>(tri_nb, g.n + 1);
+        tiles = tri_live_count<TRI>(BM, BN, (int)g.n, nb32, tiles_m);
+    }
     ScopedTimer tm(TRI != TRI_NONE ? KC_GEMMTR : KC_PLAIN_CPLX, s);
     hipLaunchKernelGGL((k_wgemm_plain<CTM, CTN, CBK, OA, OB, TRI, SCALAR>),
                        dim3((unsigned)(tiles + diag_blocks)), dim3(256), 0, s, g.m, g.n, g.k,
                        g.alpha, g.beta, g.beta_mode, g.A, g.lda, g.B, g.ldb, g.C, g.ldc, tiles_m,
-                       (int)tiles, tri_nb, TRI != TRI_NONE ? g.diag_alt : 0);
+                       (int)tiles, tri_nb, TRI != TRI_NONE ? g.diag_alt : 0, nb32);
     return hipGetLastError();
 }
 
@@ -411,9 +496,11 @@ hipError_t launch_cplx_b(const GemmArgsZ &g, hipStream_t s)
 
 // Rectangle tiles of the triangular-output GEMM (SIMD order): 32 x 64 (variant 1) unless
 // g.variant = 0 asks for 64 x 64.  Every live tile runs the whole depth, so the last,
-// partly filled round of resident tiles costs a whole tile time.  The 32 x 64 tile (164
-// VGPRs: three workgroups per CU against two) halves that tail; it was faster at every
-// size measured (n = 1024..8192, k = 1024..4096; 25.5 against 29.9 ms at n = k = 4096).
+// partly filled round of resident tiles costs a whole tile time (a CU left with fewer
+// workgroups runs them barely faster).  The 32 x 64 tile (164 VGPRs: three workgroups per CU
+// against two) halves that tail, and the launcher runs that round's tiles as 32 x 32 halves;
+// it was faster at every size measured (n = 1024..8192, k = 1024..4096; 25.5 against 29.9 ms
+// at n = k = 4096 before the halves and the compact grid).
 int gemmtr_tile_variant(const GemmArgsR &g)
 {
     return g.variant == 0 ? 0 : 1;

@@ -154,6 +154,66 @@ template <int TRI> DDI bool tri_tile_live(int64_t i0, int64_t j0, int BM, int BN
     }
 }
 
+// Live rectangle tiles of tile column jt (BM x BN tiles over the n x n C, tiles_m tile rows):
+// tile rows first .. first + count - 1, exactly those for which tri_tile_live holds.
+//   'U': live while i0 < R, R = ((min(j0 + BN, n) - 1) / nb) * nb, so rows [0, ceil(R / BM)).
+//   'L': live once i0 + BM - 1 (or n - 1 for the last row) >= X = (j0 / nb) * nb + nb, so
+//        rows [X / BM, tiles_m) when X <= n - 1 and none otherwise.
+// 32-bit arithmetic: n < 2^31, and nb is clamped to n + 1 (the same masks as any larger nb).
+template <int TRI>
+__host__ __device__ inline void tri_live_rows(int jt, int BM, int BN, int n, int nb, int tiles_m,
+                                              int &first, int &count)
+{
+    const int j0 = jt * BN;
+    if (TRI == TRI_UPPER) {
+        const int jl = (j0 + BN < n ? j0 + BN : n) - 1;
+        const int R = (jl / nb) * nb;
+        const int c = (R + BM - 1) / BM;
+        first = 0;
+        count = c < tiles_m ? c : tiles_m;
+    } else {
+        const int X = (j0 / nb) * nb + nb;
+        first = X <= n - 1 ? X / BM : 0;
+        count = X <= n - 1 ? tiles_m - X / BM : 0;
+    }
+}
+
+// The idx-th live rectangle tile, counted column by column (tile columns in ascending order,
+// live rows ascending within a column): the gemmtr grid has one workgroup per live tile
+// instead of one per tile of the n x n square.  Each wave scans 64 tile columns per step
+// (one per lane: live counts, inclusive prefix sum, then the first lane whose prefix passes
+// idx); every wave of the workgroup computes the same (i0, j0).
+template <int TRI, int BM, int BN>
+DDI void tri_live_tile(int idx, int n, int nb, int tiles_m, int64_t &i0, int64_t &j0)
+{
+    const int lane = threadIdx.x & 63;
+    const int tn = (n + BN - 1) / BN;
+    int before = 0;                      // live tiles in the columns of earlier steps
+    for (int c0 = 0; c0 < tn; c0 += 64) {
+        const int c = c0 + lane;
+        int first = 0, cnt = 0;
+        if (c < tn) tri_live_rows<TRI>(c, BM, BN, n, nb, tiles_m, first, cnt);
+        int incl = cnt;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += v;
+        }
+        const int target = idx - before;
+        const unsigned long long hit = __ballot(c < tn && target < incl);
+        if (hit) {
+            const int L = __ffsll((long long)hit) - 1;
+            const int f = __shfl(first, L, 64), ex = __shfl(incl - cnt, L, 64);
+            i0 = (int64_t)(f + target - ex) * BM;
+            j0 = (int64_t)(c0 + L) * BN;
+            return;
+        }
+        before += __shfl(incl, 63, 64);
+    }
+    i0 = (int64_t)tiles_m * BM;           // idx beyond the live tiles (not launched)
+    j0 = 0;
+}
+
 // The lane's TM x TN register-tile elements that a tile writes: in range and, for gemmtr,
 // in the rectangle; fixed for the tile, so one bit mask (one register) for the K loop.
 template <int TM, int TN, int TRI>

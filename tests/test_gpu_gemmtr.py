@@ -65,6 +65,20 @@ def test_real_medium_all_ops_bitwise(gpu, uplo):
 
 
 @pytest.mark.parametrize("uplo", ["U", "L"])
+def test_grid_rounds_bitwise(gpu, uplo):
+    """The rectangle grid has one workgroup per live 32 x 64 tile; when the last, partly
+    filled round of resident workgroups (three per CU) holds at most half a round, its tiles
+    run as 32 x 32 halves.  n = 2900 leaves a round more than half full (no halves), n = 3100
+    a small one (halves), n = 700 has fewer tiles than one round (all halves); k = 96 keeps
+    the restatement quick.  Bitwise against the restatement."""
+    rng = np.random.default_rng(530 + (uplo == "U"))
+    for n, ta, tb in [(2900, "N", "N"), (3100, "T", "N"), (700, "N", "T")]:
+        scal = (ddgen.real_scalar(rng, scale=2), ddgen.real_scalar(rng, scale=2))
+        print(f"rounds {uplo}{ta}{tb} n={n}", flush=True)
+        _check(_case(rng, False, n, 96, uplo, ta, tb, scal), False)
+
+
+@pytest.mark.parametrize("uplo", ["U", "L"])
 def test_cplx_medium_bitwise(gpu, uplo):
     rng = np.random.default_rng(520 + (uplo == "U"))
     for ta, tb in [("N", "N"), ("C", "N"), ("T", "C"), ("N", "T")]:
