@@ -400,6 +400,34 @@ int64_t tri_live_count(int BM, int BN, int n, int nb32, int tiles_m)
     return live;
 }
 
+// Workgroups of k_mgemm_plain<TM, TN, ...> resident on the current device at once (occupancy
+// query per instantiation x compute units; 0 when unknown).
+template <int TM, int TN, bool TA, bool TB, int TRI, bool SCALAR>
+int64_t resident_wgs()
+{
+    static const int per_cu = [] {
+        int b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &b, reinterpret_cast<const void *>(k_mgemm_plain<TM, TN, RBK, TA, TB, TRI, SCALAR>),
+                256, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            b = 0;
+        }
+        return b;
+    }();
+    return (int64_t)per_cu * device_cus();
+}
+
+// Modeled length of a gemmtr rectangle launch in tile times: whole rounds of `resident`
+// workgroups, plus half a tile time when the last round is at most half full (its tiles run
+// as halves) or a whole one when it is fuller.
+inline double tri_rounds(int64_t live, int64_t resident)
+{
+    if (resident <= 0) return 0.0;
+    const int64_t r = live % resident;
+    return (double)(live / resident) + (r == 0 ? 0.0 : 2 * r <= resident ? 0.5 : 1.0);
+}
+
 template <int TM, int TN, bool TA, bool TB, int TRI = TRI_NONE, bool SCALAR = false>
 hipError_t launch_real_tile(const GemmArgsR &g, hipStream_t s, int64_t tri_nb = 0)
 {
@@ -418,16 +446,7 @@ hipError_t launch_real_tile(const GemmArgsR &g, hipStream_t s, int64_t tri_nb = 
         const int64_t live = tri_live_count<TRI>(BM, BN, (int)g.n, nb32, tiles_m);
         split_from = live;
         if (!SCALAR && TN % 2 == 0) {
-            static const int per_cu = [&] {
-                int b = 0;
-                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                        &b, reinterpret_cast<const void *>(kern), 256, 0) != hipSuccess) {
-                    (void)hipGetLastError();
-                    b = 0;
-                }
-                return b;
-            }();
-            const int64_t resident = (int64_t)per_cu * device_cus();
+            const int64_t resident = resident_wgs<TM, TN, TA, TB, TRI, SCALAR>();
             const int64_t r = resident > 0 ? live % resident : 0;
             if (r > 0 && 2 * r <= resident) split_from = live - r;
         }
@@ -494,16 +513,27 @@ hipError_t launch_cplx_b(const GemmArgsZ &g, hipStream_t s)
     }
 }
 
-// Rectangle tiles of the triangular-output GEMM (SIMD order): 32 x 64 (variant 1) unless
-// g.variant = 0 asks for 64 x 64.  Every live tile runs the whole depth, so the last,
-// partly filled round of resident tiles costs a whole tile time (a CU left with fewer
-// workgroups runs them barely faster).  The 32 x 64 tile (164 VGPRs: three workgroups per CU
-// against two) halves that tail, and the launcher runs that round's tiles as 32 x 32 halves;
-// it was faster at every size measured (n = 1024..8192, k = 1024..4096; 25.5 against 29.9 ms
-// at n = k = 4096 before the halves and the compact grid).
-int gemmtr_tile_variant(const GemmArgsR &g)
+// Rectangle tiles of the triangular-output GEMM (SIMD order): 32 x 64 (variant 1; 164 VGPRs,
+// three workgroups per CU) or 64 x 64 (variant 0; two per CU).  A 64 x 64 tile does 3.5% more
+// work per CU-second, but its rounds of resident workgroups are coarser: every live tile
+// runs the whole depth, so the last, partly filled round costs a whole tile time (or half,
+// when its tiles run as halves; a CU left with fewer workgroups runs them barely faster).
+// Without a requested variant the launcher takes the tile whose modeled length (tri_rounds)
+// is shorter, a 64 x 64 round costing 1.29 times a 32 x 64 round (4.70 against 3.64 ms per
+// round at k = 4096): 32 x 64 at n = 2048 and 4096, 64 x 64 from n ~ 8192 on.
+constexpr double kRound64Cost = 1.29;
+
+template <bool TA, bool TB, int TRI>
+int gemmtr_tile_variant(const GemmArgsR &g, int64_t tri_nb)
 {
-    return g.variant == 0 ? 0 : 1;
+    if (g.variant == 0 || g.variant == 1) return g.variant;
+    const int n = (int)g.n, nb32 = (int)std::min<int64_t>(tri_nb, g.n + 1);
+    const int64_t live32 = tri_live_count<TRI>(32, 64, n, nb32, (n + 31) / 32);
+    const int64_t live64 = tri_live_count<TRI>(64, 64, n, nb32, (n + 63) / 64);
+    const double r32 = tri_rounds(live32, resident_wgs<2, 4, TA, TB, TRI, false>());
+    const double r64 = tri_rounds(live64, resident_wgs<RTM, RTN, TA, TB, TRI, false>());
+    if (r32 <= 0.0 || r64 <= 0.0) return 1;
+    return r64 * kRound64Cost < r32 ? 0 : 1;
 }
 
 // gemmtr in one launch: the masked rectangle tiles, then (workgroups past the tiles) the
@@ -516,11 +546,13 @@ hipError_t launch_gemmtr_real(const GemmArgsR &g, bool upper, int64_t nb, hipStr
     if (g.order == ORDER_SCALAR)
         return upper ? launch_real_tile<RTM, RTN, TA, TB, TRI_UPPER, true>(g, s, nb)
                      : launch_real_tile<RTM, RTN, TA, TB, TRI_LOWER, true>(g, s, nb);
-    if (gemmtr_tile_variant(g) == 1)
-        return upper ? launch_real_tile<2, 4, TA, TB, TRI_UPPER>(g, s, nb)
-                     : launch_real_tile<2, 4, TA, TB, TRI_LOWER>(g, s, nb);
-    return upper ? launch_real_tile<RTM, RTN, TA, TB, TRI_UPPER>(g, s, nb)
-                 : launch_real_tile<RTM, RTN, TA, TB, TRI_LOWER>(g, s, nb);
+    if (upper)
+        return gemmtr_tile_variant<TA, TB, TRI_UPPER>(g, nb) == 1
+                   ? launch_real_tile<2, 4, TA, TB, TRI_UPPER>(g, s, nb)
+                   : launch_real_tile<RTM, RTN, TA, TB, TRI_UPPER>(g, s, nb);
+    return gemmtr_tile_variant<TA, TB, TRI_LOWER>(g, nb) == 1
+               ? launch_real_tile<2, 4, TA, TB, TRI_LOWER>(g, s, nb)
+               : launch_real_tile<RTM, RTN, TA, TB, TRI_LOWER>(g, s, nb);
 }
 
 template <int OA, int OB>

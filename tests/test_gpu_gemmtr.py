@@ -66,16 +66,19 @@ def test_real_medium_all_ops_bitwise(gpu, uplo):
 
 @pytest.mark.parametrize("uplo", ["U", "L"])
 def test_grid_rounds_bitwise(gpu, uplo):
-    """The rectangle grid has one workgroup per live 32 x 64 tile; when the last, partly
-    filled round of resident workgroups (three per CU) holds at most half a round, its tiles
-    run as 32 x 32 halves.  n = 2900 leaves a round more than half full (no halves), n = 3100
-    a small one (halves), n = 700 has fewer tiles than one round (all halves); k = 96 keeps
-    the restatement quick.  Bitwise against the restatement."""
+    """The rectangle grid has one workgroup per live tile; when the last, partly filled round
+    of resident workgroups holds at most half a round, its tiles run as halves.  With 32 x 64
+    tiles (three workgroups per CU), n = 2900 leaves a round more than half full (no halves),
+    n = 3100 a small one (halves), n = 700 has fewer tiles than one round (all halves); from
+    n ~ 5000 the launcher models 64 x 64 tiles (two per CU) as shorter and takes them
+    (n = 5000: a last round under half full, 64 x 32 halves).  Small k keeps the restatement
+    quick.  Bitwise against the restatement."""
     rng = np.random.default_rng(530 + (uplo == "U"))
-    for n, ta, tb in [(2900, "N", "N"), (3100, "T", "N"), (700, "N", "T")]:
+    for n, k, ta, tb in [(2900, 96, "N", "N"), (3100, 96, "T", "N"), (700, 96, "N", "T"),
+                         (5000, 24, "T", "T")]:
         scal = (ddgen.real_scalar(rng, scale=2), ddgen.real_scalar(rng, scale=2))
-        print(f"rounds {uplo}{ta}{tb} n={n}", flush=True)
-        _check(_case(rng, False, n, 96, uplo, ta, tb, scal), False)
+        print(f"rounds {uplo}{ta}{tb} n={n} k={k}", flush=True)
+        _check(_case(rng, False, n, k, uplo, ta, tb, scal), False)
 
 
 @pytest.mark.parametrize("uplo", ["U", "L"])
@@ -148,7 +151,7 @@ def test_large_device_bitwise(gpu):
 
 
 def test_rectangle_64x64_tile_matches(gpu):
-    """MGEMMTR_TILE=64 selects the 64x64 rectangle tile instead of the default 32x64.  The
+    """MGEMMTR_TILE=64 selects the 64x64 rectangle tile at every size (unset: chosen per call).  The
     setting is read once per process, so the cases run in a child process: 16 fuzz cases
     plus 700 x 900 in both triangles, bitwise against the restatement."""
     import os
