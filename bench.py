@@ -11,16 +11,18 @@ oracle's restatement of the reference OpenMP mgemm_ timed on this host on a boun
 sample of the same problem (minimum of 3 runs after a warm-up) and on the whole 512^3
 workload (minimum of 40 runs after a warm-up).  The same leg, outside every timed region,
 checks the normwise error of every GPU result against the binary128 oracle
-("err_vs_binary128": the headline's both paths and the complex leg on sampled entries, the
-512^3 workload on all entries, a sample of the batched problems).
+("err_vs_binary128": the headline's both paths, the complex leg and the plain 4096^3 GEMM
+in all four op pairs on sampled entries, the 512^3 workload on all entries, a sample of the
+batched problems).
 
 Further objects on the same line (each with its own workload): "gemm_512" (the 512^3
 workload on both GPU paths), "library_int8_gemm" (the
 vendor INT8 GEMM at one residue GEMM's shape, a reference point for the roofline),
 "batched" (2000 small GEMMs in one captured launch per path), "complex" (4096^3 with
 op(A)='C', both paths), "gemmtr" (the triangular-output GEMM, 4096 x 4096 upper triangle,
-k = 4096), "plain_orders" (the plain GEMM at 4096^3 in the reference's AVX2-kernel and
-scalar-kernel operation orders), "gemv" (16384 x 16384 matrix-vector products, HBM-bound), "host_boundary" (the
+k = 4096), "plain_orders" (the plain GEMM at 4096^3, random alpha/beta, C preloaded: the
+reference's AVX2-kernel order in all four op(A)/op(B) N/T pairs and its scalar-kernel order),
+"gemv" (16384 x 16384 matrix-vector products, HBM-bound), "host_boundary" (the
 drop-in mgemm_ with host operands, PCIe transfers inside the timing) and "serial_trailing"
 (a blocked SYRK's trailing updates through mgemm_serial on host arrays).
 
@@ -405,11 +407,13 @@ def gemmtr_bench(dev, n=4096, k=4096, reps=3, seed=41):
             "kernels_ms_per_call": {kn: round(v[0] / reps, 3) for kn, v in kt.items()}}
 
 
-def plain_orders_bench(dev, n=4096, reps=2, seed=47):
-    """The plain real GEMM at n^3 (NN, random alpha/beta, inputs in HBM) in both operation
-    orders the reference has: its AVX2/FMA kernels (18 FP64 lane-ops per DD multiply-add)
-    and its scalar kernels for hosts without AVX2/FMA (29: a faithful multiply and a
-    faithful add per multiply-add).  fp64_frac = lane-ops / t against 39.3e12 lane-ops/s."""
+def plain_orders_bench(dev, n=4096, reps=2, seed=47, checks=None):
+    """The plain real GEMM at n^3 (random alpha/beta with hi ~ U[-2,2], C preloaded, inputs in
+    HBM): the reference's AVX2/FMA operation order (18 FP64 lane-ops per DD multiply-add) in
+    all four op(A)/op(B) N/T pairs, and its scalar order for hosts without AVX2/FMA (29: a
+    faithful multiply and a faithful add per multiply-add) on NN.  fp64_frac = lane-ops / t
+    against 39.3e12 lane-ops/s.  For the checker, 64 sampled entries of each SIMD-order
+    result go into ``checks``."""
     import numpy as np
     import torch
     import mwblas
@@ -419,18 +423,30 @@ def plain_orders_bench(dev, n=4096, reps=2, seed=47):
     rng = np.random.default_rng(seed)
     alpha, beta = random_scalar(rng, 1), random_scalar(rng, 1)
     stream = torch.cuda.current_stream().cuda_stream
-    out = {"workload": f"plain real DD GEMM NN {n}^3, random alpha/beta"}
+    out = {"workload": f"plain real DD GEMM {n}^3, random alpha/beta, C preloaded"}
     prev = mwblas.get_plain_order()
+    host = None
     try:
-        for name, order, ops in (("simd", mwblas.ORDER_SIMD, 18), ("scalar", mwblas.ORDER_SCALAR, 29)):
+        runs = [("simd_" + ta + tb, mwblas.ORDER_SIMD, 18, ta, tb) for ta in "NT" for tb in "NT"]
+        runs.append(("scalar_NN", mwblas.ORDER_SCALAR, 29, "N", "N"))
+        for name, order, ops, ta, tb in runs:
             mwblas.set_plain_order(order)
             C = C0.clone()
-            mwblas.gemm_device(False, "N", "N", n, n, n, alpha, A, n, B, n, beta, C, n,
+            mwblas.gemm_device(False, ta, tb, n, n, n, alpha, A, n, B, n, beta, C, n,
                                path=mwblas.PATH_PLAIN, stream=stream)
             torch.cuda.synchronize()
+            if checks is not None and order == mwblas.ORDER_SIMD:
+                if host is None:
+                    crng = np.random.default_rng(seed + 1)
+                    host = (A.cpu().numpy(), B.cpu().numpy(), C0.cpu().numpy(),
+                            (crng.integers(0, n, 64), crng.integers(0, n, 64)))
+                Ah, Bh, C0h, ent = host
+                checks[f"plain_{n}_ops/{ta}{tb}"] = _check(False, ta, tb, n, n, n, alpha, Ah, n,
+                                                           Bh, n, beta, C0h, n, C.cpu().numpy(),
+                                                           entries=ent)
             t0 = time.perf_counter()
             for _ in range(reps):
-                mwblas.gemm_device(False, "N", "N", n, n, n, alpha, A, n, B, n, beta, C, n,
+                mwblas.gemm_device(False, ta, tb, n, n, n, alpha, A, n, B, n, beta, C, n,
                                    path=mwblas.PATH_PLAIN, stream=stream)
             torch.cuda.synchronize()
             t = (time.perf_counter() - t0) / reps
@@ -775,7 +791,7 @@ def main():
         result["complex"] = complex_bench(dev, checks=None if args.no_cpu else checks)
     if not args.no_gemmtr:
         result["gemmtr"] = gemmtr_bench(dev)
-        result["plain_orders"] = plain_orders_bench(dev)
+        result["plain_orders"] = plain_orders_bench(dev, checks=None if args.no_cpu else checks)
     if not args.no_gemv:
         result["gemv"] = gemv_bench(dev)
     if not args.no_host:
