@@ -187,3 +187,48 @@ def test_rectangle_64x64_tile_matches(gpu):
     r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
                        timeout=240)
     assert r.returncode == 0 and "rectangle 64x64 ok" in r.stdout, r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("nb", [1, 5, 33, 64, 300])
+def test_block_sizes_bitwise(gpu, nb):
+    """MGEMMTR_NB / WGEMMTR_NB set the reference's diagonal-block size, which fixes which
+    entries take the scalar sequence and which the GEMM's; the live-tile grid follows it
+    (nb below, equal to and above the 32- and 64-row tiles, and nb > n, where every entry is
+    diagonal).  Read once per process, so each nb runs in a child process: real n = 97, 515
+    and 2100 (whose last round runs as halves) in both triangles and a complex n = 203,
+    bitwise against the restatement with the same nb."""
+    import os
+    import subprocess
+    import sys
+    import textwrap
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = textwrap.dedent(f"""
+        import sys
+        sys.path[:0] = [{root!r}, {os.path.join(root, 'epblas-parallel_amd')!r},
+                        {os.path.join(root, 'oracle')!r}]
+        import numpy as np
+        import ddgen, mwblas, oracle
+        nb = {nb}
+        rng = np.random.default_rng(1600 + nb)
+        cases = [(False, 97, 40), (False, 515, 40), (False, 2100, 24), (True, 203, 30)]
+        for i, (cplx, n, k) in enumerate(cases):
+            for uplo in "UL":
+                ta, tb = ("C", "N") if cplx else ("NT"[i % 2], "N")
+                ar, ac = (n, k) if ta == "N" else (k, n)
+                br, bc = (k, n) if tb == "N" else (n, k)
+                gen = ddgen.cplx_matrix if cplx else ddgen.real_matrix
+                scal = ddgen.cplx_scalar if cplx else ddgen.real_scalar
+                A, B = gen(rng, ar, ac, ar + 1), gen(rng, br, bc, br + 2)
+                C0 = gen(rng, n, n, n + 3)
+                al, be = scal(rng, scale=2), scal(rng, scale=2)
+                args = (uplo, ta, tb, n, k, al, A, ar + 1, B, br + 2, be)
+                Cg, Cp = C0.copy(), C0.copy()
+                (mwblas.wgemmtr if cplx else mwblas.mgemmtr)(*args, Cg, n + 3)
+                (oracle.port_wgemmtr if cplx else oracle.port_mgemmtr)(*args, Cp, n + 3, nb=nb)
+                assert np.array_equal(Cg, Cp, equal_nan=True), (nb, cplx, uplo, n, k)
+        print("block size ok", flush=True)
+    """)
+    env = dict(os.environ, MGEMMTR_NB=str(nb), WGEMMTR_NB=str(nb))
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                       timeout=240)
+    assert r.returncode == 0 and "block size ok" in r.stdout, r.stdout + r.stderr
