@@ -7,10 +7,11 @@
 // queue balances uneven problems.  Each tile runs exactly the single-GEMM tile routine, so
 // every problem's result is bit-identical to a separate mgemm_ call.
 //
-// Ozaki: the row/column maxima, residue split and reconstruction run over flattened unit
-// spaces covering all problems (a lane finds its problem by binary search over prefix
-// offsets); the residue GEMMs run as a persistent work queue over (problem, modulus, tile)
-// with 128x128 tiles.  All problems share one modulus plan sized for the deepest K.
+// Ozaki: the row/column maxima, residue split (maxima and split in one kernel for a
+// row-contiguous operand) and reconstruction run over flattened unit spaces covering all
+// problems (a wave finds its problem by binary search over prefix offsets); the residue GEMMs
+// run as persistent work queues over (problem, modulus group, tile), one queue per XCD.  All
+// problems share one modulus plan sized for the deepest K.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -137,11 +138,10 @@ void k_batch_plain(const BatchProb *__restrict__ probs, const int4 *__restrict__
 // ---------------------------------------------------------------------------------
 // Ozaki phases over all problems.
 // ---------------------------------------------------------------------------------
-// Row maxima.  Column-contiguous input: unit = (row r, 64-deep k chunk), rows fastest so
-// lanes read adjacent rows.  Row-contiguous input (KC): every problem's unit count is a
-// multiple of 64 and a wave takes the 256 consecutive k of one (row, chunk), four per lane,
-// reduced across the wave before a single atomic.
-template <bool A_SIDE, bool KC>
+// Column-contiguous operand (stored K x rows: A for op 'N', B for op 'T'; a row-contiguous
+// one goes through k_batch_rowsplit_kc below).
+// Row maxima: unit = (row r, 64-deep k chunk), rows fastest so lanes read adjacent rows.
+template <bool A_SIDE>
 __global__ __launch_bounds__(256)
 void k_batch_rowmax(const BatchProb *__restrict__ probs, UnitSpace us)
 {
@@ -154,41 +154,18 @@ void k_batch_rowmax(const BatchProb *__restrict__ probs, UnitSpace us)
     const ddv *X = A_SIDE ? p.A : p.B;
     const int64_t ld = A_SIDE ? p.lda : p.ldb;
     unsigned long long *rmax = A_SIDE ? p.rA : p.rB;
-    if constexpr (KC) {
-        // a wave takes 256 consecutive k of one row, four coalesced steps of 64
-        const int64_t nch = (p.k + 255) / 256, w = local / 64;
-        const int lane = (int)(local % 64);
-        const int64_t r = w / nch, kb = (w % nch) * 256 + lane;
-        unsigned long long bits = 0ull;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int64_t kk = kb + 64 * q;
-            const double mx = kk < p.k ? mag(X[r * ld + kk]) : 0.0;
-            const unsigned long long b = (unsigned long long)__double_as_longlong(mx);
-            bits = b > bits ? b : bits;
-        }
-        // NaN-aware maximum on the bit patterns of non-negative values.
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const unsigned long long o = __shfl_xor(bits, off, 64);
-            bits = o > bits ? o : bits;
-        }
-        if (lane == 0 && bits != 0ull) atomicMax(&rmax[r], bits);
-    } else {
-        if (local >= R * ((p.k + 63) / 64)) return;   // padding units
-        const int64_t r = local % R, kc = local / R;
-        double mx = 0.0;
-        const int64_t kend = (kc * 64 + 64 < p.k) ? kc * 64 + 64 : p.k;
-        for (int64_t kk = kc * 64; kk < kend; ++kk) mx = nan_max(mx, mag(X[kk * ld + r]));
-        if (mx != 0.0 || mx != mx) atomicMax(&rmax[r], (unsigned long long)__double_as_longlong(mx));
-    }
+    if (local >= R * ((p.k + 63) / 64)) return;   // padding units
+    const int64_t r = local % R, kc = local / R;
+    double mx = 0.0;
+    const int64_t kend = (kc * 64 + 64 < p.k) ? kc * 64 + 64 : p.k;
+    for (int64_t kk = kc * 64; kk < kend; ++kk) mx = nan_max(mx, mag(X[kk * ld + r]));
+    if (mx != 0.0 || mx != mx) atomicMax(&rmax[r], (unsigned long long)__double_as_longlong(mx));
 }
 
-// Residue split: unit = (row r, 16-wide k group).  Row-contiguous input (KC): k groups
-// fastest.  Column-contiguous input: blocks of 8 rows x 8 k groups per wave (unit counts
-// padded to whole blocks), so the residue chunks a wave writes for one row form a 128-byte
-// line and its loads read 128-byte segments.
-template <bool A_SIDE, bool KC>
+// Residue split: unit = (row r, 16-wide k group), in blocks of 8 rows x 8 k groups per wave
+// (unit counts padded to whole blocks), so the residue chunks a wave writes for one row form
+// a 128-byte line and its loads read 128-byte segments.
+template <bool A_SIDE>
 __global__ __launch_bounds__(256)
 void k_batch_split(const BatchProb *__restrict__ probs, UnitSpace us, int bits,
                    const ModConst *__restrict__ mcs, int nmod)
@@ -200,22 +177,102 @@ void k_batch_split(const BatchProb *__restrict__ probs, UnitSpace us, int bits,
     const int64_t R = A_SIDE ? p.m : p.n;
     const int64_t G = p.Kc / 16;
     const int64_t local = u - us.start[pi];
-    int64_t r, kg;
-    if (KC) {
-        if (local >= R * G) return;   // padding units
-        kg = local % G;
-        r = local / G;
-    } else {
-        const int64_t rb8 = (R + 7) / 8, blk = local / 64, w = local % 64;
-        r = (blk % rb8) * 8 + w / 8;
-        kg = (blk / rb8) * 8 + w % 8;
-        if (r >= R || kg >= G) return;
-    }
+    const int64_t rb8 = (R + 7) / 8, blk = local / 64, w = local % 64;
+    const int64_t r = (blk % rb8) * 8 + w / 8;
+    const int64_t kg = (blk / rb8) * 8 + w % 8;
+    if (r >= R || kg >= G) return;
     const int64_t rows = A_SIDE ? p.Mp : p.Np;
-    oz_split_unit<ddv, KC, false>(A_SIDE ? p.A : p.B, A_SIDE ? p.lda : p.ldb, p.k,
-                           A_SIDE ? p.rA : p.rB, bits, 0, mcs, nmod,
-                           A_SIDE ? p.resA : p.resB, 1, (int8_t *)nullptr, 0, p.Kc,
-                           rows * p.Kc, r, kg);
+    oz_split_unit<ddv, false, false>(A_SIDE ? p.A : p.B, A_SIDE ? p.lda : p.ldb, p.k,
+                                     A_SIDE ? p.rA : p.rB, bits, 0, mcs, nmod,
+                                     A_SIDE ? p.resA : p.resB, 1, (int8_t *)nullptr, 0, p.Kc,
+                                     rows * p.Kc, r, kg);
+}
+
+// Row-contiguous operand (KC): row maximum and residue split in one kernel.  A row takes a
+// group of L lanes (L = kc_lanes(Kc), the least power of two with 8L >= Kc, at most 64), so a
+// wave holds 64/L rows: short rows (the batches' K ~ 16..512) fill the wave instead of leaving
+// most of its lanes idle.  Lane l of a group holds entries k0 + 8l .. k0 + 8l + 7 of each
+// 8L-deep chunk (a group reads its row's 16 * 8L bytes contiguously and writes 8L contiguous
+// bytes per modulus, one 8-byte store per lane).  The row maximum is reduced across the group
+// and stored by its first lane (the row's only writer); the split then uses it exactly as
+// oz_split_unit does.  Rows of up to 512 entries stay in registers between the two passes;
+// longer rows are read again (from L2) for the split.  Entries in [k, Kc) are written as
+// zero residues.
+__host__ __device__ inline int kc_lanes(int64_t Kc)
+{
+    int L = 1;
+    while (L < 64 && 8 * (int64_t)L < Kc) L *= 2;
+    return L;
+}
+
+template <bool A_SIDE>
+__global__ __launch_bounds__(256)
+void k_batch_rowsplit_kc(const BatchProb *__restrict__ probs, UnitSpace us, int bits,
+                         const ModConst *__restrict__ mcs, int nmod)
+{
+    const int64_t u = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (u >= us.start[us.count]) return;
+    const int pi = wave_problem(us);
+    const BatchProb &p = probs[pi];
+    const int64_t R = A_SIDE ? p.m : p.n;
+    const int64_t K = p.k, Kc = p.Kc;
+    const int L = kc_lanes(Kc);
+    const int lane = (int)(threadIdx.x & 63), ll = lane & (L - 1);
+    const int64_t r = ((u - us.start[pi]) >> 6) * (64 / L) + lane / L;
+    const bool live = r < R;            // lanes of a row past the last still join the shuffles
+    const ddv *X = (A_SIDE ? p.A : p.B) + (live ? r : 0) * (A_SIDE ? p.lda : p.ldb);
+    int8_t *out = (A_SIDE ? p.resA : p.resB) + r * Kc;
+    const int64_t mod_stride = (A_SIDE ? p.Mp : p.Np) * Kc;
+    constexpr int E = 8;
+    const int64_t span = (int64_t)E * L;
+    ddv v[E];
+    auto load = [&](int64_t k0) {
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            const int64_t kk = k0 + E * ll + s;
+            v[s] = live && kk < K ? X[kk] : ddv{0.0, 0.0};
+        }
+    };
+    // pass 1: NaN-aware maximum of |x| on the bit patterns of non-negative values
+    unsigned long long mx = 0ull;
+    for (int64_t k0 = 0; k0 < K; k0 += span) {
+        load(k0);
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            const unsigned long long b = (unsigned long long)__double_as_longlong(mag(v[s]));
+            mx = b > mx ? b : mx;
+        }
+    }
+    for (int off = L / 2; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(mx, off, 64);
+        mx = o > mx ? o : mx;
+    }
+    if (!live) return;
+    if (ll == 0) (A_SIDE ? p.rA : p.rB)[r] = mx;
+    // pass 2: residues of round(x * 2^sigma)
+    bool bad;
+    const int sigma = row_sigma(mx, bits, bad);
+    for (int64_t k0 = 0; k0 < Kc; k0 += span) {
+        if (K > span) load(k0);          // else the whole row is still in v
+        const int64_t kl = k0 + E * ll;
+        if (kl >= Kc) continue;
+        double c0[E], c1[E], c2[E];
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            double hi = 0.0, lo = 0.0;
+            if (!bad) comp(v[s], 0, hi, lo);
+            int_chunks(hi, lo, sigma, c0[s], c1[s], c2[s]);
+        }
+        for (int t = 0; t < nmod; ++t) {
+            const ModConst mc = mcs[t];
+            int res[E];
+#pragma unroll
+            for (int s = 0; s < E; ++s) res[s] = sym_residue(c0[s], c1[s], c2[s], mc);
+            *reinterpret_cast<uint2 *>(out + t * mod_stride + kl) =
+                make_uint2(pack_low_bytes(res[0], res[1], res[2], res[3]),
+                           pack_low_bytes(res[4], res[5], res[6], res[7]));
+        }
+    }
 }
 
 // Work queues of the residue GEMMs, one per XCD.  Each XCD has its own L2.  The plan assigns
@@ -494,24 +551,24 @@ static int batch_enqueue(BatchPlan *P, hipStream_t s)
         B_TRY(hipGetLastError());
         UnitSpace ra{P->us_ra, P->count}, rb{P->us_rb, P->count};
         UnitSpace sa{P->us_sa, P->count}, sb{P->us_sb, P->count};
+        // A row-contiguous operand takes its maxima and split in one kernel; a
+        // column-contiguous one its maxima kernel, then (below) its split kernel.
         if (P->n_ra) {
-            if (a_kc) hipLaunchKernelGGL((k_batch_rowmax<true, true>), dim3(blocks_for(P->n_ra)), dim3(256), 0, s, P->probs, ra);
-            else      hipLaunchKernelGGL((k_batch_rowmax<true, false>), dim3(blocks_for(P->n_ra)), dim3(256), 0, s, P->probs, ra);
+            if (a_kc) hipLaunchKernelGGL((k_batch_rowsplit_kc<true>), dim3(blocks_for(P->n_ra)), dim3(256), 0, s, P->probs, ra, P->oz.bits, P->mcs, P->oz.nmod);
+            else      hipLaunchKernelGGL((k_batch_rowmax<true>), dim3(blocks_for(P->n_ra)), dim3(256), 0, s, P->probs, ra);
             B_TRY(hipGetLastError());
         }
         if (P->n_rb) {
-            if (b_kc) hipLaunchKernelGGL((k_batch_rowmax<false, true>), dim3(blocks_for(P->n_rb)), dim3(256), 0, s, P->probs, rb);
-            else      hipLaunchKernelGGL((k_batch_rowmax<false, false>), dim3(blocks_for(P->n_rb)), dim3(256), 0, s, P->probs, rb);
+            if (b_kc) hipLaunchKernelGGL((k_batch_rowsplit_kc<false>), dim3(blocks_for(P->n_rb)), dim3(256), 0, s, P->probs, rb, P->oz.bits, P->mcs, P->oz.nmod);
+            else      hipLaunchKernelGGL((k_batch_rowmax<false>), dim3(blocks_for(P->n_rb)), dim3(256), 0, s, P->probs, rb);
             B_TRY(hipGetLastError());
         }
         if (P->n_sa) {
-            if (a_kc) hipLaunchKernelGGL((k_batch_split<true, true>), dim3(blocks_for(P->n_sa)), dim3(256), 0, s, P->probs, sa, P->oz.bits, P->mcs, P->oz.nmod);
-            else      hipLaunchKernelGGL((k_batch_split<true, false>), dim3(blocks_for(P->n_sa)), dim3(256), 0, s, P->probs, sa, P->oz.bits, P->mcs, P->oz.nmod);
+            hipLaunchKernelGGL((k_batch_split<true>), dim3(blocks_for(P->n_sa)), dim3(256), 0, s, P->probs, sa, P->oz.bits, P->mcs, P->oz.nmod);
             B_TRY(hipGetLastError());
         }
         if (P->n_sb) {
-            if (b_kc) hipLaunchKernelGGL((k_batch_split<false, true>), dim3(blocks_for(P->n_sb)), dim3(256), 0, s, P->probs, sb, P->oz.bits, P->mcs, P->oz.nmod);
-            else      hipLaunchKernelGGL((k_batch_split<false, false>), dim3(blocks_for(P->n_sb)), dim3(256), 0, s, P->probs, sb, P->oz.bits, P->mcs, P->oz.nmod);
+            hipLaunchKernelGGL((k_batch_split<false>), dim3(blocks_for(P->n_sb)), dim3(256), 0, s, P->probs, sb, P->oz.bits, P->mcs, P->oz.nmod);
             B_TRY(hipGetLastError());
         }
     }
@@ -685,11 +742,13 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
         if (!P->fused) ws += rup((int64_t)nmod * q.Np * q.Mp, 256);
         // unit counts of the maxima and split kernels (layouts documented there), each
         // padded to whole waves so that a wave never spans two problems
-        const int64_t nch = (q.k + 63) / 64, nch4 = (q.k + 255) / 256, G = q.Kc / 16;
-        ra[p + 1] = a_kc ? q.m * nch4 * 64 : rup(q.m * nch, 64);
-        rb[p + 1] = b_kc ? q.n * nch4 * 64 : rup(q.n * nch, 64);
-        sa[p + 1] = a_kc ? rup(q.m * G, 64) : rup(q.m, 8) * rup(G, 8);
-        sb[p + 1] = b_kc ? rup(q.n * G, 64) : rup(q.n, 8) * rup(G, 8);
+        // (a row-contiguous operand: maxima and split in one kernel, 64 / kc_lanes rows per
+        // wave; a column-contiguous one: maxima, then split)
+        const int64_t nch = (q.k + 63) / 64, G = q.Kc / 16, rpw = 64 / kc_lanes(q.Kc);
+        ra[p + 1] = a_kc ? (q.m + rpw - 1) / rpw * 64 : rup(q.m * nch, 64);
+        rb[p + 1] = b_kc ? (q.n + rpw - 1) / rpw * 64 : rup(q.n * nch, 64);
+        sa[p + 1] = a_kc ? 0 : rup(q.m, 8) * rup(G, 8);
+        sb[p + 1] = b_kc ? 0 : rup(q.n, 8) * rup(G, 8);
         for (int64_t tn = 0; tn < (q.Np + OT - 1) / OT; ++tn)
             for (int64_t tm = 0; tm < (q.Mp + OT - 1) / OT; ++tm) tl.push_back(int4{(int)p, 0, (int)tm, (int)tn});
     }

@@ -150,6 +150,41 @@ def test_ozaki_batched_vs_binary128(gpu, ta, tb, variant, monkeypatch):
     plan.close()
 
 
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "T")])
+def test_ozaki_batched_rows_beyond_one_pass(gpu, ta, tb):
+    """A row-contiguous operand's maxima and split run in one kernel, one wave per row, 512
+    entries per pass held in registers; deeper rows take several passes and read the row
+    again for the split.  k in [513, 1400] (NN: B's rows; TT: A's rows), some rows all zero,
+    against binary128."""
+    import torch
+    rng = np.random.default_rng(41 + (ta == "T"))
+    probs = _problems(rng, 12, 1, 64, ta, tb, special=False)
+    for i, p in enumerate(probs):
+        k = ddgen.rand_int_log(rng, 513, 1400)
+        ar, ac = (p["m"], k) if ta == "N" else (k, p["m"])
+        br, bc = (k, p["n"]) if tb == "N" else (p["n"], k)
+        p["k"], p["lda"], p["ldb"] = k, ar + 1, br + 2
+        p["A"] = ddgen.real_matrix(rng, ar, ac, ar + 1)
+        p["B"] = ddgen.real_matrix(rng, br, bc, br + 2)
+        if i % 3 == 1:                      # an all-zero row of the row-contiguous operand
+            if tb == "N":
+                p["B"][0, :, :] = 0.0       # B column 0 = row 0 of op(B)^T
+            if ta == "T":
+                p["A"][0, :, :] = 0.0       # A^T row 0
+    dA, dB, dC = _device(probs, gpu)
+    print(f"batched exact product, deep rows {ta}{tb}", flush=True)
+    plan = mwblas.BatchedPlan(ta, tb, *_args(probs, dA, dB, dC), path=mwblas.PATH_OZAKI)
+    plan.run(torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    for i, p in enumerate(probs):
+        C = dC[i].cpu().numpy()
+        r = oracle.q128_gemm_err(False, ta, tb, p["m"], p["n"], p["k"], p["alpha"], p["A"],
+                                 p["lda"], p["B"], p["ldb"], p["beta"], p["C0"], p["ldc"], C,
+                                 p["ldc"])
+        assert r["err"] <= 1e-30, (ta, tb, i, p["m"], p["n"], p["k"], r)
+    plan.close()
+
+
 def test_batched_workload_2000_uneven(gpu):
     """The batched workload: 2000 problems, M,N,K log-uniform in [16,512], NN, alpha=beta=1,
     entries uniform with non-zero low limbs.  Plain plan: 12 sampled problems bitwise against
