@@ -182,29 +182,13 @@ void k_batch_split(const BatchProb *__restrict__ probs, UnitSpace us, int bits,
     const int64_t kg = (blk / rb8) * 8 + w % 8;
     if (r >= R || kg >= G) return;
     const int64_t rows = A_SIDE ? p.Mp : p.Np;
-    oz_split_unit<ddv, false, false>(A_SIDE ? p.A : p.B, A_SIDE ? p.lda : p.ldb, p.k,
-                                     A_SIDE ? p.rA : p.rB, bits, 0, mcs, nmod,
-                                     A_SIDE ? p.resA : p.resB, 1, (int8_t *)nullptr, 0, p.Kc,
-                                     rows * p.Kc, r, kg);
+    oz_split_unit<false>(A_SIDE ? p.A : p.B, A_SIDE ? p.lda : p.ldb, p.k, A_SIDE ? p.rA : p.rB, bits,
+                  mcs, nmod, A_SIDE ? p.resA : p.resB, p.Kc, rows * p.Kc, r, kg);
 }
 
-// Row-contiguous operand (KC): row maximum and residue split in one kernel.  A row takes a
-// group of L lanes (L = kc_lanes(Kc), the least power of two with 8L >= Kc, at most 64), so a
-// wave holds 64/L rows: short rows (the batches' K ~ 16..512) fill the wave instead of leaving
-// most of its lanes idle.  Lane l of a group holds entries k0 + 8l .. k0 + 8l + 7 of each
-// 8L-deep chunk (a group reads its row's 16 * 8L bytes contiguously and writes 8L contiguous
-// bytes per modulus, one 8-byte store per lane).  The row maximum is reduced across the group
-// and stored by its first lane (the row's only writer); the split then uses it exactly as
-// oz_split_unit does.  Rows of up to 512 entries stay in registers between the two passes;
-// longer rows are read again (from L2) for the split.  Entries in [k, Kc) are written as
-// zero residues.
-__host__ __device__ inline int kc_lanes(int64_t Kc)
-{
-    int L = 1;
-    while (L < 64 && 8 * (int64_t)L < Kc) L *= 2;
-    return L;
-}
-
+// Row-contiguous operand (KC): row maximum and residue split in one kernel, 64 / L rows per
+// wave (oz_rowsplit_kc_group; L = kc_lanes(Kc)), so short rows (the batches' K ~ 16..512)
+// fill the wave instead of leaving most of its lanes idle.
 template <bool A_SIDE>
 __global__ __launch_bounds__(256)
 void k_batch_rowsplit_kc(const BatchProb *__restrict__ probs, UnitSpace us, int bits,
@@ -215,64 +199,12 @@ void k_batch_rowsplit_kc(const BatchProb *__restrict__ probs, UnitSpace us, int 
     const int pi = wave_problem(us);
     const BatchProb &p = probs[pi];
     const int64_t R = A_SIDE ? p.m : p.n;
-    const int64_t K = p.k, Kc = p.Kc;
-    const int L = kc_lanes(Kc);
-    const int lane = (int)(threadIdx.x & 63), ll = lane & (L - 1);
+    const int L = kc_lanes(p.Kc);
+    const int lane = (int)(threadIdx.x & 63);
     const int64_t r = ((u - us.start[pi]) >> 6) * (64 / L) + lane / L;
-    const bool live = r < R;            // lanes of a row past the last still join the shuffles
-    const ddv *X = (A_SIDE ? p.A : p.B) + (live ? r : 0) * (A_SIDE ? p.lda : p.ldb);
-    int8_t *out = (A_SIDE ? p.resA : p.resB) + r * Kc;
-    const int64_t mod_stride = (A_SIDE ? p.Mp : p.Np) * Kc;
-    constexpr int E = 8;
-    const int64_t span = (int64_t)E * L;
-    ddv v[E];
-    auto load = [&](int64_t k0) {
-#pragma unroll
-        for (int s = 0; s < E; ++s) {
-            const int64_t kk = k0 + E * ll + s;
-            v[s] = live && kk < K ? X[kk] : ddv{0.0, 0.0};
-        }
-    };
-    // pass 1: NaN-aware maximum of |x| on the bit patterns of non-negative values
-    unsigned long long mx = 0ull;
-    for (int64_t k0 = 0; k0 < K; k0 += span) {
-        load(k0);
-#pragma unroll
-        for (int s = 0; s < E; ++s) {
-            const unsigned long long b = (unsigned long long)__double_as_longlong(mag(v[s]));
-            mx = b > mx ? b : mx;
-        }
-    }
-    for (int off = L / 2; off > 0; off >>= 1) {
-        const unsigned long long o = __shfl_xor(mx, off, 64);
-        mx = o > mx ? o : mx;
-    }
-    if (!live) return;
-    if (ll == 0) (A_SIDE ? p.rA : p.rB)[r] = mx;
-    // pass 2: residues of round(x * 2^sigma)
-    bool bad;
-    const int sigma = row_sigma(mx, bits, bad);
-    for (int64_t k0 = 0; k0 < Kc; k0 += span) {
-        if (K > span) load(k0);          // else the whole row is still in v
-        const int64_t kl = k0 + E * ll;
-        if (kl >= Kc) continue;
-        double c0[E], c1[E], c2[E];
-#pragma unroll
-        for (int s = 0; s < E; ++s) {
-            double hi = 0.0, lo = 0.0;
-            if (!bad) comp(v[s], 0, hi, lo);
-            int_chunks(hi, lo, sigma, c0[s], c1[s], c2[s]);
-        }
-        for (int t = 0; t < nmod; ++t) {
-            const ModConst mc = mcs[t];
-            int res[E];
-#pragma unroll
-            for (int s = 0; s < E; ++s) res[s] = sym_residue(c0[s], c1[s], c2[s], mc);
-            *reinterpret_cast<uint2 *>(out + t * mod_stride + kl) =
-                make_uint2(pack_low_bytes(res[0], res[1], res[2], res[3]),
-                           pack_low_bytes(res[4], res[5], res[6], res[7]));
-        }
-    }
+    oz_rowsplit_kc_group(A_SIDE ? p.A : p.B, A_SIDE ? p.lda : p.ldb, R, p.k, p.Kc, r, L,
+                         lane & (L - 1), A_SIDE ? p.rA : p.rB, bits, mcs, nmod,
+                         A_SIDE ? p.resA : p.resB, (A_SIDE ? p.Mp : p.Np) * p.Kc);
 }
 
 // Work queues of the residue GEMMs, one per XCD.  Each XCD has its own L2.  The plan assigns

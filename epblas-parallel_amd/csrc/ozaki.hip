@@ -245,19 +245,35 @@ void k_oz_rowmax_kc(const T *__restrict__ X, int64_t ld, int64_t R, int64_t K,
     if (lane == 0 && r < R && bits != 0ull) atomicMax(&rmax[r], bits);
 }
 
-// Residue split kernel: one unit (row r, 16-deep k group kg) per lane.  Row-contiguous
-// input (KC): a wave covers 64 k groups of one row.  Column-contiguous input: a wave covers
-// 8 rows x 8 k groups, so the 16-byte residue chunks of one row written by a wave form a
-// whole 128-byte line, and each input load reads eight 128-byte segments.
-template <typename T, bool KC, bool DUAL>
+// Row maxima and residue split of a row-contiguous real operand in one kernel
+// (oz_rowsplit_kc_group): groups of L = kc_lanes(Kc) lanes per row, four waves per block.
+// The operand is read once from memory (a row longer than 8L is read again from L2 for the
+// split), against twice for the separate maxima and split kernels.
 __global__ __launch_bounds__(256)
-void k_oz_split(const T *__restrict__ X, int64_t ld, int64_t R, int64_t K, int64_t kgroups,
-                const unsigned long long *__restrict__ rmax, int bits, int part,
-                const ModConst *__restrict__ mcs, int nmod, int8_t *__restrict__ out1, int s1,
-                int8_t *__restrict__ out2, int s2, int64_t kp, int64_t mod_stride)
+void k_oz_rowsplit_kc(const ddv *__restrict__ X, int64_t ld, int64_t R, int64_t K, int64_t Kc,
+                      int L, unsigned long long *__restrict__ rmax, int bits,
+                      const ModConst *__restrict__ mcs, int nmod, int8_t *__restrict__ out,
+                      int64_t mod_stride)
 {
-    int64_t r, kg;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t r = ((int64_t)blockIdx.x * 4 + w) * (64 / L) + lane / L;
+    oz_rowsplit_kc_group(X, ld, R, K, Kc, r, L, lane & (L - 1), rmax, bits, mcs, nmod, out,
+                         mod_stride);
+}
+
+// Residue split kernel of a real operand: one unit (row r, 16-deep k group kg) per lane.
+// Row-contiguous input (KC): a wave covers 64 k groups of one row.  Column-contiguous input:
+// a wave covers 8 rows x 8 k groups, so the 16-byte residue chunks of one row written by a
+// wave form a whole 128-byte line, and each input load reads eight 128-byte segments.
+template <bool KC>
+__global__ __launch_bounds__(256)
+void k_oz_split(const ddv *__restrict__ X, int64_t ld, int64_t R, int64_t K, int64_t kgroups,
+                const unsigned long long *__restrict__ rmax, int bits,
+                const ModConst *__restrict__ mcs, int nmod, int8_t *__restrict__ out, int64_t kp,
+                int64_t mod_stride)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int64_t r, kg;
     if (KC) {
         kg = (int64_t)blockIdx.y * 64 + lane;
         r = (int64_t)blockIdx.x * 4 + w;
@@ -266,8 +282,7 @@ void k_oz_split(const T *__restrict__ X, int64_t ld, int64_t R, int64_t K, int64
         kg = (int64_t)blockIdx.y * 8 + (lane & 7);
     }
     if (r >= R || kg >= kgroups) return;
-    oz_split_unit<T, KC, DUAL>(X, ld, K, rmax, bits, part, mcs, nmod, out1, s1, out2, s2, kp,
-                               mod_stride, r, kg);
+    oz_split_unit<KC>(X, ld, K, rmax, bits, mcs, nmod, out, kp, mod_stride, r, kg);
 }
 
 // Complex residue split, three-product form: one unit (row r, 8-deep k group kq) per lane.
@@ -530,21 +545,20 @@ hipError_t launch_rowmax(const T *X, int64_t ld, int64_t R, int64_t K, bool kc,
     return hipGetLastError();
 }
 
-template <typename T, bool DUAL>
-hipError_t launch_split_t(const T *X, int64_t ld, int64_t R, int64_t K, int64_t kc_len, bool kc,
-                          const unsigned long long *rmax, int bits, int part, const ModConst *mcs,
-                          int nmod, int8_t *out1, int s1, int8_t *out2, int s2, int64_t kp,
-                          int64_t mod_stride, hipStream_t s)
+// Residues of a real operand (kc: rows contiguous in k) into out.
+hipError_t launch_split(const ddv *X, int64_t ld, int64_t R, int64_t K, int64_t kc_len, bool kc,
+                        const unsigned long long *rmax, int bits, const ModConst *mcs, int nmod,
+                        int8_t *out, int64_t kp, int64_t mod_stride, hipStream_t s)
 {
     const int64_t kgroups = kc_len / 16;
     if (kc) {
         dim3 grid((unsigned)((R + 3) / 4), (unsigned)((kgroups + 63) / 64));
-        hipLaunchKernelGGL((k_oz_split<T, true, DUAL>), grid, dim3(256), 0, s, X, ld, R, K,
-                           kgroups, rmax, bits, part, mcs, nmod, out1, s1, out2, s2, kp, mod_stride);
+        hipLaunchKernelGGL(k_oz_split<true>, grid, dim3(256), 0, s, X, ld, R, K, kgroups, rmax,
+                           bits, mcs, nmod, out, kp, mod_stride);
     } else {
         dim3 grid((unsigned)((R + 31) / 32), (unsigned)((kgroups + 7) / 8));
-        hipLaunchKernelGGL((k_oz_split<T, false, DUAL>), grid, dim3(256), 0, s, X, ld, R, K,
-                           kgroups, rmax, bits, part, mcs, nmod, out1, s1, out2, s2, kp, mod_stride);
+        hipLaunchKernelGGL(k_oz_split<false>, grid, dim3(256), 0, s, X, ld, R, K, kgroups, rmax,
+                           bits, mcs, nmod, out, kp, mod_stride);
     }
     return hipGetLastError();
 }
@@ -568,19 +582,17 @@ hipError_t launch_split_k3(const zddv *X, int64_t ld, int64_t R, int64_t K, int6
     return hipGetLastError();
 }
 
-// Residues of one operand (or one part of a complex operand) into out1 (signed by s1) and,
-// when out2 is given, also into out2 (signed by s2).
-template <typename T>
-hipError_t launch_split(const T *X, int64_t ld, int64_t R, int64_t K, int64_t kc_len, bool kc,
-                        const unsigned long long *rmax, int bits, int part, const ModConst *mcs,
-                        int nmod, int8_t *out1, int s1, int8_t *out2, int s2, int64_t kp,
-                        int64_t mod_stride, hipStream_t s)
+// Row maxima and residues of a row-contiguous real operand in one kernel.
+hipError_t launch_rowsplit_kc(const ddv *X, int64_t ld, int64_t R, int64_t K, int64_t Kc,
+                              unsigned long long *rmax, int bits, const ModConst *mcs, int nmod,
+                              int8_t *out, int64_t mod_stride, hipStream_t s)
 {
-    if (out2)
-        return launch_split_t<T, true>(X, ld, R, K, kc_len, kc, rmax, bits, part, mcs, nmod, out1,
-                                       s1, out2, s2, kp, mod_stride, s);
-    return launch_split_t<T, false>(X, ld, R, K, kc_len, kc, rmax, bits, part, mcs, nmod, out1, s1,
-                                    out2, s2, kp, mod_stride, s);
+    if (R <= 0) return hipSuccess;
+    const int L = kc_lanes(Kc);
+    const int64_t rows_per_block = 4 * (64 / L);
+    hipLaunchKernelGGL(k_oz_rowsplit_kc, dim3((unsigned)((R + rows_per_block - 1) / rows_per_block)),
+                       dim3(256), 0, s, X, ld, R, K, Kc, L, rmax, bits, mcs, nmod, out, mod_stride);
+    return hipGetLastError();
 }
 
 template <int TM, int TN>
@@ -684,22 +696,38 @@ int oz_enqueue_real(const GemmArgsR &g, const OzPlan &plan, const ModConst *mcs,
     uint8_t *Ub = reinterpret_cast<uint8_t *>(base + L.oU);
     const bool a_kc = g.opa != OP_N;   // op(A) row i contiguous in k when A is transposed
     const bool b_kc = g.opb == OP_N;   // op(B) column j contiguous in k when B is not
+    // A row-contiguous operand of depth Kc <= 512 takes its maxima and split in one kernel
+    // (the row stays in registers between them; timed with the splits).  Longer rows, and
+    // column-contiguous operands, take a maxima kernel, then a split kernel: the fused one
+    // would read a long row twice (measured at 8192^3: 1.83 against 1.75 ms for the pair).
+    const bool a_fused = a_kc && Kc <= 512, b_fused = b_kc && Kc <= 512;
     {
         ScopedTimer tm(KC_OZ_EXPONENT, s);
-        if (!reuse_a) {
+        if (!reuse_a && !a_fused) {
             OZ_TRY(fill_zero(rA, Mp, s));
             OZ_TRY(launch_rowmax(g.A, g.lda, g.m, g.k, a_kc, rA, s));
         }
-        OZ_TRY(fill_zero(rB, Np, s));
-        OZ_TRY(launch_rowmax(g.B, g.ldb, g.n, g.k, b_kc, rB, s));
+        if (!b_fused) {
+            OZ_TRY(fill_zero(rB, Np, s));
+            OZ_TRY(launch_rowmax(g.B, g.ldb, g.n, g.k, b_kc, rB, s));
+        }
     }
     {
         ScopedTimer tm(KC_OZ_SPLIT, s);
-        if (!reuse_a)
-            OZ_TRY(launch_split(g.A, g.lda, g.m, g.k, Kc, a_kc, rA, plan.bits, 0, mcs, nmod, resA,
-                                1, (int8_t *)nullptr, 0, Kc, Mp * Kc, s));
-        OZ_TRY(launch_split(g.B, g.ldb, g.n, g.k, Kc, b_kc, rB, plan.bits, 0, mcs, nmod, resB, 1,
-                            (int8_t *)nullptr, 0, Kc, Np * Kc, s));
+        if (!reuse_a) {
+            if (a_fused)
+                OZ_TRY(launch_rowsplit_kc(g.A, g.lda, g.m, g.k, Kc, rA, plan.bits, mcs, nmod, resA,
+                                          Mp * Kc, s));
+            else
+                OZ_TRY(launch_split(g.A, g.lda, g.m, g.k, Kc, a_kc, rA, plan.bits, mcs, nmod, resA,
+                                    Kc, Mp * Kc, s));
+        }
+        if (b_fused)
+            OZ_TRY(launch_rowsplit_kc(g.B, g.ldb, g.n, g.k, Kc, rB, plan.bits, mcs, nmod, resB,
+                                      Np * Kc, s));
+        else
+            OZ_TRY(launch_split(g.B, g.ldb, g.n, g.k, Kc, b_kc, rB, plan.bits, mcs, nmod, resB, Kc,
+                                Np * Kc, s));
     }
     {
         ScopedTimer tm(KC_OZ_GEMM, s);

@@ -73,35 +73,14 @@ DDI uint32_t pack_low_bytes(int v0, int v1, int v2, int v3)
     return __builtin_amdgcn_perm(hi, lo, 0x05040100u);
 }
 
-// Bytewise negation modulo 256 of a packed word: 0x80 - (b & 0x7f) never borrows from the
-// next byte, and flipping bit 7 where b's bit 7 is clear turns it into -b.
-DDI uint32_t neg_bytes(uint32_t x)
-{
-    return (0x80808080u - (x & 0x7f7f7f7fu)) ^ (~x & 0x80808080u);
-}
-
-DDI void comp(const ddv &x, int part, double &hi, double &lo)
-{
-    (void)part;
-    hi = x.hi;
-    lo = x.lo;
-}
-DDI void comp(const zddv &x, int part, double &hi, double &lo)
-{
-    const ddv &c = part ? x.im : x.re;
-    hi = c.hi;
-    lo = c.lo;
-}
-
-// Residue split of one lane's unit: row r, k in [16*kg, 16*kg+16), all moduli.
-// out1[t*mod_stride + r*kp + k] = s1 * (residue of entry (r,k)), likewise out2 (optional).
-// KC: entry (r,k) at X[r*ld + k]; otherwise at X[k*ld + r].
-template <typename T, bool KC, bool DUAL>
-DDI void oz_split_unit(const T *__restrict__ X, int64_t ld, int64_t K,
-                       const unsigned long long *__restrict__ rmax, int bits, int part,
-                       const ModConst *__restrict__ mcs, int nmod, int8_t *__restrict__ out1,
-                       int s1, int8_t *__restrict__ out2, int s2, int64_t kp, int64_t mod_stride,
-                       int64_t r, int64_t kg)
+// Residue split of one lane's unit of a real operand: row r, k in [16*kg, 16*kg+16), all
+// moduli, out[t*mod_stride + r*kp + k].  KC: entry (r,k) at X[r*ld + k] (row-contiguous);
+// otherwise at X[k*ld + r].
+template <bool KC>
+DDI void oz_split_unit(const ddv *__restrict__ X, int64_t ld, int64_t K,
+                       const unsigned long long *__restrict__ rmax, int bits,
+                       const ModConst *__restrict__ mcs, int nmod, int8_t *__restrict__ out,
+                       int64_t kp, int64_t mod_stride, int64_t r, int64_t kg)
 {
     bool bad;
     const int sigma = row_sigma(rmax[r], bits, bad);
@@ -110,36 +89,99 @@ DDI void oz_split_unit(const T *__restrict__ X, int64_t ld, int64_t K,
     for (int s = 0; s < 16; ++s) {
         const int64_t k = kg * 16 + s;
         double hi = 0.0, lo = 0.0;
-        if (k < K && !bad) comp(KC ? X[r * ld + k] : X[k * ld + r], part, hi, lo);
+        if (k < K && !bad) {
+            const ddv x = KC ? X[r * ld + k] : X[k * ld + r];
+            hi = x.hi;
+            lo = x.lo;
+        }
         int_chunks(hi, lo, sigma, c0[s], c1[s], c2[s]);
     }
-    // The sign s1 is applied to the chunks once; out2 differs from out1 by a bytewise
-    // negation when the signs differ.  Every modulus but 256 is odd, so the symmetric
-    // residue of -x is minus that of x; for 256 both are -x mod 256 as a byte.
-    if (s1 < 0) {
-#pragma unroll
-        for (int s = 0; s < 16; ++s) {
-            c0[s] = -c0[s];
-            c1[s] = -c1[s];
-            c2[s] = -c2[s];
-        }
-    }
-    const bool flip = (s1 < 0) != (s2 < 0);
     for (int t = 0; t < nmod; ++t) {
         const ModConst mc = mcs[t];
         int res[16];
 #pragma unroll
         for (int s = 0; s < 16; ++s) res[s] = sym_residue(c0[s], c1[s], c2[s], mc);
-        uint32_t w1[4], w2[4];
-        (void)w2;
+        uint32_t w[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            w1[q] = pack_low_bytes(res[4 * q], res[4 * q + 1], res[4 * q + 2], res[4 * q + 3]);
-            if constexpr (DUAL) w2[q] = flip ? neg_bytes(w1[q]) : w1[q];
+        for (int q = 0; q < 4; ++q)
+            w[q] = pack_low_bytes(res[4 * q], res[4 * q + 1], res[4 * q + 2], res[4 * q + 3]);
+        *reinterpret_cast<uint4 *>(out + t * mod_stride + r * kp + kg * 16) =
+            make_uint4(w[0], w[1], w[2], w[3]);
+    }
+}
+
+// Row maximum and residue split of a row-contiguous operand (entry (r,k) at X[r*ld + k]) in
+// one pass over the row, for a group of L lanes per row (L a power of two dividing 64; the
+// least with 8L >= Kc for short rows, see kc_lanes).  Lane ll of the group holds entries
+// k0 + 8ll .. k0 + 8ll + 7 of each 8L-deep chunk: the group reads 16 * 8L contiguous bytes per
+// chunk and writes 8L contiguous bytes per modulus, one 8-byte store per lane.  The row
+// maximum (|hi| + |lo| as the bit pattern of a non-negative double, so a NaN or Inf
+// dominates) is reduced across the group and stored by its first lane, the row's only writer
+// (rmax needs no clearing); the residues then follow exactly as in oz_split_unit.  Rows of
+// at most 8L entries stay in registers between the two passes; longer ones are read again.
+// Entries in [K, Kc) get zero residues.  Every lane of the wave calls this (lanes of a row
+// r >= R join the group shuffles only).
+__host__ __device__ inline int kc_lanes(int64_t Kc)
+{
+    int L = 1;
+    while (L < 64 && 8 * (int64_t)L < Kc) L *= 2;
+    return L;
+}
+
+DDI void oz_rowsplit_kc_group(const ddv *__restrict__ X, int64_t ld, int64_t R, int64_t K,
+                              int64_t Kc, int64_t r, int L, int ll,
+                              unsigned long long *__restrict__ rmax, int bits,
+                              const ModConst *__restrict__ mcs, int nmod,
+                              int8_t *__restrict__ out, int64_t mod_stride)
+{
+    const bool live = r < R;
+    const ddv *row = X + (live ? r : 0) * ld;
+    constexpr int E = 8;
+    const int64_t span = (int64_t)E * L;
+    ddv v[E];
+    auto load = [&](int64_t k0) {
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            const int64_t kk = k0 + E * ll + s;
+            v[s] = live && kk < K ? row[kk] : ddv{0.0, 0.0};
         }
-        const int64_t off = t * mod_stride + r * kp + kg * 16;
-        *reinterpret_cast<uint4 *>(out1 + off) = make_uint4(w1[0], w1[1], w1[2], w1[3]);
-        if constexpr (DUAL) *reinterpret_cast<uint4 *>(out2 + off) = make_uint4(w2[0], w2[1], w2[2], w2[3]);
+    };
+    unsigned long long mx = 0ull;
+    for (int64_t k0 = 0; k0 < K; k0 += span) {
+        load(k0);
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            const unsigned long long b = (unsigned long long)__double_as_longlong(mag(v[s]));
+            mx = b > mx ? b : mx;
+        }
+    }
+    for (int off = L / 2; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(mx, off, 64);
+        mx = o > mx ? o : mx;
+    }
+    if (!live) return;
+    if (ll == 0) rmax[r] = mx;
+    bool bad;
+    const int sigma = row_sigma(mx, bits, bad);
+    int8_t *o = out + r * Kc;
+    for (int64_t k0 = 0; k0 < Kc; k0 += span) {
+        if (K > span) load(k0);          // else the whole row is still in v
+        const int64_t kl = k0 + E * ll;
+        if (kl >= Kc) continue;
+        double c0[E], c1[E], c2[E];
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            int_chunks(bad ? 0.0 : v[s].hi, bad ? 0.0 : v[s].lo, sigma, c0[s], c1[s], c2[s]);
+        }
+        for (int t = 0; t < nmod; ++t) {
+            const ModConst mc = mcs[t];
+            int res[E];
+#pragma unroll
+            for (int s = 0; s < E; ++s) res[s] = sym_residue(c0[s], c1[s], c2[s], mc);
+            *reinterpret_cast<uint2 *>(o + t * mod_stride + kl) =
+                make_uint2(pack_low_bytes(res[0], res[1], res[2], res[3]),
+                           pack_low_bytes(res[4], res[5], res[6], res[7]));
+        }
     }
 }
 
