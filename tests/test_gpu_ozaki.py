@@ -179,6 +179,35 @@ def test_8192_wide_exponents_vs_binary128_and_plain(gpu):
         assert r["err"] <= BAR
 
 
+@pytest.mark.parametrize("m,n,k,ta,tb", [(4096, 4096, 1000, "N", "N"), (4352, 4096, 256, "T", "N"),
+                                         (4096, 4608, 2048, "N", "T")])
+def test_ring_grouped_moduli_vs_binary128(gpu, m, n, k, ta, tb):
+    """Real products of depth <= 2048 on the 256 x 256 ring tiles run groups of 8 moduli per
+    workgroup as one ring sequence (29 = 8 + 8 + 8 + 5).  Shapes large enough for the ring
+    tiles, ragged rows, both transposes (at depth 256 the row-contiguous op(A) of 'T' and
+    op(B) of 'N' take maxima and split in one kernel); 192 sampled entries against
+    binary128."""
+    import torch
+    rng = np.random.default_rng(m + n + k)
+    ar, ac = (m, k) if ta == "N" else (k, m)
+    br, bc = (k, n) if tb == "N" else (n, k)
+    A, B = ddgen.real_matrix(rng, ar, ac, kind="wide"), ddgen.real_matrix(rng, br, bc, kind="wide")
+    C0 = ddgen.real_matrix(rng, m, n)
+    al, be = ddgen.real_scalar(rng, scale=2), ddgen.real_scalar(rng, scale=2)
+    dA, dB = torch.from_numpy(A).to(gpu), torch.from_numpy(B).to(gpu)
+    dC = torch.from_numpy(C0).to(gpu)
+    print(f"ring grouped moduli {m}x{n}x{k} {ta}{tb}", flush=True)
+    mwblas.gemm_device(False, ta, tb, m, n, k, al, dA, ar, dB, br, be, dC, m,
+                       path=mwblas.PATH_OZAKI)
+    torch.cuda.synchronize()
+    C = dC.cpu().numpy()
+    ent = (rng.integers(0, m, 192), rng.integers(0, n, 192))
+    r = oracle.q128_gemm_err(False, ta, tb, m, n, k, al, A, ar, B, br, be, C0, m, C, m,
+                             entries=ent)
+    print(f"  err={r['err']:.3e}", flush=True)
+    assert r["err"] <= BAR
+
+
 def test_4096_complex_conj_vs_binary128(gpu):
     """Complex workload: M=N=K=4096, op(A) = conjugate transpose, op(B) = N, random complex
     DD alpha/beta; plain and Ozaki, 128 sampled entries against binary128."""
