@@ -369,28 +369,34 @@ void k_batch_oz_fused(const BatchProb *__restrict__ probs, const int4 *__restric
     }
 }
 
-// Reconstruction: unit = (column j, 4-row group); inactive problems get the pre-pass only.
+// Reconstruction: 256 units (one workgroup) per 32 x 32 block of C (oz_crt_block), blocks
+// column block by column block; every problem's unit count is a multiple of 256, so a
+// workgroup never spans two problems.  Inactive problems get the beta pre-pass only.
 __global__ __launch_bounds__(256)
 void k_batch_crt(const BatchProb *__restrict__ probs, UnitSpace us, int nmod,
                  const ModConst *__restrict__ mcs, double Mhi, double Mlo, int bits)
 {
+    __shared__ ddv stage[32][33];
     const int64_t u = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (u >= us.start[us.count]) return;
+    if (u >= us.start[us.count]) return;          // whole workgroups
     const int pi = wave_problem(us);
     const BatchProb &p = probs[pi];
-    const int64_t groups = (p.m + 3) / 4;
-    const int64_t local = u - us.start[pi];
-    if (local >= p.n * groups) return;   // padding units
-    const int64_t j = local / groups, i0 = (local % groups) * 4;
+    const int64_t blk = (u - us.start[pi]) >> 8, mb = (p.m + 31) / 32;
+    const int64_t bi = blk % mb, bj = blk / mb;
     if (!p.active) {
-        for (int e = 0; e < 4 && i0 + e < p.m; ++e) {
-            ddv *cp = &p.C[j * p.ldc + i0 + e];
-            *cp = dd_beta_prepass(*cp, p.beta, p.beta_mode);
+#pragma unroll
+        for (int it = 0; it < 4; ++it) {
+            const int idx = threadIdx.x + 256 * it;
+            const int64_t i = bi * 32 + (idx & 31), j = bj * 32 + (idx >> 5);
+            if (i < p.m && j < p.n) {
+                ddv *cp = &p.C[j * p.ldc + i];
+                *cp = dd_beta_prepass(*cp, p.beta, p.beta_mode);
+            }
         }
         return;
     }
-    oz_crt_unit<false>(p.U, p.Mp, p.Np * p.Mp, nmod, mcs, Mhi, Mlo, p.rA, p.rB, bits, p.m,
-                       p.alpha, p.beta, p.beta_mode, p.C, p.ldc, j, i0);
+    oz_crt_block<false>(p.U, p.Mp, p.Np * p.Mp, nmod, mcs, Mhi, Mlo, p.rA, p.rB, bits, p.m,
+                        p.n, p.alpha, p.beta, p.beta_mode, p.C, p.ldc, bi, bj, stage);
 }
 
 // ---------------------------------------------------------------------------------
@@ -660,7 +666,7 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
         }
         // the reconstruction kernel's units; with the fused kernel only inactive problems
         // need it (their beta pre-pass)
-        cr[p + 1] = empty || (P->fused && q.active) ? 0 : rup(q.n * ((q.m + 3) / 4), 64);
+        cr[p + 1] = empty || (P->fused && q.active) ? 0 : ((q.m + 31) / 32) * ((q.n + 31) / 32) * 256;
         if (empty || !q.active) continue;
         // Residue rows are padded to 32 for the ring tiles (they clamp their loads and stores
         // to the rows that exist), to 64 for the 64-row register-staged tiles.

@@ -364,45 +364,23 @@ void k_oz_gemm_ring(const int8_t *__restrict__ A, const int8_t *__restrict__ B,
     }
 }
 
-// Reconstruction kernel: a workgroup owns 1024 consecutive rows of one column.  Each lane
-// accumulates 4 consecutive rows (one 32-bit load of residues per modulus), the products
-// are staged in LDS, and the C entries are then read (beta != 0) and written with
-// consecutive lanes on consecutive rows.
+// Reconstruction kernel: one 32 x 32 block of C per workgroup (oz_crt_block), blocks column
+// block by column block.
 template <bool CPLX>
 __global__ __launch_bounds__(256)
 void k_oz_crt(const uint8_t *__restrict__ U, int64_t ldu, int64_t sU, int nmod,
               const ModConst *__restrict__ mcs, double Mhi, double Mlo,
               const unsigned long long *__restrict__ rmaxA,
               const unsigned long long *__restrict__ rmaxB, int bits, int64_t m, int64_t n,
-              int64_t blocks_per_col, typename std::conditional<CPLX, zddv, ddv>::type alpha,
+              int64_t mblocks, typename std::conditional<CPLX, zddv, ddv>::type alpha,
               typename std::conditional<CPLX, zddv, ddv>::type beta, int beta_mode,
               typename std::conditional<CPLX, zddv, ddv>::type *__restrict__ C, int64_t ldc)
 {
     using V = typename std::conditional<CPLX, zddv, ddv>::type;
-    __shared__ V stage[1024];
-    const int64_t j = (int64_t)blockIdx.x / blocks_per_col;
-    const int64_t r0 = ((int64_t)blockIdx.x % blocks_per_col) * 1024;
-    const int64_t i0 = r0 + threadIdx.x * 4;
-    if (i0 < m) {
-        double acc[4][CPLX ? 2 : 1][4];
-        oz_crt_accumulate<CPLX, 4>(U, ldu, sU, nmod, mcs, j, i0, acc);
-        bool badj;
-        const int tau = row_sigma(rmaxB[j], bits, badj);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int64_t i = i0 + e < m ? i0 + e : m - 1;
-            bool badi;
-            const int sigma = row_sigma(rmaxA[i], bits, badi);
-            stage[threadIdx.x * 4 + e] =
-                oz_crt_value(acc[e], alpha, Mhi, Mlo, -128 - sigma - tau, badi || badj);
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        const int64_t i = r0 + threadIdx.x + 256 * e;
-        if (i < m) oz_crt_store(&C[j * ldc + i], stage[threadIdx.x + 256 * e], beta, beta_mode);
-    }
+    __shared__ V stage[32][33];
+    const int64_t bi = (int64_t)blockIdx.x % mblocks, bj = (int64_t)blockIdx.x / mblocks;
+    oz_crt_block<CPLX>(U, ldu, sU, nmod, mcs, Mhi, Mlo, rmaxA, rmaxB, bits, m, n, alpha, beta,
+                       beta_mode, C, ldc, bi, bj, stage);
 }
 
 // ==================================================================================
@@ -761,10 +739,11 @@ int oz_enqueue_real(const GemmArgsR &g, const OzPlan &plan, const ModConst *mcs,
     }
     {
         ScopedTimer tm(KC_OZ_CRT, s);
-        const int64_t bpc = (g.m + 1023) / 1024;
-        hipLaunchKernelGGL((k_oz_crt<false>), dim3((unsigned)(bpc * g.n)), dim3(256), 0, s, Ub, Mp,
+        const int64_t mb = (g.m + 31) / 32, nb = (g.n + 31) / 32;
+        if (mb * nb > (int64_t)UINT32_MAX) return record_error(-3, "ozaki: C too large");
+        hipLaunchKernelGGL((k_oz_crt<false>), dim3((unsigned)(mb * nb)), dim3(256), 0, s, Ub, Mp,
                            Np * Mp, nmod, mcs, plan.Mhi, plan.Mlo, rA, rB, plan.bits, g.m, g.n,
-                           bpc, g.alpha, g.beta, g.beta_mode, g.C, g.ldc);
+                           mb, g.alpha, g.beta, g.beta_mode, g.C, g.ldc);
         OZ_TRY(hipGetLastError());
     }
     return 0;
@@ -934,10 +913,11 @@ int ozaki_gemm_cplx(const GemmArgsZ &g, hipStream_t s, uint64_t a_tag)
     }
     {
         ScopedTimer tm(KC_OZ_CRT, s);
-        const int64_t bpc = (g.m + 1023) / 1024;
-        hipLaunchKernelGGL((k_oz_crt<true>), dim3((unsigned)(bpc * g.n)), dim3(256), 0, s, Ub, Mp,
+        const int64_t mb = (g.m + 31) / 32, nb = (g.n + 31) / 32;
+        if (mb * nb > (int64_t)UINT32_MAX) return record_error(-3, "ozaki: C too large");
+        hipLaunchKernelGGL((k_oz_crt<true>), dim3((unsigned)(mb * nb)), dim3(256), 0, s, Ub, Mp,
                            Np * Mp, nmod, mcs, plan.Mhi, plan.Mlo, rA, rB, plan.bits, g.m, g.n,
-                           bpc, g.alpha, g.beta, g.beta_mode, g.C, g.ldc);
+                           mb, g.alpha, g.beta, g.beta_mode, g.C, g.ldc);
         OZ_TRY(hipGetLastError());
     }
     return oz_workspace_done(d, s);

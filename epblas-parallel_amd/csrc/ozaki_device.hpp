@@ -259,6 +259,20 @@ DDI int fold_residue(int v, float inv_m, int m)
     return v - q * m;
 }
 
+// Layout of the reduced products U (one uint8 per entry and product, rows and columns padded
+// to multiples of 32, ldu = padded rows): the 32 x 32 blocks of an MFMA output tile, column
+// block by column block, each block 1 KB in the order the v_mfma_i32_32x32x32_i8 accumulator
+// holds it.  Word g*64 + l of a block (g = 0..3, lane l = 0..63) packs rows 8g + 4(l >> 5)
+// .. +3 of column l & 31, so a wave stores 256 contiguous bytes per instruction (a
+// column-major U took 32 separate 8-byte pieces per instruction, ~0.65 ms of the 8192^3 call).
+// Byte offset of the word holding rows i .. i+3 (i a multiple of 4) of column j:
+DDI int64_t u_word_offset(int64_t i, int64_t j, int64_t ldu)
+{
+    const int r = (int)(i & 31);
+    return ((j >> 5) * (ldu >> 5) + (i >> 5)) * 1024 + (r >> 3) * 256 +
+           ((j & 31) + 32 * ((r >> 2) & 1)) * 4;
+}
+
 // Workgroup -> output tile.  Workgroups are dispatched round-robin over the 8 XCDs (each
 // with its own L2), so ids are first regrouped so that one XCD receives a contiguous run;
 // runs are then walked in bands of GROUP tile-rows, column by column, so the workgroups
@@ -347,7 +361,7 @@ DDI void oz_gemm_tile(const int8_t *__restrict__ Ab, const int8_t *__restrict__ 
 #pragma unroll
                     for (int q = 0; q < 4; ++q)
                         word |= crt_residue(acc[a][b][4 * g + q], mc) << (8 * q);
-                    *reinterpret_cast<uint32_t *>(U + col * ldu + rowb + 8 * g) = word;
+                    *reinterpret_cast<uint32_t *>(U + u_word_offset(rowb + 8 * g, col, ldu)) = word;
                 }
             }
     };
@@ -625,7 +639,7 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
 #pragma unroll
                     for (int q = 0; q < 4; ++q)
                         word |= crt_residue(acc[a][b][4 * g + q], mc) << (8 * q);
-                    *reinterpret_cast<uint32_t *>(U + col * ldu + rowb + 8 * g) = word;
+                    *reinterpret_cast<uint32_t *>(U + u_word_offset(rowb + 8 * g, col, ldu)) = word;
                 }
             }
     };
@@ -785,7 +799,8 @@ DDI void oz_crt_accumulate(const uint8_t *__restrict__ U, int64_t ldu, int64_t s
     auto load = [&](int t, uint32_t (&word)[NQ]) {
 #pragma unroll
         for (int q = 0; q < NQ; ++q)
-            word[q] = *reinterpret_cast<const uint32_t *>(U + (int64_t)(t * NQ + q) * sU + j * ldu + i0);
+            word[q] = *reinterpret_cast<const uint32_t *>(U + (int64_t)(t * NQ + q) * sU +
+                                                          u_word_offset(i0, j, ldu));
     };
     // The residue words of TB moduli are loaded before any of their FMAs, so each lane
     // keeps TB*NQ loads in flight.  The limb sums are exact integers, so the grouping does
@@ -833,30 +848,47 @@ DDI void oz_crt_store(zddv *cp, zddv v, zddv beta, int beta_mode)
     *cp = zdd_add_exact(c0, v);
 }
 
-// Reconstruction of 4 consecutive rows i0.. of column j, stored directly.
+// Reconstruction of one 32 x 32 block (bi, bj) of C by a workgroup of 256 lanes.  Wave w,
+// lane l take rows 32bi + 8w + 4(l >> 5) .. +3 of column 32bj + (l & 31): their U words
+// (u_word_offset) are 256 contiguous bytes per wave and product.  The values alpha * (A B^T)
+// are staged in LDS (pitch 33 entries: the column-strided writes spread over the banks), then
+// C is read (beta != 0) and written with consecutive lanes on consecutive rows, 32 rows of a
+// column contiguous.  Every lane of the workgroup must call this (it holds a barrier).
 template <bool CPLX>
-DDI void oz_crt_unit(const uint8_t *__restrict__ U, int64_t ldu, int64_t sU, int nmod,
-                     const ModConst *__restrict__ mcs, double Mhi, double Mlo,
-                     const unsigned long long *__restrict__ rmaxA,
-                     const unsigned long long *__restrict__ rmaxB, int bits, int64_t m,
-                     typename std::conditional<CPLX, zddv, ddv>::type alpha,
-                     typename std::conditional<CPLX, zddv, ddv>::type beta, int beta_mode,
-                     typename std::conditional<CPLX, zddv, ddv>::type *__restrict__ C,
-                     int64_t ldc, int64_t j, int64_t i0)
+DDI void oz_crt_block(const uint8_t *__restrict__ U, int64_t ldu, int64_t sU, int nmod,
+                      const ModConst *__restrict__ mcs, double Mhi, double Mlo,
+                      const unsigned long long *__restrict__ rmaxA,
+                      const unsigned long long *__restrict__ rmaxB, int bits, int64_t m,
+                      int64_t n, typename std::conditional<CPLX, zddv, ddv>::type alpha,
+                      typename std::conditional<CPLX, zddv, ddv>::type beta, int beta_mode,
+                      typename std::conditional<CPLX, zddv, ddv>::type *__restrict__ C,
+                      int64_t ldc, int64_t bi, int64_t bj,
+                      typename std::conditional<CPLX, zddv, ddv>::type (&stage)[32][33])
 {
-    double acc[4][CPLX ? 2 : 1][4];
-    oz_crt_accumulate<CPLX, 4>(U, ldu, sU, nmod, mcs, j, i0, acc);
-    bool badj;
-    const int tau = row_sigma(rmaxB[j], bits, badj);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int64_t j = bj * 32 + (l & 31);
+    const int r0 = 8 * w + 4 * (l >> 5);
+    const int64_t i0 = bi * 32 + r0;
+    if (i0 < m && j < n) {
+        double acc[4][CPLX ? 2 : 1][4];
+        oz_crt_accumulate<CPLX, 4>(U, ldu, sU, nmod, mcs, j, i0, acc);
+        bool badj;
+        const int tau = row_sigma(rmaxB[j], bits, badj);
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        const int64_t i = i0 + e;
-        if (i >= m) break;
-        bool badi;
-        const int sigma = row_sigma(rmaxA[i], bits, badi);
-        oz_crt_store(&C[j * ldc + i],
-                     oz_crt_value(acc[e], alpha, Mhi, Mlo, -128 - sigma - tau, badi || badj),
-                     beta, beta_mode);
+        for (int e = 0; e < 4; ++e) {
+            const int64_t i = i0 + e < m ? i0 + e : m - 1;
+            bool badi;
+            const int sigma = row_sigma(rmaxA[i], bits, badi);
+            stage[l & 31][r0 + e] =
+                oz_crt_value(acc[e], alpha, Mhi, Mlo, -128 - sigma - tau, badi || badj);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+        const int idx = threadIdx.x + 256 * it, c = idx >> 5, r = idx & 31;
+        const int64_t i = bi * 32 + r, jj = bj * 32 + c;
+        if (i < m && jj < n) oz_crt_store(&C[jj * ldc + i], stage[c][r], beta, beta_mode);
     }
 }
 

@@ -417,6 +417,18 @@ def testing_residue_gemm(dA, dB, dU, mp, np_, kp, mod_index, variant=1, stream=N
         _check(rc)
 
 
+def residue_products(U, mp, np_):
+    """The reduced residue products written by testing_residue_gemm as an (np_, mp) array
+    (column j, row i).  The library stores them in 32 x 32 blocks in MFMA accumulator order:
+    blocks column block by column block (ldu = mp), within a block word g*64 + l (g = 0..3,
+    lane l = 0..63) packing rows 8g + 4(l >> 5) .. +3 of column l & 31, one byte per row."""
+    import numpy as np
+    u = np.asarray(U).reshape(-1)
+    assert mp % 32 == 0 and np_ % 32 == 0 and u.size >= mp * np_
+    blk = u[:mp * np_].reshape(np_ // 32, mp // 32, 4, 2, 32, 4)   # bj, bi, g, half, col, q
+    return blk.transpose(0, 4, 1, 2, 3, 5).reshape(np_, mp)
+
+
 def testing_set_alloc_limit(nbytes):
     """Test hook: every device buffer the library would allocate for itself above nbytes
     counts as unobtainable (0 lifts the limit).  Returns the limit that was set."""

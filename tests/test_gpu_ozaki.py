@@ -43,7 +43,7 @@ def test_residue_gemm_kernel_is_exact(gpu, mod_index, variant, kp):
     mwblas.testing_residue_gemm(dA, dB, dU, mp, np_, kp, mod_index, variant)
     torch.cuda.synchronize()
     expect = (A @ B.T) % m                        # (mp, np); y lives in the CRT limbs
-    got = dU.cpu().numpy().astype(np.int64).T
+    got = mwblas.residue_products(dU.cpu().numpy(), mp, np_).astype(np.int64).T
     assert np.array_equal(got, expect)
 
 
@@ -252,7 +252,8 @@ def test_residue_gemm_deeper_than_one_accumulation(gpu, variant, kp):
     mwblas.testing_residue_gemm(dA, dB, dU, mp, np_, kp, mod_index, variant)
     torch.cuda.synchronize()
     expect = (A.astype(np.float64) @ B.T.astype(np.float64)).astype(np.int64) % m
-    assert np.array_equal(dU.cpu().numpy().astype(np.int64).T, expect)
+    got = mwblas.residue_products(dU.cpu().numpy(), mp, np_).astype(np.int64).T
+    assert np.array_equal(got, expect)
 
 
 @pytest.mark.parametrize("variant", [0, 1])
@@ -270,7 +271,7 @@ def test_residue_gemm_sum_beyond_int32(gpu, variant):
     print(f"residue gemm variant {variant}, constant +-127 operands, kp {kp}", flush=True)
     mwblas.testing_residue_gemm(dA, dB, dU, mp, np_, kp, mod_index, variant)
     torch.cuda.synchronize()
-    got = dU.cpu().numpy().astype(np.int64)
+    got = mwblas.residue_products(dU.cpu().numpy(), mp, np_).astype(np.int64)
     assert np.all(got[0::2] == (127 * 127 * kp) % m)
     assert np.all(got[1::2] == (-127 * 127 * kp) % m)
 

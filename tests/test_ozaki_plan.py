@@ -196,3 +196,22 @@ def test_split_chunks_are_exact_and_bounded():
             assert abs(Fraction(X) - Fraction(xh_s) - Fraction(xl_s)) <= Fraction(5, 8)
             # the chunk combination stays below 2^45 for every modulus weight < 256
             assert abs(c0) + abs(c1) * 255 + abs(c2) * 255 < 2 ** 45
+
+
+def test_residue_product_layout_decoder():
+    """mwblas.residue_products decodes the MFMA block order of the residue products: the word
+    of rows i..i+3 (i % 4 == 0) of column j sits at block (j // 32) * (mp // 32) + i // 32
+    (1 KB each), word (i % 32) // 8 * 64 + (j % 32) + 32 * ((i // 4) % 2), byte i % 4.
+    Each entry is placed by that rule and must come back at (j, i)."""
+    import numpy as np
+    import mwblas
+    mp, np_ = 96, 64
+    flat = np.zeros(mp * np_, dtype=np.int64)
+    for j in range(np_):
+        for i in range(mp):
+            word = (i % 32) // 8 * 64 + (j % 32) + 32 * ((i // 4) % 2)
+            off = ((j // 32) * (mp // 32) + i // 32) * 1024 + word * 4 + i % 4
+            flat[off] = j * 1000 + i
+    got = mwblas.residue_products(flat, mp, np_)
+    expect = np.array([[j * 1000 + i for i in range(mp)] for j in range(np_)])
+    assert np.array_equal(got, expect)
