@@ -334,8 +334,11 @@ void k_oz_gemm(const int8_t *__restrict__ A, const int8_t *__restrict__ B,
 // blockIdx.y = batch (one product: modulus, or modulus x part for complex, per products per
 // modulus).  MULTI (real, per = 1): blockIdx.y = a group of G consecutive products (moduli)
 // of the tile, run as one ring sequence of `total` products in all, so the ring does not
-// fill and drain per product (short depths).
-template <bool FOLD, bool MULTI>
+// fill and drain per product (short depths).  M16: 16 x 16 x 64 MFMAs (the single-product
+// launches: 8192^3 residue GEMMs 6-7% faster than on 32 x 32 x 32); the grouped sequences
+// stay on 32 x 32 x 32, since the second A fragment buffer of the 16 x 16 x 64 form takes
+// the grouped kernel past 256 VGPRs (200 spilled).
+template <bool FOLD, bool MULTI, bool M16>
 __global__ __launch_bounds__(RING_THREADS, 1)
 void k_oz_gemm_ring(const int8_t *__restrict__ A, const int8_t *__restrict__ B,
                     uint8_t *__restrict__ U, int64_t kp, int64_t ldu, int64_t sA, int64_t sB,
@@ -350,13 +353,13 @@ void k_oz_gemm_ring(const int8_t *__restrict__ A, const int8_t *__restrict__ B,
     if constexpr (MULTI) {
         const int g0 = (int)blockIdx.y * G;
         const int gn = total - g0 < G ? total - g0 : G;
-        oz_gemm_tile_ring<4, 2, 2, 4, true, false, FOLD>(A + g0 * sA + i0 * kp,
+        oz_gemm_tile_ring<4, 2, 2, 4, true, false, FOLD, M16>(A + g0 * sA + i0 * kp,
                                                          B + g0 * sB + j0 * kp, U + g0 * sU, kp,
                                                          ldu, mcs + g0, gn, sA, sB, sU, i0, j0,
                                                          smem);
     } else {
         const int batch = blockIdx.y;
-        oz_gemm_tile_ring<4, 2, 2, 4, false, false, FOLD>(A + batch * sA + i0 * kp,
+        oz_gemm_tile_ring<4, 2, 2, 4, false, false, FOLD, M16>(A + batch * sA + i0 * kp,
                                                           B + batch * sB + j0 * kp,
                                                           U + batch * sU, kp, ldu,
                                                           mcs + batch / per, 1, 0, 0, 0, i0, j0,
@@ -616,16 +619,16 @@ hipError_t launch_gemm_ring(const int8_t *A, const int8_t *B, uint8_t *U, int64_
     constexpr int RING_GROUP = 8;
     if (per == 1 && batches > 1 && kp <= 2048) {
         dim3 grid((unsigned)(tiles_m * (np / 256)), (unsigned)((batches + RING_GROUP - 1) / RING_GROUP));
-        hipLaunchKernelGGL((k_oz_gemm_ring<false, true>), grid, dim3(RING_THREADS), 0, s, A, B, U,
+        hipLaunchKernelGGL((k_oz_gemm_ring<false, true, false>), grid, dim3(RING_THREADS), 0, s, A, B, U,
                            kp, mp, sA, sB, sU, mcs, per, tiles_m, RING_GROUP, batches);
         return hipGetLastError();
     }
     dim3 grid((unsigned)(tiles_m * (np / 256)), (unsigned)batches);
     if (kp > OZ_FOLD_DEPTH)
-        hipLaunchKernelGGL((k_oz_gemm_ring<true, false>), grid, dim3(RING_THREADS), 0, s, A, B, U,
+        hipLaunchKernelGGL((k_oz_gemm_ring<true, false, true>), grid, dim3(RING_THREADS), 0, s, A, B, U,
                            kp, mp, sA, sB, sU, mcs, per, tiles_m, 1, batches);
     else
-        hipLaunchKernelGGL((k_oz_gemm_ring<false, false>), grid, dim3(RING_THREADS), 0, s, A, B, U,
+        hipLaunchKernelGGL((k_oz_gemm_ring<false, false, true>), grid, dim3(RING_THREADS), 0, s, A, B, U,
                            kp, mp, sA, sB, sU, mcs, per, tiles_m, 1, batches);
     return hipGetLastError();
 }
