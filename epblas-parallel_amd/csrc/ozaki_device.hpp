@@ -560,7 +560,7 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
     const int8_t *ca = Ab, *cb = Bb;
     int64_t ck = 0;
     auto issue = [&](int64_t s) {
-        int8_t *slot = smem + (int)(s % NST) * SLOT;
+        int8_t *slot = smem + (int)((uint32_t)s % NST) * SLOT;
 #pragma unroll
         for (int q = 0; q < DQ; ++q) {
             const int blk = DQ * w + q;
@@ -593,7 +593,7 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
     // Fragments of 32-deep step ks (0 or 1) of stage s into fragment buffer d.  M16: ks = 1
     // reads stage s's last TN B fragments; ks = 0 its A fragments and first TN B fragments.
     auto frag = [&](int64_t s, int ks, int d) {
-        const int8_t *sl = smem + (int)(s % NST) * SLOT;
+        const int8_t *sl = smem + (int)((uint32_t)s % NST) * SLOT;
         if constexpr (M16) {
             (void)d;
             if (ks == 1) {
@@ -637,10 +637,11 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
 #pragma unroll
                 for (int a = 0; a < 2 * TM; ++a) a16[a] = an16[a];
             }
+            // Consecutive MFMAs share their B operand (measured 0.3-0.5% faster than sharing A).
 #pragma unroll
-            for (int a = 0; a < 2 * TM; ++a)
+            for (int b = 0; b < TN; ++b)
 #pragma unroll
-                for (int b = 0; b < TN; ++b) {
+                for (int a = 0; a < 2 * TM; ++a) {
                     AccT &c = acc[a][d * TN + b];
                     c = __builtin_amdgcn_mfma_i32_16x16x64_i8(a16[a], d ? bh[b] : bl[b], c, 0, 0, 0);
                 }
