@@ -689,6 +689,9 @@ def main():
     t_oz = (time.perf_counter() - t0) / args.steps
     mwblas.timer_enable(False)
     kt = mwblas.timer_read()
+    # moduli kept by the data-dependent count (the residues are formed for nmod; the residue
+    # GEMMs and the reconstruction of the others do not run)
+    kept = mwblas.ozaki_last_moduli() or nmod
 
     # plain FP64 path on the same inputs
     kp, t_plain, agree = {}, None, None
@@ -729,7 +732,7 @@ def main():
     plain_gflops = flops / t_plain / 1e9 if t_plain else None
     g_ms, g_cnt = kt.get("oz_gemm", (0.0, 0))
     g_avg = g_ms / max(g_cnt, 1)
-    ops = 2.0 * m * n * k * nmod                         # INT8 ops per residue-GEMM launch
+    ops = 2.0 * m * n * k * kept                         # INT8 ops per residue-GEMM launch
     achieved = ops / (g_avg * 1e-3) / 1e12 if g_avg > 0 else None
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "oz_gemm_8192_traffic.json")
@@ -754,7 +757,8 @@ def main():
         "config": {"workload": f"ozaki_int8_real_dd_gemm_{m}", "m": m, "n": n, "k": k,
                    "trans": "NN", "alpha": alpha, "beta": beta,
                    "inputs": "hi=+-U[1,2)*2^U_int[-20,20], lo=hi*2^-53*U[-1/2,1/2]",
-                   "moduli": nmod, "bits": bits, "parallelism": "single-gpu",
+                   "moduli": kept, "moduli_worst_case": nmod, "bits": bits,
+                   "parallelism": "single-gpu",
                    "residue_tile_variant": tile_variant,
                    "autotune_ms_per_variant": [round(x, 3) for x in tile_ms]},
         "plain_fp64": None if not t_plain else {

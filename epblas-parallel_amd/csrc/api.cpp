@@ -1456,7 +1456,10 @@ int mwgemm_wgemm_device(char transa, char transb, int64_t m, int64_t n, int64_t 
 int mwgemm_ozaki_config(int64_t k, int cplx, int *nmod, int *bits)
 {
     OzPlan p;
-    if (!ozaki_make_plan(cplx ? 2 * k : k, ozaki_target_bits(), ozaki_forced_nmod(), &p)) return -1;
+    const bool ok = !cplx && ozaki_estimate() != 0
+                        ? ozaki_make_fixed_plan(k, ozaki_estimate_bits(), &p)
+                        : ozaki_make_plan(cplx ? 2 * k : k, ozaki_target_bits(), ozaki_forced_nmod(), &p);
+    if (!ok) return -1;
     if (nmod) *nmod = p.nmod;
     if (bits) *bits = p.bits;
     return 0;
@@ -1465,7 +1468,26 @@ int mwgemm_ozaki_config(int64_t k, int cplx, int *nmod, int *bits)
 int mwgemm_ozaki_set(int bits, int nmod)
 {
     if ((bits != 0 && (bits < 8 || bits > 104)) || nmod < 0 || nmod > OZ_MAX_MOD) return -1;
-    ozaki_set_params(bits ? bits : 103, nmod);
+    ozaki_set_params(bits, nmod);
+    return 0;
+}
+
+int mwgemm_ozaki_set_estimate(int mode)
+{
+    if (mode < 0 || mode > 2) return -1;
+    ozaki_set_estimate(mode);
+    return 0;
+}
+
+int mwgemm_ozaki_last_moduli(int *nmod)
+{
+    if (!nmod) return -1;
+    return ozaki_last_estimate(nmod);
+}
+
+int mwgemm_testing_ozaki_count_offset(int offset)
+{
+    ozaki_set_count_offset(offset);
     return 0;
 }
 

@@ -34,6 +34,40 @@ struct OzPlan {
 // Returns false when k is out of range.
 bool ozaki_make_plan(int64_t k_eff, int target_bits, int force_nmod, OzPlan *plan);
 
+// Reconstruction constants of the first n moduli, n = 1..OZ_MAX_MOD, beside the set's table
+// of ModConst (the CRT weights depend on the whole set): M as a double-double and log2 M.
+// Entry n of a device array indexed by the modulus count.
+struct OzSet {
+    double Mhi, Mlo, log2M;
+};
+
+// Data-dependent modulus count (real products).  The split runs with a fixed integer width
+// b and the worst-case count (the plan of ozaki_make_fixed_plan); the row norm bounds it
+// accumulates then bound every |C'_ij| by Cauchy-Schwarz, and a device-side choice takes the
+// fewest moduli whose product exceeds twice that bound.  The residue GEMMs of the moduli
+// beyond it return at once, and the reconstruction uses the chosen set.  Mode 1 (default):
+// for products of at least OZ_ESTIMATE_MIN_MNK multiply-adds, where a modulus saved (1/29
+// of the residue GEMMs) outweighs the extra launch that chooses; 2: every real product;
+// 0 (MWGEMM_OZAKI_ESTIMATE=0, or a forced modulus count): never.
+constexpr double OZ_ESTIMATE_MIN_MNK = 8589934592.0;   // 2^33, e.g. 2048^3
+int ozaki_estimate();
+void ozaki_set_estimate(int mode);
+// Whether a real product of m x n x k uses the data-dependent count.
+bool ozaki_estimates(int64_t m, int64_t n, int64_t k);
+// Integer width of the estimating plans: the configured target, else 102 (normwise error
+// bound 2^-101 + the reconstruction's few 2^-104 < 1e-30).
+int ozaki_estimate_bits();
+// The plan with integer width exactly b and the fewest moduli for the worst case
+// M/2 > k_eff * 2^(2b).
+bool ozaki_make_fixed_plan(int64_t k_eff, int bits, OzPlan *plan);
+// The modulus count chosen by the last estimating call on the current device (waits for
+// the device); 0 when none has run.
+int ozaki_last_estimate(int *nmod);
+// Test hook: added to every data-dependent count (clamped to [1, worst case]), so a test can
+// show that one modulus fewer than the chosen count breaks an adversarial product.
+void ozaki_set_count_offset(int offset);
+int ozaki_count_offset();
+
 // Depth after which the residue GEMMs reduce their INT32 accumulators modulo m: a sum of
 // OZ_FOLD_DEPTH products of symmetric residues (|r| <= 128) stays below 2^29, so a running
 // residue in [0, m) plus the next partial sum stays inside the 2^30 range the epilogue's

@@ -50,6 +50,22 @@ DDI void int_chunks(double hi, double lo, int sigma, double &c0, double &c1, dou
     c2 = sg * a2;
 }
 
+// Data-dependent modulus count: a bound on the Euclidean norm of a scaled row.  An entry x
+// of a row scaled by 2^sigma becomes the integer X' of int_chunks, within 5/8 of
+// 2^sigma (hi + lo), so |X'| < 2^sigma (|hi| + |lo|) + 1.  With nshift = sigma - b + 16 and
+// q = floor(fl(|hi| + |lo|) 2^nshift + 2^-30) + 1, |X'| < q 2^(b-16): the margin 2^-30 (in
+// units of 2^(b-16)) covers the rounding of |hi| + |lo| (< 2^-37 there, since the scaled
+// row maximum is below 2^b) and the integer rounding (< 2^(16-b)).  q <= 2^16 + 1,
+// so the sum of q^2 over a row of up to 2^24 entries is an exact integer below 2^57, and
+// ||X'_row||^2 < sum q^2 * 2^(2b-32).  Integer sums are independent of the order in which
+// lanes and atomics add them, so the modulus count chosen from them is reproducible.
+DDI unsigned long long norm_q2(double hi, double lo, int nshift)
+{
+    const double y = ldexp(fabs(hi) + fabs(lo), nshift) + 0x1p-30;
+    const unsigned long long q = (unsigned long long)(uint32_t)y + 1ull;
+    return q * q;
+}
+
 // Symmetric residue of c0 + c1*2^35 + c2*2^70 modulo m: r = s - m*rint(s/m), where
 // s = c2*p2 + c1*p1 + c0 is an exact integer with |s| < 2^45.  The computed s*inv_m is
 // within |s/m|*2^-52 < 2^-7/m of s/m.  For odd m, s/m is at least 1/(2m) away from every
@@ -75,16 +91,20 @@ DDI uint32_t pack_low_bytes(int v0, int v1, int v2, int v3)
 
 // Residue split of one lane's unit of a real operand: row r, k in [16*kg, 16*kg+16), all
 // moduli, out[t*mod_stride + r*kp + k].  KC: entry (r,k) at X[r*ld + k] (row-contiguous);
-// otherwise at X[k*ld + r].
-template <bool KC>
-DDI void oz_split_unit(const ddv *__restrict__ X, int64_t ld, int64_t K,
-                       const unsigned long long *__restrict__ rmax, int bits,
-                       const ModConst *__restrict__ mcs, int nmod, int8_t *__restrict__ out,
-                       int64_t kp, int64_t mod_stride, int64_t r, int64_t kg)
+// otherwise at X[k*ld + r].  NORM: returns the unit's part of the row's norm bound (the sum
+// of norm_q2 over its entries k < K; 0 for a row holding a NaN or an infinity, whose
+// integers are all zero).
+template <bool KC, bool NORM = false>
+DDI unsigned long long oz_split_unit(const ddv *__restrict__ X, int64_t ld, int64_t K,
+                                     const unsigned long long *__restrict__ rmax, int bits,
+                                     const ModConst *__restrict__ mcs, int nmod,
+                                     int8_t *__restrict__ out, int64_t kp, int64_t mod_stride,
+                                     int64_t r, int64_t kg)
 {
     bool bad;
     const int sigma = row_sigma(rmax[r], bits, bad);
     double c0[16], c1[16], c2[16];
+    unsigned long long nrm = 0ull;
 #pragma unroll
     for (int s = 0; s < 16; ++s) {
         const int64_t k = kg * 16 + s;
@@ -93,6 +113,7 @@ DDI void oz_split_unit(const ddv *__restrict__ X, int64_t ld, int64_t K,
             const ddv x = KC ? X[r * ld + k] : X[k * ld + r];
             hi = x.hi;
             lo = x.lo;
+            if (NORM) nrm += norm_q2(hi, lo, sigma - bits + 16);
         }
         int_chunks(hi, lo, sigma, c0[s], c1[s], c2[s]);
     }
@@ -108,6 +129,7 @@ DDI void oz_split_unit(const ddv *__restrict__ X, int64_t ld, int64_t K,
         *reinterpret_cast<uint4 *>(out + t * mod_stride + r * kp + kg * 16) =
             make_uint4(w[0], w[1], w[2], w[3]);
     }
+    return nrm;
 }
 
 // Row maximum and residue split of a row-contiguous operand (entry (r,k) at X[r*ld + k]) in
@@ -120,7 +142,9 @@ DDI void oz_split_unit(const ddv *__restrict__ X, int64_t ld, int64_t K,
 // (rmax needs no clearing); the residues then follow exactly as in oz_split_unit.  Rows of
 // at most 8L entries stay in registers between the two passes; longer ones are read again.
 // Entries in [K, Kc) get zero residues.  Every lane of the wave calls this (lanes of a row
-// r >= R join the group shuffles only).
+// r >= R join the group shuffles only).  With rnorm, the first lane also stores the row's
+// norm bound (the sum of norm_q2 over its entries k < K; 0 for a row holding a NaN or an
+// infinity).
 __host__ __device__ inline int kc_lanes(int64_t Kc)
 {
     int L = 1;
@@ -132,7 +156,8 @@ DDI void oz_rowsplit_kc_group(const ddv *__restrict__ X, int64_t ld, int64_t R, 
                               int64_t Kc, int64_t r, int L, int ll,
                               unsigned long long *__restrict__ rmax, int bits,
                               const ModConst *__restrict__ mcs, int nmod,
-                              int8_t *__restrict__ out, int64_t mod_stride)
+                              int8_t *__restrict__ out, int64_t mod_stride,
+                              unsigned long long *__restrict__ rnorm = nullptr)
 {
     const bool live = r < R;
     const ddv *row = X + (live ? r : 0) * ld;
@@ -164,6 +189,7 @@ DDI void oz_rowsplit_kc_group(const ddv *__restrict__ X, int64_t ld, int64_t R, 
     bool bad;
     const int sigma = row_sigma(mx, bits, bad);
     int8_t *o = out + r * Kc;
+    unsigned long long nrm = 0ull;
     for (int64_t k0 = 0; k0 < Kc; k0 += span) {
         if (K > span) load(k0);          // else the whole row is still in v
         const int64_t kl = k0 + E * ll;
@@ -172,6 +198,7 @@ DDI void oz_rowsplit_kc_group(const ddv *__restrict__ X, int64_t ld, int64_t R, 
 #pragma unroll
         for (int s = 0; s < E; ++s) {
             int_chunks(bad ? 0.0 : v[s].hi, bad ? 0.0 : v[s].lo, sigma, c0[s], c1[s], c2[s]);
+            if (rnorm && !bad && kl + s < K) nrm += norm_q2(v[s].hi, v[s].lo, sigma - bits + 16);
         }
         for (int t = 0; t < nmod; ++t) {
             const ModConst mc = mcs[t];
@@ -182,6 +209,10 @@ DDI void oz_rowsplit_kc_group(const ddv *__restrict__ X, int64_t ld, int64_t R, 
                 make_uint2(pack_low_bytes(res[0], res[1], res[2], res[3]),
                            pack_low_bytes(res[4], res[5], res[6], res[7]));
         }
+    }
+    if (rnorm) {   // the row's norm bound (norm_q2), stored by the group's first lane
+        for (int off = L / 2; off > 0; off >>= 1) nrm += __shfl_xor(nrm, off, 64);
+        if (ll == 0) rnorm[r] = nrm;
     }
 }
 

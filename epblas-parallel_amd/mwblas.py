@@ -384,12 +384,46 @@ def ozaki_config(k, cplx=False):
 
 
 def ozaki_set(bits=0, nmod=0):
-    """Target integer width (0 = default 103) and forced modulus count (0 = automatic)."""
+    """Integer width (0 = automatic: 102 with the data-dependent modulus count, else at
+    least 103) and forced modulus count (0 = automatic)."""
     f = lib().mwgemm_ozaki_set
     f.restype = ctypes.c_int
     f.argtypes = [ctypes.c_int, ctypes.c_int]
     if f(bits, nmod) != 0:
         raise ValueError("bad Ozaki parameters")
+
+
+ESTIMATE_OFF, ESTIMATE_LARGE, ESTIMATE_ALWAYS = 0, 1, 2
+
+
+def ozaki_set_estimate(mode):
+    """Data-dependent modulus count of real exact products: ESTIMATE_OFF, ESTIMATE_LARGE
+    (products of at least 2^33 multiply-adds; the default) or ESTIMATE_ALWAYS."""
+    f = lib().mwgemm_ozaki_set_estimate
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int]
+    if f(int(mode)) != 0:
+        raise ValueError(f"bad estimate mode {mode}")
+
+
+def testing_ozaki_count_offset(offset):
+    """Test hook: offset added to the data-dependent modulus count (0 = normal)."""
+    f = lib().mwgemm_testing_ozaki_count_offset
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int]
+    f(offset)
+
+
+def ozaki_last_moduli():
+    """The modulus count the last real exact product on the current device kept with the
+    data-dependent count (waits for the device; 0 when none has run)."""
+    f = lib().mwgemm_ozaki_last_moduli
+    f.restype = ctypes.c_int
+    f.argtypes = [_vp]
+    nm = ctypes.c_int()
+    if f(ctypes.byref(nm)) != 0:
+        raise MwgemmError("mwgemm_ozaki_last_moduli failed")
+    return nm.value
 
 
 def ozaki_moduli(nmod):

@@ -305,14 +305,27 @@ MWGEMM_API int mwgemm_autotune_lookup(int kind, int64_t a, int64_t b, int64_t c)
 MWGEMM_API int mwgemm_autotune_clear(void);
 
 /* Ozaki path parameters.  mwgemm_ozaki_config reports, for contraction depth k (real or
- * complex), how many moduli the exact-product path uses and the integer width b every
- * scaled row/column is rounded to (normwise error bound ~4*2^-b).  mwgemm_ozaki_set sets the
- * target width (0 = default 103; also MWGEMM_OZAKI_BITS) and optionally forces the modulus
+ * complex), how many moduli the exact-product path provides for and the integer width b
+ * every scaled row/column is rounded to (normwise error bound ~4*2^-b).  mwgemm_ozaki_set
+ * sets the width (0 = automatic; also MWGEMM_OZAKI_BITS) and optionally forces the modulus
  * count (0 = automatic; also MWGEMM_OZAKI_NMOD).  mwgemm_ozaki_moduli lists the first nmod
- * moduli and their CRT weights y_t = ((M/m_t)^-1 mod m_t) for that set.  Return 0 or -1. */
+ * moduli and their CRT weights y_t = ((M/m_t)^-1 mod m_t) for that set.
+ *
+ * Data-dependent modulus count of real products (mwgemm_ozaki_set_estimate or
+ * MWGEMM_OZAKI_ESTIMATE: 1 = products of at least 2^33 multiply-adds, the default; 2 = every
+ * product; 0 = never; a forced count overrides it): the width is fixed (102 unless set), the
+ * residues are formed for the worst-case count, and on the device the call then keeps only
+ * as many moduli as a Cauchy-Schwarz bound of its scaled operands' row norms requires.
+ * Without it, the count covers the worst case k * 2^(2b) and b is raised as far as that
+ * count allows (103 by default).  For real k, mwgemm_ozaki_config reports the estimating
+ * plan (width and worst-case count) whenever estimation is enabled.
+ * mwgemm_ozaki_last_moduli waits for the current device and reports the count the last
+ * estimating call on it kept (0: none yet).  Return 0 or -1. */
 MWGEMM_API int mwgemm_ozaki_config(int64_t k, int cplx, int *nmod, int *bits);
 MWGEMM_API int mwgemm_ozaki_set(int bits, int nmod);
 MWGEMM_API int mwgemm_ozaki_moduli(int nmod, int *m, int *y);
+MWGEMM_API int mwgemm_ozaki_set_estimate(int mode);
+MWGEMM_API int mwgemm_ozaki_last_moduli(int *nmod);
 
 /* Test hook: while bytes > 0, every device buffer the library would allocate for itself
  * (exact-product workspace and modulus table, host-operand staging, batched plan memory)
@@ -320,6 +333,11 @@ MWGEMM_API int mwgemm_ozaki_moduli(int nmod, int *m, int *y);
  * the allocation-failure behaviour can be tested with ordinary shapes; 0 lifts the limit.
  * Returns 0, or -1 for a negative size. */
 MWGEMM_API int mwgemm_testing_set_alloc_limit(int64_t bytes);
+
+/* Test hook: offset added to every data-dependent modulus count (clamped to [1, the
+ * worst-case count]); 0 restores normal operation.  A negative offset deliberately keeps
+ * too few moduli, so a test can show that the chosen count is needed.  Returns 0. */
+MWGEMM_API int mwgemm_testing_ozaki_count_offset(int offset);
 
 /* Test hook: runs the production INT8 MFMA residue kernel once on device buffers,
  * U[j*mp + i] = (sum_k A[i*kp + k] * B[j*kp + k]) mod m with m the mod_index-th

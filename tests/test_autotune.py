@@ -37,7 +37,7 @@ def test_autotune_single_gemm_records_a_variant(gpu):
     assert v in (0, 1) and ms > 0
     assert mwblas.autotune_lookup(mwblas.TUNE_PLAIN_REAL, 300, 300, 300) == v
     v2, ms2 = mwblas.autotune(600, 600, 600, path=mwblas.PATH_OZAKI)
-    assert v2 in (0, 1) and ms2 > 0
+    assert v2 in (0, 1, 2, 3) and ms2 > 0     # the four residue-GEMM tile variants
     assert mwblas.autotune_lookup(mwblas.TUNE_OZ_GEMM, 768, 768, 640) == v2
 
 

@@ -308,3 +308,102 @@ def test_exact_product_deeper_than_65536(gpu, cplx, k):
           flush=True)
     assert r["err"] <= BAR
     assert r["err"] <= 4 * 2.0 ** -bits
+
+
+ONE, ZERO = np.array([1.0, 0.0]), np.zeros(2)
+
+
+@pytest.fixture
+def estimate_always():
+    """The data-dependent count on every real product (by default only on products of at
+    least 2^33 multiply-adds); the count-offset test hook reset afterwards."""
+    mwblas.ozaki_set_estimate(mwblas.ESTIMATE_ALWAYS)
+    yield
+    mwblas.ozaki_set_estimate(mwblas.ESTIMATE_LARGE)
+    mwblas.testing_ozaki_count_offset(0)
+
+
+def _real_nn(gpu, A, B, m, n, k):
+    import torch
+    dA, dB = torch.from_numpy(A).to(gpu), torch.from_numpy(B).to(gpu)
+    dC = torch.zeros((n, m, 2), dtype=torch.float64, device=gpu)
+    mwblas.gemm_device(False, "N", "N", m, n, k, [1.0, 0.0], dA, m, dB, k, [0.0, 0.0], dC, m,
+                       path=mwblas.PATH_OZAKI)
+    torch.cuda.synchronize()
+    return dC.cpu().numpy()
+
+
+def test_estimated_count_saves_a_modulus_and_one_fewer_fails(gpu, estimate_always):
+    """Data-dependent modulus count on the 512^3 workload (alpha = 1, beta = 0, uniform
+    entries): the Cauchy-Schwarz bound of the scaled rows keeps one modulus fewer than the
+    worst case k 2^(2b); every entry is within 1e-30 of binary128; a second call is bitwise
+    equal (the bounds are integer sums); and one more modulus removed (test hook) wraps the
+    reconstruction, so the kept count is not more than needed by a wide margin."""
+    rng = np.random.default_rng(512)
+    m = n = k = 512
+    A = ddgen.real_matrix(rng, m, k)
+    B = ddgen.real_matrix(rng, k, n)
+    C0 = np.zeros((n, m, 2))
+    worst, bits = mwblas.ozaki_config(k)
+    assert bits == 102
+    C = _real_nn(gpu, A, B, m, n, k)
+    kept = mwblas.ozaki_last_moduli()
+    r = oracle.q128_gemm_err(False, "N", "N", m, n, k, ONE, A, m, B, k, ZERO, C0,
+                             m, C, m)
+    print(f"512^3 uniform: kept {kept} of {worst} moduli, err={r['err']:.3e}", flush=True)
+    assert kept == worst - 1
+    assert r["err"] <= BAR and r["err"] <= 4 * 2.0 ** -bits
+    assert np.array_equal(_real_nn(gpu, A, B, m, n, k), C)
+    mwblas.testing_ozaki_count_offset(-1)
+    Cbad = _real_nn(gpu, A, B, m, n, k)
+    assert mwblas.ozaki_last_moduli() == kept - 1
+    rb = oracle.q128_gemm_err(False, "N", "N", m, n, k, ONE, A, m, B, k, ZERO, C0,
+                              m, Cbad, m)
+    print(f"  with {kept - 1} moduli: err={rb['err']:.3e}", flush=True)
+    assert rb["err"] > 1e-3
+
+
+def test_estimated_count_keeps_the_worst_case_when_it_is_attained(gpu, estimate_always):
+    """Adversarial operands: every entry 1 (each row's maximum and every scaled integer
+    equal 2^(b-1)), so C'_ij = k 2^(2b-2) and the Cauchy-Schwarz bound is attained.  At
+    k = 2048 the bound needs the worst-case count (29); the exact result 2048 comes out, and
+    with one modulus fewer (test hook) C' exceeds M/2 and the result is wrong."""
+    m, n, k = 256, 320, 2048
+    A = np.zeros((k, m, 2))
+    A[..., 0] = 1.0
+    B = np.zeros((n, k, 2))
+    B[..., 0] = 1.0
+    worst, bits = mwblas.ozaki_config(k)
+    C = _real_nn(gpu, A, B, m, n, k)
+    kept = mwblas.ozaki_last_moduli()
+    print(f"all-ones k={k}: kept {kept} of {worst} moduli", flush=True)
+    assert kept == worst
+    assert np.all(C[..., 0] == float(k)) and np.all(C[..., 1] == 0.0)
+    mwblas.testing_ozaki_count_offset(-1)
+    Cbad = _real_nn(gpu, A, B, m, n, k)
+    assert mwblas.ozaki_last_moduli() == worst - 1
+    assert not np.any(np.abs(Cbad[..., 0] - k) < 1.0)
+
+
+def test_estimated_and_worst_case_counts_agree_with_binary128(gpu, estimate_always):
+    """Wide-exponent operands, ragged shape: the data-dependent count keeps fewer moduli than
+    the worst case, and both settings are within 1e-30 of binary128."""
+    rng = np.random.default_rng(4242)
+    m, n, k = 700, 520, 3000
+    A = ddgen.real_matrix(rng, m, k, kind="wide")
+    B = ddgen.real_matrix(rng, k, n, kind="wide")
+    C0 = np.zeros((n, m, 2))
+    worst, _ = mwblas.ozaki_config(k)
+    errs = {}
+    for est in (True, False):
+        mwblas.ozaki_set_estimate(mwblas.ESTIMATE_ALWAYS if est else mwblas.ESTIMATE_OFF)
+        C = _real_nn(gpu, A, B, m, n, k)
+        if est:
+            kept = mwblas.ozaki_last_moduli()
+        r = oracle.q128_gemm_err(False, "N", "N", m, n, k, ONE, A, m, B, k, ZERO, C0, m, C, m,
+                                 entries=(rng.integers(0, m, 256), rng.integers(0, n, 256)))
+        errs[est] = r["err"]
+    print(f"wide {m}x{n}x{k}: kept {kept} of {worst}; err est={errs[True]:.3e} "
+          f"worst={errs[False]:.3e}", flush=True)
+    assert kept < worst
+    assert errs[True] <= BAR and errs[False] <= BAR

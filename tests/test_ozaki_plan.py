@@ -26,18 +26,31 @@ def test_moduli_pairwise_coprime_and_weights_are_inverses():
 @pytest.mark.parametrize("k,cplx", [(1, False), (512, False), (8192, False), (65536, False),
                                     (70000, False), (1 << 20, False), (4096, True),
                                     (32768, True), (40000, True)])
-def test_plan_covers_the_product_range(k, cplx):
-    nmod, bits = mwblas.ozaki_config(k, cplx)
+@pytest.mark.parametrize("estimate", [True, False])
+def test_plan_covers_the_product_range(k, cplx, estimate):
+    """The count covers the worst case |C'| <= k_eff 2^(2b) and one fewer would not.  Real
+    products with the data-dependent count split at the fixed width b = 102 (normwise
+    bound 4 * 2^-102 < 1e-30) and report that worst-case count; otherwise (and for complex
+    products) b is at least 103."""
+    mwblas.ozaki_set_estimate(mwblas.ESTIMATE_LARGE if estimate else mwblas.ESTIMATE_OFF)
+    try:
+        nmod, bits = mwblas.ozaki_config(k, cplx)
+    finally:
+        mwblas.ozaki_set_estimate(mwblas.ESTIMATE_LARGE)
     mm, _ = mwblas.ozaki_moduli(nmod)
     M = math.prod(int(v) for v in mm)
     k_eff = 2 * k if cplx else k
     # |C'| <= k_eff * 2^(2b) must lie strictly inside the symmetric range (-M/2, M/2)
     assert M > 2 * k_eff * 2 ** (2 * bits)
-    assert bits >= 103          # normwise error bound 4 * 2^-b <= 4e-31
+    if estimate and not cplx:
+        assert bits == 102
+    else:
+        assert bits >= 103          # normwise error bound 4 * 2^-b <= 4e-31
+    assert 4 * 2.0 ** -bits < 1e-30
     # and one modulus fewer would not do
     if nmod > 1:
         M1 = M // int(mm[-1])
-        assert not (M1 > 2 * k_eff * 2 ** (2 * 103))
+        assert not (M1 > 2 * k_eff * 2 ** (2 * min(bits, 103)))
 
 
 def test_depth_limit_is_rejected():
