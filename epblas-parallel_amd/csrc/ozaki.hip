@@ -461,6 +461,31 @@ void k_oz_gemm_ring(const int8_t *__restrict__ A, const int8_t *__restrict__ B,
     }
 }
 
+// The single-product ring launch with persistent workgroups (one per CU): workgroup b runs
+// the items b, b + gridDim.x, ... of the launch's tiles x products item space (RingWalk), as
+// the non-persistent grid would dispatch them, with the ring kept full across items.  dyn:
+// products of moduli t >= *dyn are not part of the item space.  Depth up to OZ_FOLD_DEPTH.
+__global__ __launch_bounds__(RING_THREADS, 1)
+void k_oz_gemm_ring_persist(const int8_t *__restrict__ A, const int8_t *__restrict__ B,
+                            uint8_t *__restrict__ U, int64_t kp, int64_t ldu, int64_t sA,
+                            int64_t sB, int64_t sU, const ModConst *__restrict__ mcs, int per,
+                            int tiles_m, int tiles, int products, const int *__restrict__ dyn)
+{
+    __shared__ __attribute__((aligned(1024))) int8_t smem[RING_NST * RING_SLOT];
+    if (dyn && *dyn * per < products) products = *dyn * per;
+    const int items = tiles * products;
+    RingWalk walk;
+    walk.first = (int)blockIdx.x;
+    walk.stride = (int)gridDim.x;
+    walk.count = walk.first < items ? (items - walk.first + walk.stride - 1) / walk.stride : 0;
+    walk.tiles_m = tiles_m;
+    walk.tiles = tiles;
+    walk.per = per;
+    if (walk.count == 0) return;
+    oz_gemm_tile_ring<4, 2, 2, 4, false, false, false, true, NoRingEpilogue, RingWalk>(
+        A, B, U, kp, ldu, mcs, 1, sA, sB, sU, 0, 0, smem, 0, 0, NoRingEpilogue{}, walk);
+}
+
 // Reconstruction kernel: one 32 x 32 block of C per workgroup (oz_crt_block), blocks column
 // block by column block.
 // With dyn (data-dependent modulus count), mcs is the first of the per-count tables
@@ -834,6 +859,32 @@ hipError_t launch_gemm_tile(const int8_t *A, const int8_t *B, uint8_t *U, int64_
     return hipGetLastError();
 }
 
+// Persistent single-product ring launches (MWGEMM_RING_PERSIST=0: one workgroup per item).
+bool ring_persist()
+{
+    static const bool on = [] {
+        const char *e = getenv("MWGEMM_RING_PERSIST");
+        return !(e && atoi(e) == 0);
+    }();
+    return on;
+}
+
+// Compute units of the current device (cached per device).
+int device_cus()
+{
+    static std::mutex mu;
+    static std::vector<int> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    std::lock_guard<std::mutex> lk(mu);
+    if ((int)cache.size() <= dev) cache.resize(dev + 1, 0);
+    if (!cache[dev]) {
+        hipDeviceProp_t prop;
+        cache[dev] = hipGetDeviceProperties(&prop, dev) == hipSuccess ? prop.multiProcessorCount : 256;
+    }
+    return cache[dev];
+}
+
 hipError_t launch_gemm_ring(const int8_t *A, const int8_t *B, uint8_t *U, int64_t mp, int64_t np,
                             int64_t kp, int64_t sA, int64_t sB, int64_t sU, const ModConst *mcs,
                             int per, int batches, hipStream_t s, const int *dyn)
@@ -850,7 +901,18 @@ hipError_t launch_gemm_ring(const int8_t *A, const int8_t *B, uint8_t *U, int64_
                            kp, mp, sA, sB, sU, mcs, per, tiles_m, RING_GROUP, batches, dyn);
         return hipGetLastError();
     }
-    dim3 grid((unsigned)(tiles_m * (np / 256)), (unsigned)batches);
+    const int tiles = tiles_m * (int)(np / 256);
+    // Persistent workgroups unless the accumulators must fold (the folding walk does not fit
+    // in 256 VGPRs).
+    if (ring_persist() && kp <= OZ_FOLD_DEPTH) {
+        const int cus = device_cus();
+        const int64_t items = (int64_t)tiles * batches;
+        const dim3 pg((unsigned)(items < cus ? items : cus));
+        hipLaunchKernelGGL(k_oz_gemm_ring_persist, pg, dim3(RING_THREADS), 0, s, A, B, U, kp, mp,
+                           sA, sB, sU, mcs, per, tiles_m, tiles, batches, dyn);
+        return hipGetLastError();
+    }
+    dim3 grid((unsigned)tiles, (unsigned)batches);
     if (kp > OZ_FOLD_DEPTH)
         hipLaunchKernelGGL((k_oz_gemm_ring<true, false, true>), grid, dim3(RING_THREADS), 0, s, A, B, U,
                            kp, mp, sA, sB, sU, mcs, per, tiles_m, 1, batches, dyn);

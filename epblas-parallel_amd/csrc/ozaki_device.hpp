@@ -557,15 +557,45 @@ DDI void wait_lds() { __builtin_amdgcn_s_waitcnt(15 | (7 << 4) | (0 << 8) | (3 <
 // same (it follows the sub-tile, not the instruction).  Step 0 of a stage multiplies the
 // stage's A fragments by the first TN B fragments, step 1 by the other TN; the next stage's
 // A fragments are read during step 1 into a second buffer.
+//
+// Walk (persistent workgroups, single products): instead of one tile, the workgroup runs the
+// items walk.first, walk.first + walk.stride, ... (walk.count of them) of a launch's item
+// space as one sequence of stages: item L is product L / tiles (modulus L / tiles / per)
+// of tile L % tiles in tile_order, i.e. the order in which a grid of one workgroup per item
+// would be dispatched, so the tiles resident on an XCD at a time are the same.  The ring is
+// not drained and refilled between items and no workgroup is launched per item: the next
+// item's first stages are in flight while the current item's accumulators are reduced and
+// stored.  Ab, Bb, Ub, mcs are then the bases of product 0; i0, j0 and nmod are unused.
 struct NoRingEpilogue {};
+struct NoRingWalk {
+    static constexpr int per = 1;
+};
+struct RingWalk {
+    int first, stride, count;   // this workgroup's items
+    int tiles_m, tiles, per;    // tiles per product (tiles_m of them per tile column)
+    DDI void item(int q, int64_t kp, int64_t sA, int64_t sB, const int8_t *Ab, const int8_t *Bb,
+                  const int8_t *&a, const int8_t *&b, int64_t &i0, int64_t &j0, int &t) const
+    {
+        const int L = first + q * stride;
+        t = L / tiles;
+        int tm, tn;
+        tile_order(L - t * tiles, tiles_m, tiles / tiles_m, tm, tn);
+        i0 = (int64_t)tm * 256;
+        j0 = (int64_t)tn * 256;
+        a = Ab + t * sA + i0 * kp;
+        b = Bb + t * sB + j0 * kp;
+    }
+};
 template <int WM, int WN, int TM, int TN, bool MULTI, bool CLAMP = false, bool FOLD = true,
-          bool M16 = false, typename Epi = NoRingEpilogue>
+          bool M16 = false, typename Epi = NoRingEpilogue, typename Walk = NoRingWalk>
 DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restrict__ Bb,
                            uint8_t *__restrict__ Ub, int64_t kp, int64_t ldu,
                            const ModConst *__restrict__ mcs, int nmod, int64_t sA, int64_t sB,
                            int64_t sU, int64_t i0, int64_t j0, int8_t *smem, int rows_a = 0,
-                           int rows_b = 0, Epi &&epi = Epi{})
+                           int rows_b = 0, Epi &&epi = Epi{}, const Walk &walk = Walk{})
 {
+    constexpr bool PERSIST = std::is_same<Walk, RingWalk>::value;
+    static_assert(!PERSIST || (!MULTI && !CLAMP && 32 * TM * WM == 256), "persistent walk");
     constexpr int BT = 32 * TM * WM, NW = WM * WN;
     static_assert(BT == 32 * TN * WN, "square tile");
     constexpr int NST = RING_NST, BK = RING_BK, SLOT = ring_slot<BT>();
@@ -577,7 +607,9 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wm = w % WM, wn = w / WM;
     const int64_t nk = kp / BK;
-    const int64_t S = MULTI ? nk * nmod : nk;   // stages of the whole sequence
+    int nitems = 1;
+    if constexpr (PERSIST) nitems = walk.count;
+    const int64_t S = MULTI ? nk * nmod : nk * nitems;   // stages of the whole sequence
     const int drow = lane >> 2, dslot = lane & 3;
     // Chunk c of stage row r sits in 16-byte slot c ^ swz(r).  Measured with
     // SQ_LDS_BANK_CONFLICT: (r >> 2) & 3 leaves the 32 x 32 operand reads (32 rows of one
@@ -590,6 +622,11 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
     // Instruction q of wave w covers rows (DQ*w + q) * 16 + lane / 4 of both panels.
     const int8_t *ca = Ab, *cb = Bb;
     int64_t ck = 0;
+    // PERSIST: the output tile and product of the item being finished, and the items of the
+    // DMA and compute cursors
+    int64_t fi0 = i0, fj0 = j0;
+    int ft = 0, dq = 0, cq = 0;
+    if constexpr (PERSIST) walk.item(0, kp, sA, sB, Ab, Bb, ca, cb, fi0, fj0, ft);
     auto issue = [&](int64_t s) {
         int8_t *slot = smem + (int)((uint32_t)s % NST) * SLOT;
 #pragma unroll
@@ -606,10 +643,19 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
                 (const __attribute__((address_space(1))) void *)(cb + (int64_t)rb * kp + ck * BK + c * 16),
                 (__attribute__((address_space(3))) void *)(slot + BT * BK + blk * 1024), 16, 0, 0);
         }
-        if (++ck == nk && MULTI) {
-            ck = 0;
-            ca += sA;
-            cb += sB;
+        if (++ck == nk) {
+            if constexpr (MULTI) {
+                ck = 0;
+                ca += sA;
+                cb += sB;
+            } else if constexpr (PERSIST) {
+                ck = 0;
+                if (++dq < nitems) {
+                    int tt;
+                    int64_t d0, d1;
+                    walk.item(dq, kp, sA, sB, Ab, Bb, ca, cb, d0, d1, tt);
+                }
+            }
         }
     };
     v4i af[2][TM], bf[2][TN];
@@ -713,7 +759,7 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
             epi(t, acc);
             return;
         }
-        const ModConst mc = mcs[t];
+        const ModConst mc = mcs[PERSIST ? t / walk.per : t];
         uint8_t *U = Ub + (int64_t)t * sU;
         if constexpr (M16) {
             // Tile (a, b): lane l holds rows 4(l >> 4) .. +3 of column l & 15.
@@ -727,7 +773,7 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
                     uint32_t word = 0;
 #pragma unroll
                     for (int q = 0; q < 4; ++q) word |= crt_residue(acc[a][b][q], mc) << (8 * q);
-                    *reinterpret_cast<uint32_t *>(U + u_word_offset(i0 + rl, j0 + cl, ldu)) = word;
+                    *reinterpret_cast<uint32_t *>(U + u_word_offset(fi0 + rl, fj0 + cl, ldu)) = word;
                 }
         } else {
 #pragma unroll
@@ -736,7 +782,7 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
                 for (int b = 0; b < TN; ++b) {
                     const int cl = wn * 32 * TN + b * 32 + (lane & 31);
                     const int rl = wm * 32 * TM + a * 32 + 4 * (lane >> 5);
-                    const int64_t col = j0 + cl, rowb = i0 + rl;
+                    const int64_t col = fj0 + cl, rowb = fi0 + rl;
 #pragma unroll
                     for (int g = 0; g < 4; ++g) {
                         // rows rl+8g .. +3 lie all inside or all outside rows_a (a multiple of 32)
@@ -753,8 +799,9 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
     // Deep products: every OZ_FOLD_DEPTH of depth the accumulators are replaced by a small
     // representative of their class modulo m_t, which keeps the INT32 sums exact at any depth.
     auto fold = [&](int t) {
-        const float rm = (float)mcs[t].inv_m;
-        const int mi = (int)mcs[t].m;
+        const int tc = PERSIST ? t / walk.per : t;
+        const float rm = (float)mcs[tc].inv_m;
+        const int mi = (int)mcs[tc].m;
 #pragma unroll
         for (int a = 0; a < AM; ++a)
 #pragma unroll
@@ -778,7 +825,17 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
                 return;
             }
         }
-        if (FOLD && ckt % FOLD_STAGES == 0) fold(ct);
+        if constexpr (PERSIST) {
+            if (ckt == nk) {   // the last stage of an item before the last one
+                flush(ft);
+                clear();
+                ckt = 0;
+                const int8_t *a, *b;
+                walk.item(++cq, kp, sA, sB, Ab, Bb, a, b, fi0, fj0, ft);
+                return;
+            }
+        }
+        if (FOLD && ckt % FOLD_STAGES == 0) fold(PERSIST ? ft : ct);
     };
     clear();
 
@@ -818,7 +875,7 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
     interleave(false);
     wait_lds();
     mma(1);
-    flush(MULTI ? nmod - 1 : 0);
+    flush(MULTI ? nmod - 1 : (PERSIST ? ft : 0));
     __syncthreads();   // the ring may be refilled by the caller's next tile
 }
 
