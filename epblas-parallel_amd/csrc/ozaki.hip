@@ -241,9 +241,6 @@ void fill_plan(int n, int b, double lg, OzPlan *p)
         ModConst &c = p->mc[t];
         c.m = (double)m;
         c.inv_m = 1.0 / (double)m;
-        c.inv_up = std::nextafter(c.inv_m, 1.0);
-        if (c.inv_up * (double)m < 1.0) c.inv_up = std::nextafter(c.inv_up, 1.0);
-        c.off_acc = std::ldexp((double)m, 24);
         c.y = (double)mod_inverse(others, m);
         c.p1 = (double)mod_pow2(35, m);
         c.p2 = (double)mod_pow2(70, m);

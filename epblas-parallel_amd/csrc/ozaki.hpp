@@ -12,8 +12,6 @@ constexpr int OZ_MAX_MOD = 40;
 // Per-modulus constants, read by every Ozaki kernel (uniform across a wave).
 struct ModConst {
     double m, inv_m;      // modulus and 1/m (rounded)
-    double inv_up;        // 1/m rounded up: floor(d * inv_up) == floor(d / m) for 0 <= d < 2^51/m
-    double off_acc;       // m * 2^24 >= 2^30: makes an INT32 residue-GEMM sum non-negative
     double y;             // ((M/m)^-1 mod m): CRT weight numerator
     double p1, p2;        // 2^35 mod m, 2^70 mod m: chunk weights for the residue split
     double w[4];          // round(2^128/m) as four 32-bit limbs (w[0] least significant)

@@ -269,13 +269,17 @@ DDI void oz_split_unit_k3(const zddv *__restrict__ X, int64_t ld, int64_t K,
     }
 }
 
-// u = v mod m for an exact INT32 dot product v (|v| <= 2^30): shifted by m*2^24 to a
-// non-negative integer below 2^33, the quotient is exact with inv_up.  (The CRT weight
+// The symmetric residue u = v - m*rint(v/m) of an exact INT32 dot product v (|v| <= 2^30),
+// stored by the residue GEMMs as a signed byte: sym_residue's argument with s = v (v*fl(1/m)
+// is within 2^-28 of v/m, never that close to a half-integer for odd m, and exact for
+// m = 256, where a tie gives +-128, byte 0x80 either way).  The reconstruction takes any
+// representative of v's class modulo m (oz_crt_accumulate), so the canonical
+// [0, m) form, one more FP64 operation per entry, is not needed.  (The CRT weight
 // y = (M/m)^-1 mod m lives in the reconstruction's fixed-point limbs.)
-DDI uint32_t crt_residue(int v, const ModConst &mc)
+DDI int crt_residue(int v, const ModConst &mc)
 {
-    const double d = (double)v + mc.off_acc;
-    return (uint32_t)__builtin_fma(-floor(d * mc.inv_up), mc.m, d);
+    const double d = (double)v;
+    return (int)__builtin_fma(-rint(d * mc.inv_m), mc.m, d);
 }
 
 // A representative of v modulo m with |r| < 4m, from 32-bit arithmetic only (light on
@@ -388,10 +392,9 @@ DDI void oz_gemm_tile(const int8_t *__restrict__ Ab, const int8_t *__restrict__ 
                 const int64_t rowb = i0 + wm * 32 * TM + a * 32 + 4 * (lane >> 5);
 #pragma unroll
                 for (int g = 0; g < 4; ++g) {
-                    uint32_t word = 0;
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                        word |= crt_residue(acc[a][b][4 * g + q], mc) << (8 * q);
+                    const uint32_t word = pack_low_bytes(
+                        crt_residue(acc[a][b][4 * g], mc), crt_residue(acc[a][b][4 * g + 1], mc),
+                        crt_residue(acc[a][b][4 * g + 2], mc), crt_residue(acc[a][b][4 * g + 3], mc));
                     *reinterpret_cast<uint32_t *>(U + u_word_offset(rowb + 8 * g, col, ldu)) = word;
                 }
             }
@@ -770,9 +773,9 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
                     const int cl = wn * 32 * TN + b * 16 + (lane & 15);
                     const int rl = wm * 32 * TM + a * 16 + 4 * (lane >> 4);
                     if (CLAMP && (rl >= rows_a || cl >= rows_b)) continue;
-                    uint32_t word = 0;
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) word |= crt_residue(acc[a][b][q], mc) << (8 * q);
+                    const uint32_t word = pack_low_bytes(
+                        crt_residue(acc[a][b][0], mc), crt_residue(acc[a][b][1], mc),
+                        crt_residue(acc[a][b][2], mc), crt_residue(acc[a][b][3], mc));
                     *reinterpret_cast<uint32_t *>(U + u_word_offset(fi0 + rl, fj0 + cl, ldu)) = word;
                 }
         } else {
@@ -787,10 +790,9 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
                     for (int g = 0; g < 4; ++g) {
                         // rows rl+8g .. +3 lie all inside or all outside rows_a (a multiple of 32)
                         if (CLAMP && (rl + 8 * g >= rows_a || cl >= rows_b)) continue;
-                        uint32_t word = 0;
-#pragma unroll
-                        for (int q = 0; q < 4; ++q)
-                            word |= crt_residue(acc[a][b][4 * g + q], mc) << (8 * q);
+                        const uint32_t word = pack_low_bytes(
+                            crt_residue(acc[a][b][4 * g], mc), crt_residue(acc[a][b][4 * g + 1], mc),
+                            crt_residue(acc[a][b][4 * g + 2], mc), crt_residue(acc[a][b][4 * g + 3], mc));
                         *reinterpret_cast<uint32_t *>(U + u_word_offset(rowb + 8 * g, col, ldu)) = word;
                     }
                 }
@@ -912,7 +914,12 @@ DDI ddv reconstruct(const double acc[4], double Mhi, double Mlo, int e, bool bad
 
 // Reconstruction + alpha/beta update of rows i0..i0+3 (those < m) of column j.
 // Fixed-point CRT sums of RPL consecutive rows i0.. of column j: acc[e][p][l] accumulates
-// u_t * limb l of round(2^128/m_t) over the moduli (part p: real / imaginary).
+// u_t * limb l of W_t = round(2^128 y_t / m_t) over the moduli (part p: real / imaginary).
+// Any representative u_t of the class of C' modulo m_t will do: u_t + k m_t changes the sum
+// by k m_t W_t = k y_t 2^128 + k m_t (W_t - 2^128 y_t / m_t), i.e. by at most |k| m_t / 2
+// units of 2^-128 of M, so the residue GEMMs' signed bytes (|u_t| <= 128) and the complex
+// combinations below (|u_t| <= 384) are used as they are; every limb product stays below
+// 2^41 and every limb sum below 2^47, exact in FP64.
 template <bool CPLX, int RPL>
 DDI void oz_crt_accumulate(const uint8_t *__restrict__ U, int64_t ldu, int64_t sU, int nmod,
                            const ModConst *__restrict__ mcs, int64_t j, int64_t i0,
@@ -930,25 +937,19 @@ DDI void oz_crt_accumulate(const uint8_t *__restrict__ U, int64_t ldu, int64_t s
 #pragma unroll
             for (int l = 0; l < 4; ++l) acc[e][p][l] = 0.0;
     // One modulus: the residue bytes of RPL rows times the four weight limbs.
+    // byte e of a word as a signed integer (the residue GEMMs store symmetric residues)
+    auto sbyte = [](uint32_t word, int e) { return ((int)(word << (24 - 8 * e))) >> 24; };
     auto step = [&](int t, const uint32_t (&word)[NQ]) {
         const double w0 = mcs[t].w[0], w1 = mcs[t].w[1], w2 = mcs[t].w[2], w3 = mcs[t].w[3];
-        const int mi = (int)mcs[t].m;
 #pragma unroll
         for (int e = 0; e < RPL; ++e) {
             int u[P];
             if constexpr (CPLX) {
-                const int p0 = (int)((word[0] >> (8 * e)) & 0xffu);
-                const int p1 = (int)((word[1] >> (8 * e)) & 0xffu);
-                const int p2 = (int)((word[2] >> (8 * e)) & 0xffu);
-                int re = p0 - p1;
-                re += re < 0 ? mi : 0;
-                int im = p2 - p0 - p1;
-                im += im < 0 ? mi : 0;
-                im += im < 0 ? mi : 0;
-                u[0] = re;
-                u[P - 1] = im;
+                const int p0 = sbyte(word[0], e), p1 = sbyte(word[1], e), p2 = sbyte(word[2], e);
+                u[0] = p0 - p1;               // representatives of Re and Im modulo m,
+                u[P - 1] = p2 - p0 - p1;      // |u| <= 384
             } else {
-                u[0] = (int)((word[0] >> (8 * e)) & 0xffu);
+                u[0] = sbyte(word[0], e);
             }
 #pragma unroll
             for (int p = 0; p < P; ++p) {

@@ -452,12 +452,15 @@ def testing_residue_gemm(dA, dB, dU, mp, np_, kp, mod_index, variant=1, stream=N
 
 
 def residue_products(U, mp, np_):
-    """The reduced residue products written by testing_residue_gemm as an (np_, mp) array
-    (column j, row i).  The library stores them in 32 x 32 blocks in MFMA accumulator order:
-    blocks column block by column block (ldu = mp), within a block word g*64 + l (g = 0..3,
-    lane l = 0..63) packing rows 8g + 4(l >> 5) .. +3 of column l & 31, one byte per row."""
+    """The reduced residue products written by testing_residue_gemm as an (np_, mp) int8
+    array (column j, row i): symmetric residues, |u| <= (m - 1) / 2 (+-128 for m = 256).
+    The library stores them in 32 x 32 blocks in MFMA accumulator order: blocks column block
+    by column block (ldu = mp), within a block word g*64 + l (g = 0..3, lane l = 0..63)
+    packing rows 8g + 4(l >> 5) .. +3 of column l & 31, one byte per row."""
     import numpy as np
     u = np.asarray(U).reshape(-1)
+    if u.dtype == np.uint8:
+        u = u.view(np.int8)       # the bytes are signed residues
     assert mp % 32 == 0 and np_ % 32 == 0 and u.size >= mp * np_
     blk = u[:mp * np_].reshape(np_ // 32, mp // 32, 4, 2, 32, 4)   # bj, bi, g, half, col, q
     return blk.transpose(0, 4, 1, 2, 3, 5).reshape(np_, mp)

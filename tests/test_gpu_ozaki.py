@@ -22,6 +22,15 @@ def ozaki_path():
     mwblas.set_path(prev)
 
 
+def _assert_symmetric_residues(got, expect, m):
+    """The residue GEMMs store v - m*rint(v/m): congruent to the exact product modulo m, in
+    [-(m-1)/2, (m-1)/2] (odd m) or [-128, 128] (m = 256, whose ties give +-128 = byte 0x80,
+    read back as -128)."""
+    assert np.array_equal(got % m, expect)
+    h = 128 if m == 256 else (m - 1) // 2
+    assert got.min() >= -h and got.max() <= h
+
+
 @pytest.mark.parametrize("mod_index,variant,kp", [
     (0, 1, 320), (1, 1, 320), (5, 1, 320), (28, 1, 320), (39, 1, 320),
     (0, 2, 320), (28, 3, 192),
@@ -44,7 +53,7 @@ def test_residue_gemm_kernel_is_exact(gpu, mod_index, variant, kp):
     torch.cuda.synchronize()
     expect = (A @ B.T) % m                        # (mp, np); y lives in the CRT limbs
     got = mwblas.residue_products(dU.cpu().numpy(), mp, np_).astype(np.int64).T
-    assert np.array_equal(got, expect)
+    _assert_symmetric_residues(got, expect, m)
 
 
 def _check(cplx, ta, tb, m, n, k, kind, rng, ent=None):
@@ -253,7 +262,7 @@ def test_residue_gemm_deeper_than_one_accumulation(gpu, variant, kp):
     torch.cuda.synchronize()
     expect = (A.astype(np.float64) @ B.T.astype(np.float64)).astype(np.int64) % m
     got = mwblas.residue_products(dU.cpu().numpy(), mp, np_).astype(np.int64).T
-    assert np.array_equal(got, expect)
+    _assert_symmetric_residues(got, expect, m)
 
 
 @pytest.mark.parametrize("variant", [0, 1])
@@ -272,8 +281,8 @@ def test_residue_gemm_sum_beyond_int32(gpu, variant):
     mwblas.testing_residue_gemm(dA, dB, dU, mp, np_, kp, mod_index, variant)
     torch.cuda.synchronize()
     got = mwblas.residue_products(dU.cpu().numpy(), mp, np_).astype(np.int64)
-    assert np.all(got[0::2] == (127 * 127 * kp) % m)
-    assert np.all(got[1::2] == (-127 * 127 * kp) % m)
+    _assert_symmetric_residues(got[0::2], np.full_like(got[0::2], (127 * 127 * kp) % m), m)
+    _assert_symmetric_residues(got[1::2], np.full_like(got[1::2], (-127 * 127 * kp) % m), m)
 
 
 @pytest.mark.parametrize("cplx,k", [(False, 70000), (True, 40000)])

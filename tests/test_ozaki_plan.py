@@ -106,39 +106,22 @@ def test_fixed_point_crt_reconstruction_identity():
         plain = [X % m for m in mm]
         assert abs(recon(plain, Wy) - X) <= M / 2 ** 115
         assert abs(recon([(u * y) % m for u, y, m in zip(plain, yy, mm)], W) - X) <= M / 2 ** 115
+        # any representative of the class will do: the symmetric residues the residue GEMMs
+        # store, and the unreduced combinations of the complex reconstruction (|u| <= 384)
+        sym = [((X + m // 2) % m) - m // 2 for m in mm]
+        assert abs(recon(sym, Wy) - X) <= M / 2 ** 115
+        shifted = [u + rng.choice((-1, 0, 1)) * m for u, m in zip(sym, mm)]
+        assert abs(recon(shifted, Wy) - X) <= M / 2 ** 114
     # dropping the weight on one side is caught
     X = lim // 3
     assert abs(recon([X % m for m in mm], W) - X) > M / 2 ** 20
 
 
-def _inv_up(m):
-    import numpy as np
-    return float(np.nextafter(1.0 / m, 2.0))
-
-
-def test_rounded_up_reciprocal_gives_exact_quotients():
-    """The residue kernels take floor(d * RU(1/m)) as the exact quotient floor(d/m) for
-    non-negative integers d below 2^33 (GEMM epilogue) and 2^48 (split); check it on random
-    and worst-case (multiples of m, one below a multiple) integers for every modulus."""
-    m_all, _ = mwblas.ozaki_moduli(40)
-    rng = random.Random(9)
-    for m in (int(v) for v in m_all):
-        inv = _inv_up(m)
-        for lim in (2 ** 33, 2 ** 48):
-            cases = [rng.randrange(0, lim) for _ in range(3000)]
-            top = lim // m
-            cases += [q * m for q in (0, 1, 2, top - 1, top)] + [q * m - 1 for q in (1, 2, top)]
-            cases += [rng.randrange(0, top) * m + d for d in (0, m - 1) for _ in range(200)]
-            for d in cases:
-                if d < 0 or d >= lim:
-                    continue
-                assert math.floor(float(d) * inv) == d // m, (m, d)
-
-
 def _sym_residue(s, m):
-    """The split kernel's residue step in binary64: s is an exact integer (|s| < 2^45), the
-    FMA after the quotient is an exact integer operation, only s * fl(1/m) rounds, and the
-    quotient is rounded to the nearest integer (ties to even, as rint)."""
+    """The residue step of the split kernels and of the residue GEMMs' epilogue in binary64:
+    s is an exact integer (|s| < 2^45 in the split, |s| <= 2^30 for an INT32 product sum),
+    the FMA after the quotient is an exact integer operation, only s * fl(1/m) rounds, and
+    the quotient is rounded to the nearest integer (ties to even, as rint)."""
     q = round(float(s) * (1.0 / m))
     return s - q * m
 
