@@ -776,7 +776,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(achieved, 1) if achieved else None,
                      "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / INT8_PEAK_TOPS, 4) if achieved else None,
-                     "traffic": traffic, "kernel": "k_oz_gemm_ring (INT8 residue GEMM, int8 ops)"},
+                     "traffic": traffic, "kernel": "k_oz_gemm_ring_persist (INT8 residue GEMMs, int8 ops)"},
     }
     try:
         result["library_int8_gemm"] = library_int8_bench(dev, n)
