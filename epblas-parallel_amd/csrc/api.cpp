@@ -818,6 +818,7 @@ int mgemm_host(char transa, char transb, int64_t m, int64_t n, int64_t k,
     g.A = reinterpret_cast<const ddv *>(a); g.lda = lda;
     g.B = reinterpret_cast<const ddv *>(b); g.ldb = ldb;
     g.C = reinterpret_cast<ddv *>(c); g.ldc = ldc;
+    g.host_call = true;
     const bool need_ab = k > 0 && !dd_is0(g.alpha);
     const int64_t ar = need_ab ? (g.opa == OP_N ? m : k) : 0, ac = g.opa == OP_N ? k : m;
     const int64_t br = need_ab ? (g.opb == OP_N ? k : n) : 0, bc = g.opb == OP_N ? n : k;

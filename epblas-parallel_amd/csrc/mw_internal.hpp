@@ -20,6 +20,12 @@ struct GemmArgsR {
     ddv *C; int64_t ldc;
     int variant = -1;        // tile variant; -1: autotuned choice or heuristic
     int order = 0;           // plain path: ORDER_SIMD or ORDER_SCALAR (plain_tile.hpp)
+    // A call of the drop-in entry points (mgemm_, mgemm_64_, mgemm_serial), whose operands are
+    // staged in one piece or in column panels: its exact product keeps the worst-case
+    // modulus count, because column panels would each choose a count from their own
+    // columns, and a different count rounds the reconstruction differently; so a call gives
+    // the same bits however it is staged.
+    bool host_call = false;
 };
 
 struct GemmArgsZ {

@@ -1068,7 +1068,7 @@ bool real_plan(int64_t k, bool est, OzPlan *plan)
 int ozaki_gemm_real(const GemmArgsR &g, hipStream_t s, uint64_t a_tag)
 {
     std::lock_guard<std::mutex> lk(g_oz_mu);
-    const bool est = ozaki_estimates(g.m, g.n, g.k);
+    const bool est = !g.host_call && ozaki_estimates(g.m, g.n, g.k);
     OzPlan plan;
     if (!real_plan(g.k, est, &plan))
         return record_error(-5, "ozaki: contraction depth %lld out of range", (long long)g.k);

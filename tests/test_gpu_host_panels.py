@@ -18,7 +18,8 @@ import mwblas
 pytestmark = pytest.mark.gpu
 
 
-def _run(cplx, ta, tb, m, n, k, seed, beta_zero, path=mwblas.PATH_PLAIN):
+def _run(cplx, ta, tb, m, n, k, seed, beta_zero, path=mwblas.PATH_PLAIN, device_estimate=None,
+         kind="uniform"):
     import torch
     rng = np.random.default_rng(seed)
     gen = ddgen.cplx_matrix if cplx else ddgen.real_matrix
@@ -26,7 +27,8 @@ def _run(cplx, ta, tb, m, n, k, seed, beta_zero, path=mwblas.PATH_PLAIN):
     ar, ac = (m, k) if ta == "N" else (k, m)
     br, bc = (k, n) if tb == "N" else (n, k)
     lda, ldb, ldc = ar + 3, br + 5, m + 2
-    A, B, C0 = gen(rng, ar, ac, lda), gen(rng, br, bc, ldb), gen(rng, m, n, ldc)
+    A, B = gen(rng, ar, ac, lda, kind=kind), gen(rng, br, bc, ldb, kind=kind)
+    C0 = gen(rng, m, n, ldc)
     al = scal(rng, scale=2)
     be = np.zeros(4 if cplx else 2) if beta_zero else scal(rng, scale=2)
     print(f"host panels {'wgemm' if cplx else 'mgemm'} {ta}{tb} m={m} n={n} k={k} "
@@ -46,13 +48,19 @@ def _run(cplx, ta, tb, m, n, k, seed, beta_zero, path=mwblas.PATH_PLAIN):
         else:
             os.environ["MWGEMM_HOST_PANELS"] = prev
     dA, dB, dC = (torch.from_numpy(x).to("cuda:0") for x in (A, B, C0))
-    mwblas.gemm_device(cplx, ta, tb, m, n, k, al, dA, lda, dB, ldb, be, dC, ldc,
-                       path=path, stream=0)
-    torch.cuda.synchronize()
+    if device_estimate is not None:
+        mwblas.ozaki_set_estimate(device_estimate)
+    try:
+        mwblas.gemm_device(cplx, ta, tb, m, n, k, al, dA, lda, dB, ldb, be, dC, ldc,
+                           path=path, stream=0)
+        torch.cuda.synchronize()
+    finally:
+        mwblas.ozaki_set_estimate(mwblas.ESTIMATE_LARGE)
     Cd = dC.cpu().numpy()
     assert np.array_equal(Cp, Cs, equal_nan=True)
     assert np.array_equal(Cp, Cd, equal_nan=True)
     assert np.array_equal(Cp[:, m:, :], C0[:, m:, :])
+    return dA, dB, C0, al, be, lda, ldb, ldc
 
 
 @pytest.mark.parametrize("ta,tb,beta_zero", [("N", "N", False), ("T", "N", True),
@@ -89,6 +97,31 @@ def test_exact_product_panels_share_a_and_match_one_call(gpu, ta, tb, beta_zero)
                      ddgen.cplx_scalar(rng), C, 40)
         _run(False, ta, tb, 600, 1100, 1700, 1430 + (ta == "T"), beta_zero, mwblas.PATH_OZAKI)
     finally:
+        mwblas.set_path(prev)
+
+
+def test_dropin_exact_product_keeps_the_worst_case_count(gpu):
+    """The drop-in's exact product keeps the worst-case modulus count (its column panels
+    would each choose a count from their own columns, and another count rounds the
+    reconstruction differently).  With the data-dependent count on for every product, the
+    panelled call and the single staged call give the same bits, those of the device call
+    with the count off, on wide-exponent operands for which the device call with the count
+    on keeps fewer moduli than the worst case."""
+    import torch
+    m, n, k = 600, 1100, 1700
+    prev = mwblas.set_path(mwblas.PATH_OZAKI)
+    mwblas.ozaki_set_estimate(mwblas.ESTIMATE_ALWAYS)
+    try:
+        dA, dB, C0, al, be, lda, ldb, ldc = _run(
+            False, "N", "N", m, n, k, 1460, False, mwblas.PATH_OZAKI,
+            device_estimate=mwblas.ESTIMATE_OFF, kind="wide")
+        mwblas.ozaki_set_estimate(mwblas.ESTIMATE_ALWAYS)
+        dC = torch.from_numpy(C0).to("cuda:0")
+        mwblas.gemm_device(False, "N", "N", m, n, k, al, dA, lda, dB, ldb, be, dC, ldc,
+                           path=mwblas.PATH_OZAKI, stream=0)
+        assert mwblas.ozaki_last_moduli() < mwblas.ozaki_config(k)[0]
+    finally:
+        mwblas.ozaki_set_estimate(mwblas.ESTIMATE_LARGE)
         mwblas.set_path(prev)
 
 
