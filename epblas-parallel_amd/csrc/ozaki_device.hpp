@@ -269,17 +269,22 @@ DDI void oz_split_unit_k3(const zddv *__restrict__ X, int64_t ld, int64_t K,
     }
 }
 
-// The symmetric residue u = v - m*rint(v/m) of an exact INT32 dot product v (|v| <= 2^30),
-// stored by the residue GEMMs as a signed byte: sym_residue's argument with s = v (v*fl(1/m)
-// is within 2^-28 of v/m, never that close to a half-integer for odd m, and exact for
-// m = 256, where a tie gives +-128, byte 0x80 either way).  The reconstruction takes any
-// representative of v's class modulo m (oz_crt_accumulate), so the canonical
-// [0, m) form, one more FP64 operation per entry, is not needed.  (The CRT weight
-// y = (M/m)^-1 mod m lives in the reconstruction's fixed-point limbs.)
+// The symmetric residue u = v - m*q, q = rint(v/m), of an exact INT32 dot product v
+// (|v| <= 2^29: the residue GEMMs fold their sums below that), stored by the residue GEMMs
+// as a signed byte.  The FMA t = v*fl(1/m) + 1.5*2^52 rounds the exact product v*fl(1/m)
+// once, to the integer grid of [2^52, 2^53): v*fl(1/m) is within 2^-23 of v/m, never that
+// close to a half-integer for odd m and exact for m = 256 (where a tie gives +-128, byte
+// 0x80 either way), so q is the nearest integer to v/m; the low 32 bits of t hold q
+// (2^51 + q, reduced mod 2^32) in two's complement.  |q| < 2^23, so u = v - q*m is one
+// exact 24-bit multiply and a subtraction.  Four operations per entry, against six for
+// floor(d*RU(1/m)) on an offset v with FP64 conversions both ways.  The reconstruction
+// takes any representative of v's class modulo m (oz_crt_accumulate), so the canonical
+// [0, m) form is not needed.  (The CRT weight y = (M/m)^-1 mod m lives in the
+// reconstruction's fixed-point limbs.)
 DDI int crt_residue(int v, const ModConst &mc)
 {
-    const double d = (double)v;
-    return (int)__builtin_fma(-rint(d * mc.inv_m), mc.m, d);
+    const double t = __builtin_fma((double)v, mc.inv_m, 0x1.8p52);
+    return v - __mul24(__double2loint(t), (int)mc.m);
 }
 
 // A representative of v modulo m with |r| < 4m, from 32-bit arithmetic only (light on

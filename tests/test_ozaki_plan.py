@@ -117,11 +117,48 @@ def test_fixed_point_crt_reconstruction_identity():
     assert abs(recon([X % m for m in mm], W) - X) > M / 2 ** 20
 
 
+def _epilogue_residue(v, m):
+    """The residue GEMMs' epilogue (crt_residue): t = fma(v, fl(1/m), 1.5 * 2^52) in binary64
+    (the exact product plus the constant, rounded once), q = the low 32 bits of t as a signed
+    integer, u = v - q * m with q inside the signed 24-bit range of the multiply."""
+    import struct
+    from fractions import Fraction
+    t = float(Fraction(v) * Fraction(1.0 / m) + Fraction(3, 2) * 2 ** 52)
+    lo = struct.unpack("<q", struct.pack("<d", t))[0] & 0xFFFFFFFF
+    q = lo - 2 ** 32 if lo >= 2 ** 31 else lo
+    assert -2 ** 23 <= q < 2 ** 23, (v, m, q)
+    return v - q * m
+
+
+def test_epilogue_residue_is_a_symmetric_representative():
+    """The residue GEMMs reduce every INT32 sum v (|v| <= 2^29 + 2^10: sums of up to 32768
+    products of residues, plus a folded remainder below 4m) to v - m * round(v/m) with one
+    FMA against 1.5 * 2^52 and one 24-bit multiply.  For every modulus: random sums, the
+    range ends, and the integers next to every half-integer multiple of m near 0, the ends
+    and random points; the result must be congruent to v modulo m and lie in the symmetric
+    range (+-128 for m = 256)."""
+    m_all, _ = mwblas.ozaki_moduli(40)
+    rng = random.Random(17)
+    lim = 2 ** 29 + 2 ** 10
+    for m in (int(v) for v in m_all):
+        top = lim // m
+        h = m // 2
+        cases = [rng.randrange(-lim, lim + 1) for _ in range(1500)]
+        for q in (0, 1, top - 1, top) + tuple(rng.randrange(0, top) for _ in range(40)):
+            cases += [q * m + d for d in (-h - 1, -h, -h + 1, -1, 0, 1, h - 1, h, h + 1)]
+        cases += [-c for c in cases]
+        bound = 128 if m == 256 else (m - 1) // 2
+        for v in cases:
+            if abs(v) > lim:
+                continue
+            u = _epilogue_residue(v, m)
+            assert (u - v) % m == 0 and -bound <= u <= bound, (m, v, u)
+
+
 def _sym_residue(s, m):
-    """The residue step of the split kernels and of the residue GEMMs' epilogue in binary64:
-    s is an exact integer (|s| < 2^45 in the split, |s| <= 2^30 for an INT32 product sum),
-    the FMA after the quotient is an exact integer operation, only s * fl(1/m) rounds, and
-    the quotient is rounded to the nearest integer (ties to even, as rint)."""
+    """The split kernels' residue step in binary64: s is an exact integer (|s| < 2^45), the
+    FMA after the quotient is an exact integer operation, only s * fl(1/m) rounds, and the
+    quotient is rounded to the nearest integer (ties to even, as rint)."""
     q = round(float(s) * (1.0 / m))
     return s - q * m
 
