@@ -5,7 +5,10 @@ needs a larger workspace must not free or overwrite that buffer: it moves to a n
 the graph keeps the old.  The test captures a small exact-product GEMM, makes a larger
 eager call (which used to free the captured buffer), then replays the graph several times,
 with eager calls of the library on another stream between and beside the replays, and
-checks every replay and every eager call against the same call made alone."""
+checks every replay and every eager call against the same call made alone.  With the
+data-dependent modulus count on for every product, the graph also holds the row norm
+bounds' clearing and atomics, the choice of the count and the GEMMs and reconstruction that
+read it on the device."""
 import numpy as np
 import pytest
 
@@ -15,7 +18,18 @@ import mwblas
 pytestmark = pytest.mark.gpu
 
 
-def test_replays_survive_a_larger_eager_call(gpu):
+@pytest.mark.parametrize("estimate", ["large", "always"])
+def test_replays_survive_a_larger_eager_call(gpu, estimate):
+    import torch
+    mwblas.ozaki_set_estimate(mwblas.ESTIMATE_ALWAYS if estimate == "always"
+                              else mwblas.ESTIMATE_LARGE)
+    try:
+        _replays_survive_a_larger_eager_call(gpu)
+    finally:
+        mwblas.ozaki_set_estimate(mwblas.ESTIMATE_LARGE)
+
+
+def _replays_survive_a_larger_eager_call(gpu):
     import torch
     rng = np.random.default_rng(1301)
     small, big = (300, 260, 410), (1100, 900, 1300)

@@ -55,6 +55,31 @@ def test_two_calls_match_the_allocating_entry(gpu, path):
         assert torch.equal(got, want), name
 
 
+def test_data_dependent_count_in_the_scratch(gpu):
+    """With the data-dependent modulus count on for every product, the no-allocation entry
+    point chooses the count from the norm bounds it keeps in the scratch memory, with every
+    count's constants in the scratch's head: the same bits as the allocating entry point,
+    on two calls in a row (the second smaller), and fewer moduli than the worst case for
+    the wide-exponent first call."""
+    import torch
+    rng = np.random.default_rng(1420)
+    first = _problem(rng, gpu, 700, 650, 900, "N", "T", "wide")
+    second = _problem(rng, gpu, 130, 70, 333, "T", "N", "uniform")
+    mwblas.ozaki_set_estimate(mwblas.ESTIMATE_ALWAYS)
+    try:
+        for name, p in (("first 700x650x900", first), ("second 130x70x333", second)):
+            print(f"no-allocation entry, data-dependent count, {name}", flush=True)
+            got = _run(p, mwblas.gemm_device_noalloc, mwblas.PATH_OZAKI)
+            kept = mwblas.ozaki_last_moduli()
+            want = _run(p, _existing, mwblas.PATH_OZAKI)
+            assert mwblas.ozaki_last_moduli() == kept
+            assert torch.equal(got, want), name
+            if p is first:
+                assert kept < mwblas.ozaki_config(p["k"])[0]
+    finally:
+        mwblas.ozaki_set_estimate(mwblas.ESTIMATE_LARGE)
+
+
 def test_workspace_beyond_the_scratch_is_rejected(gpu):
     import torch
     rng = np.random.default_rng(1410)
