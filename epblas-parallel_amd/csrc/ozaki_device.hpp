@@ -287,6 +287,23 @@ DDI int crt_residue(int v, const ModConst &mc)
     return v - __mul24(__double2loint(t), (int)mc.m);
 }
 
+// The byte that the residue GEMMs store for v: crt_residue(v, mc) modulo 2^8, in three
+// operations.  With neg_m = 2^24 - m (residue_neg_m), (q mod 2^24) * neg_m = -q*m modulo
+// 2^24, so v + that product (one v_mad_u32_u24, which takes the low 24 bits of q by itself)
+// agrees with u = v - q*m in its low 8 bits; the bits above them are not kept.
+DDI uint32_t residue_neg_m(const ModConst &mc) { return 0x1000000u - (uint32_t)mc.m; }
+DDI uint32_t crt_residue_byte(int v, const ModConst &mc, uint32_t neg_m)
+{
+    const double t = __builtin_fma((double)v, mc.inv_m, 0x1.8p52);
+    return (uint32_t)v + __umul24((uint32_t)__double2loint(t), neg_m);
+}
+// The stored bytes of four entries as one word, v0 in byte 0.
+DDI uint32_t residue_word(int v0, int v1, int v2, int v3, const ModConst &mc, uint32_t neg_m)
+{
+    return pack_low_bytes((int)crt_residue_byte(v0, mc, neg_m), (int)crt_residue_byte(v1, mc, neg_m),
+                          (int)crt_residue_byte(v2, mc, neg_m), (int)crt_residue_byte(v3, mc, neg_m));
+}
+
 // A representative of v modulo m with |r| < 4m, from 32-bit arithmetic only (light on
 // registers, for use inside the residue GEMMs' main loops).  For |v| <= 2^30 and m >= 71,
 // v/m < 2^24; the three roundings of fl(fl(v) * fl(1/m)) (relative 2^-24 each) move it by
@@ -311,6 +328,11 @@ DDI int64_t u_word_offset(int64_t i, int64_t j, int64_t ldu)
     const int r = (int)(i & 31);
     return ((j >> 5) * (ldu >> 5) + (i >> 5)) * 1024 + (r >> 3) * 256 +
            ((j & 31) + 32 * ((r >> 2) & 1)) * 4;
+}
+// Byte offset of the 32 x 32 block whose first row and column are i0, j0 (multiples of 32).
+DDI int64_t u_block_offset(int64_t i0, int64_t j0, int64_t ldu)
+{
+    return ((j0 >> 5) * (ldu >> 5) + (i0 >> 5)) * 1024;
 }
 
 // Workgroup -> output tile.  Workgroups are dispatched round-robin over the 8 XCDs (each
@@ -388,21 +410,22 @@ DDI void oz_gemm_tile(const int8_t *__restrict__ Ab, const int8_t *__restrict__ 
     // Reduce the accumulators of modulus t and store them, four rows per 32-bit word.
     auto flush = [&](int t) {
         const ModConst mc = mcs[t];
-        uint8_t *U = Ub + (int64_t)t * sU;
+        const uint32_t neg_m = residue_neg_m(mc);
+        // Word g of accumulator tile (a, b): rows 8g + 4(lane >> 5) of 32 x 32 block (a, b) of
+        // the wave's sub-tile, column lane & 31, i.e. byte 4 * lane + 256 g of that block.
+        const int64_t cstride = (ldu >> 5) * 1024;
+        uint8_t *U = Ub + (int64_t)t * sU +
+                     u_block_offset(i0 + wm * 32 * TM, j0 + wn * 32 * TN, ldu) + lane * 4;
 #pragma unroll
         for (int a = 0; a < TM; ++a)
 #pragma unroll
-            for (int b = 0; b < TN; ++b) {
-                const int64_t col = j0 + wn * 32 * TN + b * 32 + (lane & 31);
-                const int64_t rowb = i0 + wm * 32 * TM + a * 32 + 4 * (lane >> 5);
+            for (int b = 0; b < TN; ++b)
 #pragma unroll
                 for (int g = 0; g < 4; ++g) {
-                    const uint32_t word = pack_low_bytes(
-                        crt_residue(acc[a][b][4 * g], mc), crt_residue(acc[a][b][4 * g + 1], mc),
-                        crt_residue(acc[a][b][4 * g + 2], mc), crt_residue(acc[a][b][4 * g + 3], mc));
-                    *reinterpret_cast<uint32_t *>(U + u_word_offset(rowb + 8 * g, col, ldu)) = word;
+                    const uint32_t word = residue_word(acc[a][b][4 * g], acc[a][b][4 * g + 1],
+                                                       acc[a][b][4 * g + 2], acc[a][b][4 * g + 3], mc, neg_m);
+                    *reinterpret_cast<uint32_t *>(U + b * cstride + a * 1024 + g * 256) = word;
                 }
-            }
     };
     // Deep products: every OZ_FOLD_DEPTH of depth the accumulators are replaced by a small
     // representative of their class modulo m_t, which keeps the INT32 sums exact at any depth.
@@ -768,9 +791,19 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
             return;
         }
         const ModConst mc = mcs[PERSIST ? t / walk.per : t];
-        uint8_t *U = Ub + (int64_t)t * sU;
+        const uint32_t neg_m = residue_neg_m(mc);
+        // Every word of the wave's sub-tile lies at a fixed offset from one per-lane base:
+        // the wave's first 32 x 32 block, the lane's place in a block, then whole blocks
+        // (1 KB down a block column, cstride bytes across block columns).  u_word_offset per
+        // word cost ~10 instructions of 64-bit index arithmetic, more than the reduction.
+        const int64_t cstride = (ldu >> 5) * 1024;
+        const int lane_off = M16 ? (lane >> 5) * 256 + ((lane >> 4) & 1) * 128 + (lane & 15) * 4
+                                 : lane * 4;
+        uint8_t *U = Ub + (int64_t)t * sU +
+                     u_block_offset(fi0 + wm * 32 * TM, fj0 + wn * 32 * TN, ldu) + lane_off;
         if constexpr (M16) {
-            // Tile (a, b): lane l holds rows 4(l >> 4) .. +3 of column l & 15.
+            // Tile (a, b): lane l holds rows 4(l >> 4) .. +3 of column l & 15, i.e. rows
+            // 16(a & 1) + 4(l >> 4) of 32 x 32 block (a >> 1, b >> 1), column 16(b & 1) + (l & 15).
 #pragma unroll
             for (int a = 0; a < AM; ++a)
 #pragma unroll
@@ -778,10 +811,10 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
                     const int cl = wn * 32 * TN + b * 16 + (lane & 15);
                     const int rl = wm * 32 * TM + a * 16 + 4 * (lane >> 4);
                     if (CLAMP && (rl >= rows_a || cl >= rows_b)) continue;
-                    const uint32_t word = pack_low_bytes(
-                        crt_residue(acc[a][b][0], mc), crt_residue(acc[a][b][1], mc),
-                        crt_residue(acc[a][b][2], mc), crt_residue(acc[a][b][3], mc));
-                    *reinterpret_cast<uint32_t *>(U + u_word_offset(fi0 + rl, fj0 + cl, ldu)) = word;
+                    const uint32_t word = residue_word(acc[a][b][0], acc[a][b][1], acc[a][b][2],
+                                                       acc[a][b][3], mc, neg_m);
+                    *reinterpret_cast<uint32_t *>(U + (b >> 1) * cstride + (a >> 1) * 1024 +
+                                                  (a & 1) * 512 + (b & 1) * 64) = word;
                 }
         } else {
 #pragma unroll
@@ -790,15 +823,13 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
                 for (int b = 0; b < TN; ++b) {
                     const int cl = wn * 32 * TN + b * 32 + (lane & 31);
                     const int rl = wm * 32 * TM + a * 32 + 4 * (lane >> 5);
-                    const int64_t col = fj0 + cl, rowb = fi0 + rl;
 #pragma unroll
                     for (int g = 0; g < 4; ++g) {
                         // rows rl+8g .. +3 lie all inside or all outside rows_a (a multiple of 32)
                         if (CLAMP && (rl + 8 * g >= rows_a || cl >= rows_b)) continue;
-                        const uint32_t word = pack_low_bytes(
-                            crt_residue(acc[a][b][4 * g], mc), crt_residue(acc[a][b][4 * g + 1], mc),
-                            crt_residue(acc[a][b][4 * g + 2], mc), crt_residue(acc[a][b][4 * g + 3], mc));
-                        *reinterpret_cast<uint32_t *>(U + u_word_offset(rowb + 8 * g, col, ldu)) = word;
+                        const uint32_t word = residue_word(acc[a][b][4 * g], acc[a][b][4 * g + 1],
+                                                           acc[a][b][4 * g + 2], acc[a][b][4 * g + 3], mc, neg_m);
+                        *reinterpret_cast<uint32_t *>(U + b * cstride + a * 1024 + g * 256) = word;
                     }
                 }
         }

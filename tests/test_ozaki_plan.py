@@ -120,14 +120,17 @@ def test_fixed_point_crt_reconstruction_identity():
 def _epilogue_residue(v, m):
     """The residue GEMMs' epilogue (crt_residue): t = fma(v, fl(1/m), 1.5 * 2^52) in binary64
     (the exact product plus the constant, rounded once), q = the low 32 bits of t as a signed
-    integer, u = v - q * m with q inside the signed 24-bit range of the multiply."""
+    integer, u = v - q * m with q inside the signed 24-bit range of the multiply.  Also
+    returns the byte the stores take (crt_residue_byte): the low 8 bits of the 32-bit
+    v + (low 24 bits of the word q) * (2^24 - m), one unsigned 24-bit multiply-add."""
     import struct
     from fractions import Fraction
     t = float(Fraction(v) * Fraction(1.0 / m) + Fraction(3, 2) * 2 ** 52)
     lo = struct.unpack("<q", struct.pack("<d", t))[0] & 0xFFFFFFFF
     q = lo - 2 ** 32 if lo >= 2 ** 31 else lo
     assert -2 ** 23 <= q < 2 ** 23, (v, m, q)
-    return v - q * m
+    byte = ((v & 0xFFFFFFFF) + (lo & 0xFFFFFF) * (2 ** 24 - m)) & 0xFF
+    return v - q * m, byte
 
 
 def test_epilogue_residue_is_a_symmetric_representative():
@@ -136,7 +139,8 @@ def test_epilogue_residue_is_a_symmetric_representative():
     FMA against 1.5 * 2^52 and one 24-bit multiply.  For every modulus: random sums, the
     range ends, and the integers next to every half-integer multiple of m near 0, the ends
     and random points; the result must be congruent to v modulo m and lie in the symmetric
-    range (+-128 for m = 256)."""
+    range (+-128 for m = 256), and the stored byte from the multiply-add form must be its
+    low byte."""
     m_all, _ = mwblas.ozaki_moduli(40)
     rng = random.Random(17)
     lim = 2 ** 29 + 2 ** 10
@@ -151,8 +155,9 @@ def test_epilogue_residue_is_a_symmetric_representative():
         for v in cases:
             if abs(v) > lim:
                 continue
-            u = _epilogue_residue(v, m)
+            u, byte = _epilogue_residue(v, m)
             assert (u - v) % m == 0 and -bound <= u <= bound, (m, v, u)
+            assert byte == u & 0xFF, (m, v, u, byte)
 
 
 def _sym_residue(s, m):
