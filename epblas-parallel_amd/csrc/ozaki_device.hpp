@@ -739,7 +739,15 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
 #pragma unroll
                 for (int e = 0; e < AE; ++e) acc[a][b][e] = 0;
     };
-    auto mma = [&](int d) {
+    // Z: the first stage of a product, whose MFMAs start from zero instead of the accumulators
+    // (M16: both steps, which write disjoint accumulators; else step 0), so the accumulators
+    // need no clearing after a flush.  Only the grouped products (MULTI, one product per
+    // modulus of a short depth) start that way: for the persistent walk, whose items are 128
+    // stages deep, the second copy of the stage loop cost 11% of the 8192^3 residue GEMMs,
+    // against the 0.8% that the clearing takes.
+    constexpr bool ZSTART = !PERSIST;
+    auto mma = [&](int d, auto Z) {
+        constexpr bool z = decltype(Z)::value;
         if constexpr (M16) {
             if (d == 0) {
 #pragma unroll
@@ -751,14 +759,16 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
 #pragma unroll
                 for (int a = 0; a < 2 * TM; ++a) {
                     AccT &c = acc[a][d * TN + b];
-                    c = __builtin_amdgcn_mfma_i32_16x16x64_i8(a16[a], d ? bh[b] : bl[b], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_i32_16x16x64_i8(a16[a], d ? bh[b] : bl[b],
+                                                             z ? AccT{0, 0, 0, 0} : c, 0, 0, 0);
                 }
         } else {
 #pragma unroll
             for (int a = 0; a < TM; ++a)
 #pragma unroll
                 for (int b = 0; b < TN; ++b)
-                    acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[d][a], bf[d][b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                        af[d][a], bf[d][b], (z && d == 0) ? AccT{} : acc[a][b], 0, 0, 0);
         }
     };
     // Scheduling groups: each MFMA followed by one DMA (when refilling) and one fragment read.
@@ -852,30 +862,35 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
     // sequence is finished after the loops (flush of the last modulus).
     int ct = 0;
     int64_t ckt = 0;
+    // Returns whether the accumulators were flushed (the next stage starts a product).
     auto stage_done = [&]() {
         ++ckt;
         if constexpr (MULTI) {
             if (ckt == nk) {   // the last stage of a modulus before the last one
                 flush(ct);
-                clear();
+                if constexpr (!ZSTART) clear();
                 ckt = 0;
                 ++ct;
-                return;
+                return true;
             }
         }
         if constexpr (PERSIST) {
             if (ckt == nk) {   // the last stage of an item before the last one
                 flush(ft);
-                clear();
+                if constexpr (!ZSTART) clear();
                 ckt = 0;
                 const int8_t *a, *b;
                 walk.item(++cq, kp, sA, sB, Ab, Bb, a, b, fi0, fj0, ft);
-                return;
+                return true;
             }
         }
         if (FOLD && ckt % FOLD_STAGES == 0) fold(PERSIST ? ft : ct);
+        return false;
     };
-    clear();
+    using ZERO = std::true_type;
+    using ACC = std::false_type;
+    bool fresh = true;   // the next stage starts a product
+    if constexpr (!ZSTART) clear();
 
     const int64_t npro = S < NST ? S : NST;
     for (int64_t s = 0; s < npro; ++s) issue(s);
@@ -885,34 +900,46 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
     // Steady state.  Step 0 of stage kt runs while step 1's fragments are read; the barrier
     // before step 1 frees slot kt (its last fragments are in registers), which is refilled
     // with stage kt + NST while step 1 runs and stage kt + 1's first fragments are read.
-    for (; kt + NST < S; ++kt) {
+    auto steady = [&](auto Z) {
         frag(kt, 1, 1);
-        mma(0);
+        mma(0, Z);
         interleave(false);
         stage_barrier(NST - 2);
         issue(kt + NST);
         frag(kt + 1, 0, 0);
-        mma(1);
+        mma(1, Z);
         interleave(true);
-        stage_done();
+    };
+    for (; kt + NST < S; ++kt) {
+        if (ZSTART && fresh) steady(ZERO{});
+        else steady(ACC{});
+        fresh = stage_done();
     }
     // Drain: the last NST - 1 stage transitions issue no refills.
-    for (; kt + 1 < S; ++kt) {
+    auto drain = [&](auto Z) {
         frag(kt, 1, 1);
-        mma(0);
+        mma(0, Z);
         interleave(false);
         const int64_t later = S - kt - 2;
         stage_barrier((int)(later < NST - 2 ? later : NST - 2));
         frag(kt + 1, 0, 0);
-        mma(1);
+        mma(1, Z);
         interleave(false);
-        stage_done();
+    };
+    for (; kt + 1 < S; ++kt) {
+        if (ZSTART && fresh) drain(ZERO{});
+        else drain(ACC{});
+        fresh = stage_done();
     }
-    frag(kt, 1, 1);
-    mma(0);
-    interleave(false);
-    wait_lds();
-    mma(1);
+    auto last = [&](auto Z) {
+        frag(kt, 1, 1);
+        mma(0, Z);
+        interleave(false);
+        wait_lds();
+        mma(1, Z);
+    };
+    if (ZSTART && fresh) last(ZERO{});
+    else last(ACC{});
     flush(MULTI ? nmod - 1 : (PERSIST ? ft : 0));
     __syncthreads();   // the ring may be refilled by the caller's next tile
 }
