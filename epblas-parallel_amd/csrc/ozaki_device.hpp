@@ -690,29 +690,28 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
         }
     };
     v4i af[2][TM], bf[2][TN];
-    // M16 fragments: a16 = A of the stage being multiplied, an16 = A of the next stage (read
-    // during step 1), bl / bh = the first / last TN B fragments of 16 columns.
-    v4i a16[2 * TM], an16[2 * TM], bl[TN], bh[TN];
+    // M16 fragments: a16 / bh = A and the last TN B fragments (16 columns each) of the stage
+    // being multiplied, an16 / bl / bhn = A, the first and the last TN B fragments of the
+    // next stage (bl is read only after step 0 has used it).
+    v4i a16[2 * TM], an16[2 * TM], bl[TN], bh[TN], bhn[TN];
     // A 16-row fragment of a 64-deep stage: lane l holds row r0 + (l & 15), chunk l >> 4.
     auto frag16 = [&](const int8_t *base, int r0) {
         const int r = r0 + (lane & 15), c = lane >> 4;
         return *reinterpret_cast<const v4i *>(base + r * 64 + ((c ^ swz(r)) << 4));
     };
-    // Fragments of 32-deep step ks (0 or 1) of stage s into fragment buffer d.  M16: ks = 1
-    // reads stage s's last TN B fragments; ks = 0 its A fragments and first TN B fragments.
+    // Fragments of 32-deep step ks (0 or 1) of stage s into fragment buffer d.  M16: all of
+    // stage s's fragments (an16, bl, bhn), whatever ks and d.
     auto frag = [&](int64_t s, int ks, int d) {
         const int8_t *sl = smem + (int)((uint32_t)s % NST) * SLOT;
         if constexpr (M16) {
+            (void)ks;
             (void)d;
-            if (ks == 1) {
 #pragma unroll
-                for (int b = 0; b < TN; ++b) bh[b] = frag16(sl + BT * BK, wn * 32 * TN + (TN + b) * 16);
-            } else {
+            for (int a = 0; a < 2 * TM; ++a) an16[a] = frag16(sl, wm * 32 * TM + a * 16);
 #pragma unroll
-                for (int a = 0; a < 2 * TM; ++a) an16[a] = frag16(sl, wm * 32 * TM + a * 16);
+            for (int b = 0; b < TN; ++b) bl[b] = frag16(sl + BT * BK, wn * 32 * TN + b * 16);
 #pragma unroll
-                for (int b = 0; b < TN; ++b) bl[b] = frag16(sl + BT * BK, wn * 32 * TN + b * 16);
-            }
+            for (int b = 0; b < TN; ++b) bhn[b] = frag16(sl + BT * BK, wn * 32 * TN + (TN + b) * 16);
             return;
         }
         const int c = 2 * ks + (lane >> 5);
@@ -741,17 +740,19 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
     };
     // Z: the first stage of a product, whose MFMAs start from zero instead of the accumulators
     // (M16: both steps, which write disjoint accumulators; else step 0), so the accumulators
-    // need no clearing after a flush.  Only the grouped products (MULTI, one product per
-    // modulus of a short depth) start that way: for the persistent walk, whose items are 128
-    // stages deep, the second copy of the stage loop cost 11% of the 8192^3 residue GEMMs,
-    // against the 0.8% that the clearing takes.
-    constexpr bool ZSTART = !PERSIST;
+    // need no clearing after a flush.  Only the 32 x 32 x 32 tiles (the grouped products,
+    // one product per modulus of a short depth) start that way: for the persistent walk,
+    // whose items are 128 stages deep, the second copy of the stage loop cost 11% of the
+    // 8192^3 residue GEMMs, against the 0.8% that the clearing takes.
+    constexpr bool ZSTART = !PERSIST && !M16;
     auto mma = [&](int d, auto Z) {
         constexpr bool z = decltype(Z)::value;
         if constexpr (M16) {
             if (d == 0) {
 #pragma unroll
                 for (int a = 0; a < 2 * TM; ++a) a16[a] = an16[a];
+#pragma unroll
+                for (int b = 0; b < TN; ++b) bh[b] = bhn[b];
             }
             // Consecutive MFMAs share their B operand (measured 0.3-0.5% faster than sharing A).
 #pragma unroll
@@ -771,12 +772,10 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
                         af[d][a], bf[d][b], (z && d == 0) ? AccT{} : acc[a][b], 0, 0, 0);
         }
     };
-    // Scheduling groups: each MFMA followed by one DMA (when refilling) and one fragment read.
-    // The fragment reads interleaved with a step are those issued just before it; for M16,
-    // step 0 (no refill) the last B half, step 1 the next stage's A and first B half.
-    auto interleave = [&](bool with_dma) {
+    // Scheduling groups: each MFMA followed by one DMA (when refilling) and one of the step's
+    // nr fragment reads.
+    auto interleave = [&](bool with_dma, int nr) {
         constexpr int NMS = M16 ? 2 * TM * TN : NM;
-        const int nr = M16 ? (with_dma ? 2 * TM + TN : TN) : NR;
 #pragma unroll
         for (int i = 0; i < NMS; ++i) {
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
@@ -895,51 +894,85 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
     const int64_t npro = S < NST ? S : NST;
     for (int64_t s = 0; s < npro; ++s) issue(s);
     stage_barrier((int)(npro - 1));
-    frag(0, 0, 0);
     int64_t kt = 0;
-    // Steady state.  Step 0 of stage kt runs while step 1's fragments are read; the barrier
-    // before step 1 frees slot kt (its last fragments are in registers), which is refilled
-    // with stage kt + NST while step 1 runs and stage kt + 1's first fragments are read.
-    auto steady = [&](auto Z) {
-        frag(kt, 1, 1);
-        mma(0, Z);
-        interleave(false);
-        stage_barrier(NST - 2);
-        issue(kt + NST);
-        frag(kt + 1, 0, 0);
-        mma(1, Z);
-        interleave(true);
-    };
-    for (; kt + NST < S; ++kt) {
-        if (ZSTART && fresh) steady(ZERO{});
-        else steady(ACC{});
-        fresh = stage_done();
+    if constexpr (M16) {
+        // All of stage kt + 1's fragments are read during step 1 of stage kt, after the
+        // barrier that published it, into buffers that step 1 does not read (the last B half
+        // has its own: bhn, copied to bh at the next step 0).  So the reads issue early among
+        // step 1's MFMAs, and the next barrier, two MFMAs into step 0, finds them complete.
+        // Reading the last B half at the start of step 0 instead left every wave waiting out
+        // the LDS latency in front of that barrier (the kept form measured 1.7-2.2% faster on
+        // two boxes).
+        frag(0, 0, 0);
+        for (; kt + NST < S; ++kt) {
+            mma(0, ACC{});
+            interleave(false, 0);
+            stage_barrier(NST - 2);
+            issue(kt + NST);
+            frag(kt + 1, 0, 0);
+            mma(1, ACC{});
+            interleave(true, 2 * TM + 2 * TN);
+            stage_done();
+        }
+        for (; kt + 1 < S; ++kt) {
+            mma(0, ACC{});
+            interleave(false, 0);
+            const int64_t later = S - kt - 2;
+            stage_barrier((int)(later < NST - 2 ? later : NST - 2));
+            frag(kt + 1, 0, 0);
+            mma(1, ACC{});
+            interleave(false, 2 * TM + 2 * TN);
+            stage_done();
+        }
+        mma(0, ACC{});
+        interleave(false, 0);
+        mma(1, ACC{});
+    } else {
+        frag(0, 0, 0);
+        // Steady state.  Step 0 of stage kt runs while step 1's fragments are read; the barrier
+        // before step 1 frees slot kt (its last fragments are in registers), which is refilled
+        // with stage kt + NST while step 1 runs and stage kt + 1's first fragments are read.
+        auto steady = [&](auto Z) {
+            frag(kt, 1, 1);
+            mma(0, Z);
+            interleave(false, NR);
+            stage_barrier(NST - 2);
+            issue(kt + NST);
+            frag(kt + 1, 0, 0);
+            mma(1, Z);
+            interleave(true, NR);
+        };
+        for (; kt + NST < S; ++kt) {
+            if (ZSTART && fresh) steady(ZERO{});
+            else steady(ACC{});
+            fresh = stage_done();
+        }
+        // Drain: the last NST - 1 stage transitions issue no refills.
+        auto drain = [&](auto Z) {
+            frag(kt, 1, 1);
+            mma(0, Z);
+            interleave(false, NR);
+            const int64_t later = S - kt - 2;
+            stage_barrier((int)(later < NST - 2 ? later : NST - 2));
+            frag(kt + 1, 0, 0);
+            mma(1, Z);
+            interleave(false, NR);
+        };
+        for (; kt + 1 < S; ++kt) {
+            if (ZSTART && fresh) drain(ZERO{});
+            else drain(ACC{});
+            fresh = stage_done();
+        }
+        auto last = [&](auto Z) {
+            frag(kt, 1, 1);
+            mma(0, Z);
+            interleave(false, NR);
+            wait_lds();
+            mma(1, Z);
+        };
+        if (ZSTART && fresh) last(ZERO{});
+        else last(ACC{});
     }
-    // Drain: the last NST - 1 stage transitions issue no refills.
-    auto drain = [&](auto Z) {
-        frag(kt, 1, 1);
-        mma(0, Z);
-        interleave(false);
-        const int64_t later = S - kt - 2;
-        stage_barrier((int)(later < NST - 2 ? later : NST - 2));
-        frag(kt + 1, 0, 0);
-        mma(1, Z);
-        interleave(false);
-    };
-    for (; kt + 1 < S; ++kt) {
-        if (ZSTART && fresh) drain(ZERO{});
-        else drain(ACC{});
-        fresh = stage_done();
-    }
-    auto last = [&](auto Z) {
-        frag(kt, 1, 1);
-        mma(0, Z);
-        interleave(false);
-        wait_lds();
-        mma(1, Z);
-    };
-    if (ZSTART && fresh) last(ZERO{});
-    else last(ACC{});
     flush(MULTI ? nmod - 1 : (PERSIST ? ft : 0));
     __syncthreads();   // the ring may be refilled by the caller's next tile
 }
