@@ -902,8 +902,61 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
         // step 1's MFMAs, and the next barrier, two MFMAs into step 0, finds them complete.
         // Reading the last B half at the start of step 0 instead left every wave waiting out
         // the LDS latency in front of that barrier (the kept form measured 1.7-2.2% faster on
-        // two boxes).
+        // three boxes).
         frag(0, 0, 0);
+        // Stage pairs in ping-pong buffers: stage kt's A and last-B fragments in pa[kt & 1],
+        // ph[kt & 1], so no fragment is copied between stages (the single-buffer form below
+        // moves 16 register pairs per stage; the pairs measured 1.6-3.0% faster on two
+        // boxes).  The remaining stages (refills left over from the pairs, then the drain)
+        // take the single-buffer form, handed over through an16 / bhn.
+        {
+            using I0 = std::integral_constant<int, 0>;
+            using I1 = std::integral_constant<int, 1>;
+            v4i pa[2][2 * TM], ph[2][TN];
+#pragma unroll
+            for (int a = 0; a < 2 * TM; ++a) pa[0][a] = an16[a];
+#pragma unroll
+            for (int b = 0; b < TN; ++b) ph[0][b] = bhn[b];
+            auto pmma = [&](auto P, int d) {
+                constexpr int p = decltype(P)::value;
+#pragma unroll
+                for (int b = 0; b < TN; ++b)
+#pragma unroll
+                    for (int a = 0; a < 2 * TM; ++a) {
+                        AccT &c = acc[a][d * TN + b];
+                        c = __builtin_amdgcn_mfma_i32_16x16x64_i8(pa[p][a], d ? ph[p][b] : bl[b], c, 0, 0, 0);
+                    }
+            };
+            auto pfrag = [&](auto P, int64_t st) {
+                constexpr int p = decltype(P)::value;
+                const int8_t *sl = smem + (int)((uint32_t)st % NST) * SLOT;
+#pragma unroll
+                for (int a = 0; a < 2 * TM; ++a) pa[p][a] = frag16(sl, wm * 32 * TM + a * 16);
+#pragma unroll
+                for (int b = 0; b < TN; ++b) bl[b] = frag16(sl + BT * BK, wn * 32 * TN + b * 16);
+#pragma unroll
+                for (int b = 0; b < TN; ++b) ph[p][b] = frag16(sl + BT * BK, wn * 32 * TN + (TN + b) * 16);
+            };
+            auto pstage = [&](auto P, int64_t k) {
+                constexpr int p = decltype(P)::value;
+                pmma(P, 0);
+                interleave(false, 0);
+                stage_barrier(NST - 2);
+                issue(k + NST);
+                pfrag(std::integral_constant<int, 1 - p>{}, k + 1);
+                pmma(P, 1);
+                interleave(true, 2 * TM + 2 * TN);
+                stage_done();
+            };
+            for (; kt + 1 + NST < S; kt += 2) {
+                pstage(I0{}, kt);
+                pstage(I1{}, kt + 1);
+            }
+#pragma unroll
+            for (int a = 0; a < 2 * TM; ++a) an16[a] = pa[0][a];
+#pragma unroll
+            for (int b = 0; b < TN; ++b) bhn[b] = ph[0][b];
+        }
         for (; kt + NST < S; ++kt) {
             mma(0, ACC{});
             interleave(false, 0);
