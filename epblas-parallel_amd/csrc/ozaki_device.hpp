@@ -635,7 +635,9 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
     constexpr int NR = TM + TN, NM = TM * TN;   // fragment reads and MFMAs per 32-deep step
     static_assert(!M16 || std::is_same<std::decay_t<Epi>, NoRingEpilogue>::value,
                   "the epilogue receives 32 x 32 accumulator tiles");
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    // w is uniform across the wave: taken through readfirstlane, the LDS addresses of the DMA
+    // (one per wave and instruction) are scalar arithmetic.
+    const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = w % WM, wn = w / WM;
     const int64_t nk = kp / BK;
     int nitems = 1;
@@ -658,20 +660,29 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
     int64_t fi0 = i0, fj0 = j0;
     int ft = 0, dq = 0, cq = 0;
     if constexpr (PERSIST) walk.item(0, kp, sA, sB, Ab, Bb, ca, cb, fi0, fj0, ft);
+    // Per-lane parts of the DMA source addresses (row and chunk within the panel), fixed for
+    // the whole sequence; a stage adds the uniform panel base and K offset to them.
+    int64_t loff_a[DQ], loff_b[DQ];
+#pragma unroll
+    for (int q = 0; q < DQ; ++q) {
+        const int r = (DQ * w + q) * 16 + drow;
+        const int c = dslot ^ swz(r);
+        const int ra = CLAMP && r >= rows_a ? rows_a - 1 : r;
+        const int rb = CLAMP && r >= rows_b ? rows_b - 1 : r;
+        loff_a[q] = (int64_t)ra * kp + c * 16;
+        loff_b[q] = (int64_t)rb * kp + c * 16;
+    }
     auto issue = [&](int64_t s) {
         int8_t *slot = smem + (int)((uint32_t)s % NST) * SLOT;
+        const int8_t *sa = ca + ck * BK, *sb = cb + ck * BK;
 #pragma unroll
         for (int q = 0; q < DQ; ++q) {
             const int blk = DQ * w + q;
-            const int r = blk * 16 + drow;
-            const int c = dslot ^ swz(r);
-            const int ra = CLAMP && r >= rows_a ? rows_a - 1 : r;
-            const int rb = CLAMP && r >= rows_b ? rows_b - 1 : r;
             __builtin_amdgcn_global_load_lds(
-                (const __attribute__((address_space(1))) void *)(ca + (int64_t)ra * kp + ck * BK + c * 16),
+                (const __attribute__((address_space(1))) void *)(sa + loff_a[q]),
                 (__attribute__((address_space(3))) void *)(slot + blk * 1024), 16, 0, 0);
             __builtin_amdgcn_global_load_lds(
-                (const __attribute__((address_space(1))) void *)(cb + (int64_t)rb * kp + ck * BK + c * 16),
+                (const __attribute__((address_space(1))) void *)(sb + loff_b[q]),
                 (__attribute__((address_space(3))) void *)(slot + BT * BK + blk * 1024), 16, 0, 0);
         }
         if (++ck == nk) {
