@@ -913,12 +913,12 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
         // step 1's MFMAs, and the next barrier, two MFMAs into step 0, finds them complete.
         // Reading the last B half at the start of step 0 instead left every wave waiting out
         // the LDS latency in front of that barrier (the kept form measured 1.7-2.2% faster on
-        // three boxes).
+        // three MI355X GPUs).
         frag(0, 0, 0);
         // Stage pairs in ping-pong buffers: stage kt's A and last-B fragments in pa[kt & 1],
         // ph[kt & 1], so no fragment is copied between stages (the single-buffer form below
         // moves 16 register pairs per stage; the pairs measured 1.6-3.0% faster on two
-        // boxes).  The remaining stages (refills left over from the pairs, then the drain)
+        // MI355X GPUs).  The remaining stages (refills left over from the pairs, then the drain)
         // take the single-buffer form, handed over through an16 / bhn.
         {
             using I0 = std::integral_constant<int, 0>;
