@@ -34,8 +34,11 @@ def _assert_symmetric_residues(got, expect, m):
 @pytest.mark.parametrize("mod_index,variant,kp", [
     (0, 1, 320), (1, 1, 320), (5, 1, 320), (28, 1, 320), (39, 1, 320),
     (0, 2, 320), (28, 3, 192),
-    # the DMA-ring kernel: fewer, as many and more 64-deep stages than its ring holds (4)
-    (0, 0, 64), (5, 0, 128), (28, 0, 192), (39, 0, 256), (1, 0, 320), (0, 0, 1088)])
+    # the DMA-ring kernel: fewer, as many and more 64-deep stages than its ring holds (4);
+    # 6, 7, 10 and 17 stages hand over from the stage pairs to the single-buffer stages
+    # after an even or an odd number of refills
+    (0, 0, 64), (5, 0, 128), (28, 0, 192), (39, 0, 256), (1, 0, 320), (3, 0, 384),
+    (7, 0, 448), (11, 0, 640), (0, 0, 1088)])
 def test_residue_gemm_kernel_is_exact(gpu, mod_index, variant, kp):
     import torch
     rng = np.random.default_rng(100 + mod_index + 7 * variant + kp)
