@@ -271,12 +271,18 @@ void k_batch_oz_gemm(const BatchProb *__restrict__ probs, const int4 *__restrict
 // The same queue through the DMA-fed LDS ring (128 x 128 tiles, four stages of 64-deep
 // slabs in flight): with the short K of small problems a register-staged tile waits on
 // memory at every slab, the ring keeps three slabs in flight across modulus boundaries.
-__global__ __launch_bounds__(256, 2)
+// Three stages of 16 KB and three workgroups per CU (12 VGPRs of each lane spilled under the
+// 168-register limit that three workgroups impose): the kernel waits on its DMA, and a third
+// workgroup with 9 stages in flight per CU took the residue GEMMs from 1.35-1.39 to 1.18-1.21
+// ms on the 2000-problem workload (four stages and two workgroups per CU before; two stages
+// and four workgroups spilled 215 registers and took 8 ms).
+constexpr int BRING_NST = 3, BRING_WGS = 3;
+__global__ __launch_bounds__(256, BRING_WGS)
 void k_batch_oz_ring(const BatchProb *__restrict__ probs, const int4 *__restrict__ tiles,
                      const int *__restrict__ qstart, int *__restrict__ counters,
                      const ModConst *__restrict__ mcs, int nmod, int group)
 {
-    __shared__ __attribute__((aligned(1024))) int8_t smem[RING_NST * ring_slot<128>()];
+    __shared__ __attribute__((aligned(1024))) int8_t smem[BRING_NST * ring_slot<128>()];
     __shared__ int s_tile;
     int visited = 0;
     for (;;) {
@@ -293,10 +299,9 @@ void k_batch_oz_ring(const BatchProb *__restrict__ probs, const int4 *__restrict
         // residue rows exist up to Mp / Np (multiples of 32), possibly short of the tile
         const int ra = (int)(p.Mp - i0 < 128 ? p.Mp - i0 : 128);
         const int rb = (int)(p.Np - j0 < 128 ? p.Np - j0 : 128);
-        oz_gemm_tile_ring<2, 2, 2, 2, true, true>(p.resA + tr.y * sA + i0 * p.Kc,
-                                                  p.resB + tr.y * sB + j0 * p.Kc, p.U + tr.y * sU,
-                                                  p.Kc, p.Mp, mcs + tr.y, nm, sA, sB, sU, i0, j0,
-                                                  smem, ra, rb);
+        oz_gemm_tile_ring<2, 2, 2, 2, true, true, true, false, NoRingEpilogue, NoRingWalk, BRING_NST>(
+            p.resA + tr.y * sA + i0 * p.Kc, p.resB + tr.y * sB + j0 * p.Kc, p.U + tr.y * sU, p.Kc,
+            p.Mp, mcs + tr.y, nm, sA, sB, sU, i0, j0, smem, ra, rb);
     }
 }
 
@@ -437,7 +442,8 @@ struct BatchPlan {
     int64_t n_ra = 0, n_rb = 0, n_sa = 0, n_sb = 0, n_crt = 0;
     unsigned long long *rmax_base = nullptr;
     int64_t rmax_count = 0;
-    int grid_persist = 0;
+    int grid_persist = 0;   // persistent workgroups: two per CU
+    int grid_ring = 0;      // k_batch_oz_ring: BRING_WGS per CU
     hipEvent_t done = nullptr;          // completion of the last run, and its stream: a run on
     hipStream_t done_stream = nullptr;  // another stream waits for it (the runs share the
     bool done_recorded = false;         // plan's work counters and workspace)
@@ -517,7 +523,7 @@ static int batch_enqueue(BatchPlan *P, hipStream_t s)
                                P->probs, P->tiles, P->qstart, P->counters, P->mcs, P->oz.nmod,
                                P->oz.Mhi, P->oz.Mlo, P->oz.bits);
         else if (P->oz_tile == 128)
-            hipLaunchKernelGGL(k_batch_oz_ring, dim3((unsigned)P->grid_persist), dim3(256), 0, s,
+            hipLaunchKernelGGL(k_batch_oz_ring, dim3((unsigned)P->grid_ring), dim3(256), 0, s,
                                P->probs, P->tiles, P->qstart, P->counters, P->mcs, P->oz.nmod,
                                P->mod_group);
         else
@@ -743,6 +749,7 @@ static int plan_build(BatchPlan **out, int64_t count, char transa, char transb, 
     P->n_ra = ra[count]; P->n_rb = rb[count]; P->n_sa = sa[count]; P->n_sb = sb[count]; P->n_crt = cr[count];
     P->rmax_count = rmax_n;
     P->grid_persist = std::max(1, std::min<int>(P->ntiles, cus * 2));
+    P->grid_ring = std::max(1, std::min<int>(P->ntiles, cus * BRING_WGS));
 
     // One device allocation: [probs | tiles | counters | 5 prefix arrays | rmax | workspace]
     size_t off = 0;

@@ -618,7 +618,8 @@ struct RingWalk {
     }
 };
 template <int WM, int WN, int TM, int TN, bool MULTI, bool CLAMP = false, bool FOLD = true,
-          bool M16 = false, typename Epi = NoRingEpilogue, typename Walk = NoRingWalk>
+          bool M16 = false, typename Epi = NoRingEpilogue, typename Walk = NoRingWalk,
+          int NSTT = RING_NST>
 DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restrict__ Bb,
                            uint8_t *__restrict__ Ub, int64_t kp, int64_t ldu,
                            const ModConst *__restrict__ mcs, int nmod, int64_t sA, int64_t sB,
@@ -629,7 +630,7 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
     static_assert(!PERSIST || (!MULTI && !CLAMP && 32 * TM * WM == 256), "persistent walk");
     constexpr int BT = 32 * TM * WM, NW = WM * WN;
     static_assert(BT == 32 * TN * WN, "square tile");
-    constexpr int NST = RING_NST, BK = RING_BK, SLOT = ring_slot<BT>();
+    constexpr int NST = NSTT, BK = RING_BK, SLOT = ring_slot<BT>();
     constexpr int DQ = BT / 16 / NW;            // DMA instructions per operand, wave and stage
     static_assert(DQ * NW * 16 == BT, "DMA rows");
     constexpr int NR = TM + TN, NM = TM * TN;   // fragment reads and MFMAs per 32-deep step
