@@ -377,7 +377,10 @@ void k_batch_oz_fused(const BatchProb *__restrict__ probs, const int4 *__restric
 // Reconstruction: 256 units (one workgroup) per 32 x 32 block of C (oz_crt_block), blocks
 // column block by column block; every problem's unit count is a multiple of 256, so a
 // workgroup never spans two problems.  Inactive problems get the beta pre-pass only.
-__global__ __launch_bounds__(256)
+// At least five workgroups per CU (at most 102 VGPRs; it needs 80 then): the reconstruction
+// waits on its loads, and 5-6 workgroups per CU took it from 0.85 to 0.68-0.69 ms on the
+// 2000-problem workload (four, at 107 VGPRs, before).
+__global__ __launch_bounds__(256, 5)
 void k_batch_crt(const BatchProb *__restrict__ probs, UnitSpace us, int nmod,
                  const ModConst *__restrict__ mcs, double Mhi, double Mlo, int bits)
 {
