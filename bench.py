@@ -198,7 +198,7 @@ def gemm_512_bench(dev, reps=20, seed=512):
     return out, host
 
 
-def batched_bench(dev, reps=5, count=2000, seed=2000, checks=None, check_every=125):
+def batched_bench(dev, reps=20, count=2000, seed=2000, checks=None, check_every=125):
     """The batched workload: count real DD GEMMs, M,N,K log-uniform integers in [16,512],
     NN, alpha=beta=1, uniform entries with non-zero low limbs; one captured plan per path,
     replayed reps times.  Effective DD-GFLOP/s = sum(2MNK) / t.  For the checker, every
@@ -242,7 +242,8 @@ def batched_bench(dev, reps=5, count=2000, seed=2000, checks=None, check_every=1
     for name, path in (("plain", mwblas.PATH_PLAIN), ("ozaki", mwblas.PATH_OZAKI),
                        ("auto", mwblas.PATH_AUTO)):
         plan = mwblas.BatchedPlan("N", "N", ms, ns, ks, one, A, ms, B, ks, one, C, ms, path=path)
-        plan.run(stream)
+        for _ in range(3):                  # warm-up replays (clock and caches settle)
+            plan.run(stream)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
