@@ -838,8 +838,10 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
                     if (CLAMP && (rl >= rows_a || cl >= rows_b)) continue;
                     const uint32_t word = residue_word(acc[a][b][0], acc[a][b][1], acc[a][b][2],
                                                        acc[a][b][3], mc, neg_m);
-                    *reinterpret_cast<uint32_t *>(U + (b >> 1) * cstride + (a >> 1) * 1024 +
-                                                  (a & 1) * 512 + (b & 1) * 64) = word;
+                    // non-temporal: U is read only after the launch (by the reconstruction),
+                    // so its lines leave L2 to the residue panels
+                    __builtin_nontemporal_store(word, reinterpret_cast<uint32_t *>(
+                        U + (b >> 1) * cstride + (a >> 1) * 1024 + (a & 1) * 512 + (b & 1) * 64));
                 }
         } else {
 #pragma unroll
@@ -854,7 +856,8 @@ DDI void oz_gemm_tile_ring(const int8_t *__restrict__ Ab, const int8_t *__restri
                         if (CLAMP && (rl + 8 * g >= rows_a || cl >= rows_b)) continue;
                         const uint32_t word = residue_word(acc[a][b][4 * g], acc[a][b][4 * g + 1],
                                                            acc[a][b][4 * g + 2], acc[a][b][4 * g + 3], mc, neg_m);
-                        *reinterpret_cast<uint32_t *>(U + b * cstride + a * 1024 + g * 256) = word;
+                        __builtin_nontemporal_store(
+                            word, reinterpret_cast<uint32_t *>(U + b * cstride + a * 1024 + g * 256));
                     }
                 }
         }
